@@ -1,0 +1,119 @@
+/*
+ * semtex_amd.h -- C ABI of the MI355X (gfx950) per-Fourier-mode Helmholtz
+ * solver that stands in for semtex's DIRECT solve path.
+ *
+ * All entry points are extern "C", take plain pointers and sizes, and return
+ * an int status (SX_OK = 0, negative on error; sx_last_error() gives the
+ * message).  Device pointers are HIP device addresses in HBM; `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Storage conventions are the reference's:
+ *   - a Fourier field is [nz][plane_stride] doubles, plane k holding mode
+ *     k>>1 (real part for even k, imaginary for odd k; plane 1 of mode 0 is
+ *     the Nyquist plane and is never solved) -- src/field.cpp:648-655;
+ *   - within a plane, element e occupies [e*np*np, (e+1)*np*np), row-major
+ *     np x np (src/element.h, src/auxfield.cpp:1062);
+ *   - btog/bmask are [nel][4(np-1)] in emap order, essential nodes numbered
+ *     last (src/numbersys.h);
+ *   - boundary line data are [nz][nbound*np]: segment b's np values in CCW
+ *     traverse order (src/field.cpp:1078-1102, src/condition.cpp:80).
+ */
+#ifndef SEMTEX_AMD_H
+#define SEMTEX_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SX_OK            0
+#define SX_ERR_ARG      -1   /* bad argument / size                        */
+#define SX_ERR_HIP      -2   /* HIP runtime error                          */
+#define SX_ERR_GEOM     -3   /* non-positive element Jacobian              */
+#define SX_ERR_FACTOR   -4   /* Cholesky breakdown (matrix not SPD)        */
+#define SX_ERR_LIMIT    -5   /* problem exceeds an implemented size limit  */
+
+#define SX_ESSENTIAL 0       /* boundary kinds (src/condition.h)           */
+#define SX_NATURAL   1
+
+#define SX_FP64  0           /* solver precisions                          */
+#define SX_MIXED 1           /* FP32 elemental applies + FP64 refinement   */
+
+typedef struct sx_mesh*  sx_mesh_t;   /* geometry + numbering, on the GPU  */
+typedef struct sx_msys*  sx_msys_t;   /* factorised per-mode systems       */
+
+/* Last error message of this thread ("" if none). */
+const char* sx_last_error (void);
+
+/* Fills `buf` with the device name / arch; returns SX_OK when a gfx950
+ * device is present. */
+int sx_device_info (char* buf, int len);
+
+/* GLL points, weights, derivative matrix D[i*np+j] = h_j'(z_i) as the
+ * library computes them (femlib/polylib.c:312 zwglj, :513 Dglj).  Host
+ * arrays, no GPU needed. */
+int sx_gll (int np, double* z, double* w, double* D);
+
+/* Geometry, numbering and boundary system of one 2D mesh, replacing the
+ * Element vector + NumberSys + BoundarySys that ModalMatrixSys receives
+ * (src/matrix.h:61-66; Element::mapping src/element.cpp:1253-1408;
+ * NumberSys src/numbersys.h:4-80; BoundarySys src/bsys.cpp:42-157).
+ * x, y: [nel][np*np] node coordinates (Element::_xmesh/_ymesh).
+ * cyl: 1 for cylindrical (y = r) coordinates.  All arrays are host. */
+int sx_mesh_create (int np, int nel, int cyl,
+                    const double* x, const double* y,
+                    const int* btog, const int* bmask, const int* emask,
+                    int nglobal, int nsolve, int nband,
+                    int nbound, const int* bnd_elmt, const int* bnd_side,
+                    const int* bnd_kind, sx_mesh_t* out);
+int sx_mesh_destroy (sx_mesh_t m);
+/* Number of distinct element geometries found (Family sharing,
+ * src/family.cpp). */
+int sx_mesh_nclass (sx_mesh_t m);
+
+/* ModalMatrixSys::ModalMatrixSys (src/matrix.cpp:33-105) with method
+ * DIRECT: builds, statically condenses (Element::HelmholtzSC
+ * src/element.cpp:131-214) and Cholesky-factors (MatrixSys::MatrixSys
+ * src/matrix.cpp:185-267) the system of every mode base_mode ..
+ * base_mode+num_modes-1, with betak2 = (beta*mode)^2
+ * (Field::modeConstant, Cartesian), all on the GPU.  A mode whose
+ * lambda2+betak2 vanishes with no essential nodes is singular and drops its
+ * last unknown (src/matrix.cpp:154-162).  precision: SX_FP64 or SX_MIXED. */
+int sx_msys_create (sx_mesh_t m, double lambda2, double beta, int base_mode,
+                    int num_modes, int precision, void* stream,
+                    sx_msys_t* out);
+int sx_msys_destroy (sx_msys_t s);
+/* Unknowns solved for mode index `mode` (0-based within the system). */
+int sx_msys_nsolve (sx_msys_t s, int mode);
+
+/* Field::solve (src/field.cpp:598-709, DIRECT branch) over nz planes.
+ * d_f:     forcing planes [nz][plane_stride] (device; DESTROYED, as in the
+ *          reference, src/field.cpp:611).
+ * d_u:     output planes [nz][plane_stride] (device).
+ * d_lines: boundary values [nz][nbound*np] (device) or NULL for all zero.
+ * Planes k >= 2*num_modes are rejected; the Nyquist plane 1 of base mode 0
+ * is left untouched.  For SX_MIXED systems, refine_tol is the relative
+ * residual target of the FP64 refinement (<= 0: 1e-12) and max_iter its
+ * iteration cap; *iters (may be NULL) returns the iterations used. */
+int sx_field_solve (sx_msys_t s, int nz, int plane_stride, double* d_f,
+                    double* d_u, const double* d_lines, void* stream);
+int sx_field_solve_mixed (sx_msys_t s, int nz, int plane_stride, double* d_f,
+                          double* d_u, const double* d_lines,
+                          double refine_tol, int max_iter, int* iters,
+                          void* stream);
+
+/* Host-pointer convenience wrapper for drop-in callers whose Field data
+ * lives in host memory (Domain::u, src/domain.cpp:75-79): copies in,
+ * solves, copies u back (f is not copied back). */
+int sx_field_solve_host (sx_msys_t s, int nz, int plane_stride,
+                         const double* f, double* u, const double* lines);
+
+/* Elemental Helmholtz operator applied plane by plane, u -> H u for every
+ * element (Element::HelmholtzOp src/element.cpp:1593-1622), mode-dependent
+ * betak2 as in sx_msys_create.  Device planes [nz][plane_stride]. */
+int sx_helmholtz_op (sx_msys_t s, int nz, int plane_stride, const double* d_u,
+                     double* d_hu, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,63 @@
+"""ctypes binding of libsemtex_amd.so (the C ABI in include/semtex_amd.h).
+
+The product path has no CPU fallback: if the HIP library is missing or
+fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsemtex_amd.so")
+
+SX_OK = 0
+SX_FP64, SX_MIXED = 0, 1
+
+# Every symbol include/semtex_amd.h declares, with (restype, argtypes).
+_vp, _i, _d, _ip, _dp = C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p
+SIGNATURES = {
+    "sx_last_error": (C.c_char_p, []),
+    "sx_device_info": (_i, [C.c_char_p, _i]),
+    "sx_gll": (_i, [_i, _dp, _dp, _dp]),
+    "sx_mesh_create": (_i, [_i, _i, _i, _dp, _dp, _ip, _ip, _ip, _i, _i, _i,
+                            _i, _ip, _ip, _ip, C.POINTER(_vp)]),
+    "sx_mesh_destroy": (_i, [_vp]),
+    "sx_mesh_nclass": (_i, [_vp]),
+    "sx_msys_create": (_i, [_vp, _d, _d, _i, _i, _i, _vp, C.POINTER(_vp)]),
+    "sx_msys_destroy": (_i, [_vp]),
+    "sx_msys_nsolve": (_i, [_vp, _i]),
+    "sx_field_solve": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
+    "sx_field_solve_mixed": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i,
+                                  C.POINTER(_i), _vp]),
+    "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
+    "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
+}
+
+_lib = None
+
+
+class SxError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library; raises if it is absent (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SxError(f"{LIB_PATH} is missing: run `python -m semtex_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc != SX_OK:
+        msg = load().sx_last_error().decode()
+        raise SxError(f"{what} failed ({rc}): {msg}")
+    return rc

@@ -1,0 +1,88 @@
+// sx_internal.h -- shared declarations of the semtex_amd HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/semtex_amd.h"
+
+namespace sx {
+
+// Per-node geometric arrays kept per element on the device, in this order
+// (Element::mapping, src/element.cpp:1253-1408).
+enum { G_Q1 = 0, G_Q2, G_Q3, G_Q4, G_Q8, G_RX, G_RY, G_SX, G_SY, G_Y, NGEO };
+
+void set_error (const std::string& msg);
+int  hip_fail (hipError_t e, const char* what);
+
+#define SX_HIP(call)                                                  \
+  do {                                                                \
+    hipError_t _e = (call);                                           \
+    if (_e != hipSuccess) return sx::hip_fail (_e, #call);            \
+  } while (0)
+
+// GLL rule (femlib/polylib.c:312 zwglj, :513 Dglj), host.
+void gll (int np, double* z, double* w, double* D);
+// edge-major permutation (femlib/mapping.c:46-132), host.
+void edgemaps (int np, int* emap, int* pmap);
+// geometric factors of one element (src/element.cpp:1253-1408), host.
+// geo: NGEO*np*np.  Returns false if the Jacobian is not positive.
+bool element_geometry (int np, int cyl, const double* x, const double* y,
+                       double* geo);
+// side quadrature weights |dx/dr| w of side `side` in CCW order
+// (src/element.cpp:1625-1742 sideGeom).
+void side_area (int np, const double* x, const double* y, int side,
+                double* area);
+
+}  // namespace sx
+
+struct sx_mesh {
+  int np = 0, nel = 0, cyl = 0, npnp = 0, next = 0, nint = 0;
+  int nglobal = 0, nsolve = 0, nband = 0, kd = 0, nbound = 0, nclass = 0;
+  int npairs = 0;
+  std::vector<int> btog, bmask, emask, cls, emap;
+
+  // device arrays
+  double* d_geo = nullptr;     // [nel][NGEO][npnp]
+  double* d_cgeo = nullptr;    // [nclass][NGEO][npnp] class representatives
+  double* d_D = nullptr;       // [np][np]
+  int* d_cls = nullptr;        // [nel]
+  int* d_btog = nullptr;       // [nel][next]
+  int* d_bmask = nullptr;      // [nel][next]
+  int* d_emask = nullptr;      // [nel]
+  int* d_emap = nullptr;       // [npnp]
+  int* d_gptr = nullptr;       // [nglobal+1] CSR: global node -> plane slots
+  int* d_gsrc = nullptr;       //   slot = e*npnp + i (emap-ordered storage)
+  int* d_ess = nullptr;        // [nglobal] line index of essential value / -1
+  int* d_nptr = nullptr;       // [nglobal+1] CSR natural-BC contributions
+  int* d_nidx = nullptr;       //   line index
+  double* d_nw = nullptr;      //   quadrature weight
+  long long* d_ppos = nullptr; // [npairs] band slot of assembled entry
+  int* d_pmn = nullptr;        // [npairs][2] (m, n) global rows
+  int* d_pptr = nullptr;       // [npairs+1] CSR: entry -> condensed-matrix slots
+  int* d_psrc = nullptr;       //   slot = cls*next*next + i*next + k
+};
+
+struct sx_msys {
+  sx_mesh* m = nullptr;
+  int nmodes = 0, base_mode = 0, precision = SX_FP64;
+  double lambda2 = 0.0, beta = 0.0;
+  std::vector<double> betak2;
+  std::vector<int> nsolve;
+  int nmax = 0;                // max nsolve over modes
+  size_t band_stride = 0;      // doubles per mode band
+  double* d_betak2 = nullptr;  // [nmodes]
+  int* d_nsolve = nullptr;     // [nmodes]
+  double* d_Bm = nullptr;      // [nmodes][nclass][next][nint]  hbi*inv(hii)
+  double* d_Hinv = nullptr;    // [nmodes][nclass][nint][nint]  inv(hii)
+  float* d_Bm32 = nullptr;     // FP32 copies for SX_MIXED
+  float* d_Hinv32 = nullptr;
+  double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband
+  double* d_bandR = nullptr;   // [nmodes][band_stride] reversed row copy
+  double* d_rhs = nullptr;     // [nmodes][2][nmax] solve workspace
+  double* d_G = nullptr;       // [2*nmodes][nglobal] essential values
+  double* d_work = nullptr;    // mixed-precision workspace
+  size_t work_doubles = 0;
+};

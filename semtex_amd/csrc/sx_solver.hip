@@ -1,0 +1,805 @@
+// sx_solver.hip -- gfx950 kernels and C ABI of the per-Fourier-mode
+// Helmholtz solver (semtex ModalMatrixSys + Field::solve, DIRECT).
+//
+// Set-up (once per system, not timed):
+//   k_elem_sc       elemental Helmholtz matrix + static condensation per
+//                   (mode, element class)  [Element::HelmholtzSC]
+//   k_band_assemble deterministic direct-stiffness assembly of the band
+//   k_band_factor   FP64 banded Cholesky, one workgroup per mode [dpbtrf]
+//   k_band_reverse  reversed row copy of the factor for the backward sweep
+// Solve (the timed path, per call):
+//   k_essential     essential BC values into global numbering
+//   k_condense      -(M f + H g), emap gather, F_b -= hbi hii^-1 F_i
+//   k_assemble      direct-stiffness sum of F_b (+ natural BC integrals)
+//   k_band_solve    FP64 U^T U x = b, one workgroup per mode, both RHS
+//   k_recover       u_i = hii^-1 F_i - (hbi hii^-1)^T u_b, essential insert
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "sx_internal.h"
+
+using namespace sx;
+
+namespace {
+
+constexpr double EPSDP = 6.0e-14;   // femlib/cfemdef.h
+constexpr double EPSSP = 6.0e-7;
+
+__device__ __forceinline__ double wave_sum (double v)
+{
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor (v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ double readlane_d (double v, int l)
+{
+  const long long x = __double_as_longlong (v);
+  const int lo = __builtin_amdgcn_readlane ((int) (x & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane ((int) (x >> 32), l);
+  return __longlong_as_double (((long long) hi << 32) | (unsigned) lo);
+}
+
+// Map the y-th solved plane to its plane index: plane 1 (Nyquist of mode
+// 0) is skipped when the base mode is 0 and there is more than one plane
+// (src/field.cpp:650).
+__device__ __forceinline__ int solved_plane (int y, int nz, int base_mode)
+{
+  return (base_mode == 0 && nz > 1 && y >= 1) ? y + 1 : y;
+}
+
+// ---------------------------------------------------------------------------
+// Element matrix entry h[r][c] (r, c row-major node indices), following
+// Element::HelmholtzRow (src/element.cpp:1411-1481), SVV off.
+// ---------------------------------------------------------------------------
+__device__ double helm_entry (int np, int r, int c, const double* Q1,
+                              const double* Q2, const double* Q3,
+                              const double* Q4, const double* Y,
+                              const double* D, double lambda2, double betak2,
+                              int cyl, bool hasQ3)
+{
+  const int i = r / np, j = r % np, m = c / np, n = c % np;
+  double h = 0.0;
+  if (i == m) {
+    double s = 0.0;
+    for (int q = 0; q < np; q++) s += Q1[i * np + q] * (D[q * np + j] * D[q * np + n]);
+    h += s;
+  }
+  if (j == n) {
+    double s = 0.0;
+    for (int p = 0; p < np; p++) s += Q2[p * np + j] * (D[p * np + i] * D[p * np + m]);
+    h += s;
+  }
+  if (hasQ3) {
+    h += Q3[i * np + n] * D[n * np + j] * D[i * np + m];
+    h += Q3[m * np + j] * D[j * np + n] * D[m * np + i];
+  }
+  if (r == c) {
+    const double r2 = Y[r] * Y[r];
+    const double hCon = (cyl && r2 > EPSDP) ? betak2 / r2 + lambda2 : betak2 + lambda2;
+    h += Q4[r] * hCon;
+  }
+  return h;
+}
+
+// One workgroup per (mode, class).  ws: per-block workspace of
+// npnp*npnp + nint*nint doubles.  Outputs: S (condensed hbb, next^2),
+// Bm = hbi inv(hii) (next x nint), Hinv = inv(hii) (nint^2).
+__global__ __launch_bounds__(256) void k_elem_sc (
+    int np, int nclass, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ cgeo, const double* __restrict__ Dg,
+    const int* __restrict__ emap_g, double* ws, double* S, double* Bm,
+    double* Hinv, int* status)
+{
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int blk = blockIdx.x, mode = blk / nclass, cls = blk % nclass;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __shared__ double D[256], G[5 * 256];
+  __shared__ int emap[256];
+  __shared__ int s_bad, s_hasQ3;
+  __shared__ double s_piv;
+
+  const double* geo = cgeo + (size_t) cls * NGEO * npnp;
+  for (int k = tid; k < npnp; k += nt) {
+    D[k] = Dg[k];
+    emap[k] = emap_g[k];
+    G[0 * npnp + k] = geo[G_Q1 * npnp + k];
+    G[1 * npnp + k] = geo[G_Q2 * npnp + k];
+    G[2 * npnp + k] = geo[G_Q3 * npnp + k];
+    G[3 * npnp + k] = geo[G_Q4 * npnp + k];
+    G[4 * npnp + k] = geo[G_Y * npnp + k];
+  }
+  if (tid == 0) { s_bad = 0; s_hasQ3 = 0; }
+  __syncthreads ();
+  for (int k = tid; k < npnp; k += nt)
+    if (G[2 * npnp + k] != 0.0) s_hasQ3 = 1;
+  __syncthreads ();
+  const bool hasQ3 = s_hasQ3 != 0;
+  const double bk2 = betak2[mode];
+
+  double* H = ws + (size_t) blk * (npnp * npnp + nint * nint);  // permuted
+  double* Li = H + npnp * npnp;                                  // inv(L)
+  for (int idx = tid; idx < npnp * npnp; idx += nt) {
+    const int a = idx / npnp, b = idx % npnp;
+    H[idx] = helm_entry (np, emap[a], emap[b], G, G + npnp, G + 2 * npnp,
+                         G + 3 * npnp, G + 4 * npnp, D, lambda2, bk2, cyl, hasQ3);
+  }
+  __syncthreads ();
+
+  // Cholesky of the interior block hii (lower, in place)
+#define HII(a, b) H[(size_t) (next + (a)) * npnp + next + (b)]
+  for (int k = 0; k < nint; k++) {
+    if (tid == 0) {
+      const double p = HII (k, k);
+      if (!(p > 0.0)) s_bad = 1;
+      s_piv = std::sqrt (p > 0.0 ? p : 1.0);
+      HII (k, k) = s_piv;
+    }
+    __syncthreads ();
+    const double piv = s_piv;
+    for (int i = k + 1 + tid; i < nint; i += nt) HII (i, k) /= piv;
+    __syncthreads ();
+    const int w = nint - k - 1;
+    for (int idx = tid; idx < w * w; idx += nt) {
+      const int ii = idx / w, jj = idx % w;
+      if (jj <= ii) HII (k + 1 + ii, k + 1 + jj) -= HII (k + 1 + ii, k) * HII (k + 1 + jj, k);
+    }
+    __syncthreads ();
+  }
+  if (s_bad) { if (tid == 0) atomicExch (status, 1); return; }
+
+  // inverse of L, column by column
+  for (int c = tid; c < nint; c += nt) {
+    for (int i = 0; i < c; i++) Li[i * nint + c] = 0.0;
+    Li[c * nint + c] = 1.0 / HII (c, c);
+    for (int i = c + 1; i < nint; i++) {
+      double s = 0.0;
+      for (int k = c; k < i; k++) s += HII (i, k) * Li[k * nint + c];
+      Li[i * nint + c] = -s / HII (i, i);
+    }
+  }
+  __syncthreads ();
+  double* Hi = Hinv + ((size_t) mode * nclass + cls) * nint * nint;
+  for (int idx = tid; idx < nint * nint; idx += nt) {
+    const int a = idx / nint, b = idx % nint;
+    double s = 0.0;
+    for (int k = max (a, b); k < nint; k++) s += Li[k * nint + a] * Li[k * nint + b];
+    Hi[idx] = s;
+  }
+  __syncthreads ();
+  double* B = Bm + ((size_t) mode * nclass + cls) * next * nint;
+  for (int idx = tid; idx < next * nint; idx += nt) {
+    const int a = idx / nint, k = idx % nint;
+    double s = 0.0;
+    for (int b = 0; b < nint; b++) s += H[(size_t) a * npnp + next + b] * Hi[b * nint + k];
+    B[idx] = s;
+  }
+  __syncthreads ();
+  double* So = S + ((size_t) mode * nclass + cls) * next * next;
+  for (int idx = tid; idx < next * next; idx += nt) {
+    const int a = idx / next, b = idx % next;
+    double s = 0.0;
+    for (int k = 0; k < nint; k++) s += B[a * nint + k] * H[(size_t) b * npnp + next + k];
+    So[idx] = H[(size_t) a * npnp + b] - s;
+  }
+#undef HII
+}
+
+__global__ void k_band_assemble (int npairs, const long long* __restrict__ ppos,
+                                 const int* __restrict__ pmn,
+                                 const int* __restrict__ pptr,
+                                 const int* __restrict__ psrc,
+                                 const double* __restrict__ S, size_t s_stride,
+                                 double* band, size_t band_stride,
+                                 const int* __restrict__ nsolve)
+{
+  const int p = blockIdx.x * blockDim.x + threadIdx.x, mode = blockIdx.y;
+  if (p >= npairs) return;
+  if (pmn[2 * p + 1] >= nsolve[mode]) return;
+  const double* Sm = S + (size_t) mode * s_stride;
+  double s = 0.0;
+  for (int c = pptr[p]; c < pptr[p + 1]; c++) s += Sm[psrc[c]];
+  band[(size_t) mode * band_stride + ppos[p]] = s;
+}
+
+// Right-looking banded Cholesky ('U', dpbtrf semantics), one workgroup per
+// mode; rows of the pivot are staged in LDS, the trailing window is
+// updated column by column with one wavefront per column.
+__global__ __launch_bounds__(1024) void k_band_factor (
+    double* band, size_t band_stride, int ld, const int* __restrict__ nsolve,
+    int* status)
+{
+  extern __shared__ double row[];
+  __shared__ double s_dinv;
+  __shared__ int s_bad;
+  const int mode = blockIdx.x, n = nsolve[mode], kd = ld - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6,
+            nw = blockDim.x >> 6;
+  double* A = band + (size_t) mode * band_stride;
+  if (tid == 0) s_bad = 0;
+  for (int j = 0; j < n; j++) {
+    if (tid == 0) {
+      const double d = A[(size_t) j * ld + kd];
+      if (!(d > 0.0)) { s_bad = j + 1; s_dinv = 0.0; }
+      else { const double r = std::sqrt (d); A[(size_t) j * ld + kd] = r; s_dinv = 1.0 / r; }
+    }
+    __syncthreads ();
+    if (s_bad) break;
+    const double dinv = s_dinv;
+    const int last = min (n - 1, j + kd), w = last - j;
+    for (int t = tid; t < w; t += blockDim.x) {
+      const int i = j + 1 + t;
+      const size_t a = (size_t) i * ld + kd + j - i;
+      const double v = A[a] * dinv;
+      A[a] = v;
+      row[t] = v;
+    }
+    __syncthreads ();
+    for (int cl = wave; cl < w; cl += nw) {
+      const int l = j + 1 + cl;
+      const double rl = row[cl];
+      double* col = A + (size_t) l * ld + kd - l;
+      for (int ii = lane; ii <= cl; ii += 64) col[j + 1 + ii] -= row[ii] * rl;
+    }
+    __syncthreads ();
+  }
+  if (tid == 0 && s_bad) status[mode] = s_bad;
+}
+
+// R[i'][o] = U(i, i+kd-o), i = n-1-i': the backward sweep U x = y becomes
+// a forward sweep on R with reversed vectors.
+__global__ void k_band_reverse (const double* __restrict__ U, double* R,
+                                size_t band_stride, int ld,
+                                const int* __restrict__ nsolve)
+{
+  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) n * ld) return;
+  const int ip = (int) (idx / ld), o = (int) (idx % ld);
+  const int i = n - 1 - ip, jj = i + kd - o;
+  const double* Um = U + (size_t) mode * band_stride;
+  R[(size_t) mode * band_stride + idx] = (jj < n) ? Um[(size_t) jj * ld + o] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Solve path
+// ---------------------------------------------------------------------------
+__global__ void k_essential (int nglobal, const int* __restrict__ ess,
+                             const double* __restrict__ lines, int lstride,
+                             double* G, int nz, int base_mode)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = solved_plane (blockIdx.y, nz, base_mode);
+  if (g >= nglobal) return;
+  const int s = ess[g];
+  G[(size_t) k * nglobal + g] = (s >= 0 && lines) ? lines[(size_t) k * lstride + s] : 0.0;
+}
+
+// Elemental Helmholtz operator out = H src on one element by one
+// wavefront (Element::HelmholtzOp/HelmholtzKern, src/element.cpp:1536-1622).
+// src, R, S, out are LDS arrays of npnp; geo points to the element's
+// NGEO arrays in global memory.
+__device__ void helm_op_wave (int np, const double* D, const double* geo,
+                              double lambda2, double betak2, int cyl,
+                              const double* src, double* R, double* S,
+                              double* out, int lane)
+{
+  const int npnp = np * np;
+  for (int ij = lane; ij < npnp; ij += 64) {
+    const int i = ij / np, j = ij % np;
+    double r = 0.0, s = 0.0;
+    for (int k = 0; k < np; k++) {
+      r += src[i * np + k] * D[j * np + k];
+      s += D[i * np + k] * src[k * np + j];
+    }
+    const double g1 = geo[G_Q1 * npnp + ij], g2 = geo[G_Q2 * npnp + ij],
+                 g3 = geo[G_Q3 * npnp + ij], g4 = geo[G_Q4 * npnp + ij];
+    double hCon;
+    if (cyl) {
+      const double y = geo[G_Y * npnp + ij], r2 = y * y;
+      hCon = (r2 > EPSDP) ? betak2 / r2 + lambda2 : 0.0;
+    } else hCon = betak2 + lambda2;
+    R[ij] = g1 * r + g3 * s;
+    S[ij] = g2 * s + g3 * r;
+    out[ij] = g4 * src[ij] * hCon;
+  }
+  __syncthreads ();
+  for (int ij = lane; ij < npnp; ij += 64) {
+    const int i = ij / np, j = ij % np;
+    double t = 0.0;
+    for (int k = 0; k < np; k++) t += D[k * np + i] * S[k * np + j] + R[i * np + k] * D[k * np + j];
+    out[ij] += t;
+  }
+  __syncthreads ();
+}
+
+// One wavefront (= one block of 64) per (element, solved plane).
+__global__ __launch_bounds__(64) void k_condense (
+    int np, int nel, int nclass, int cyl, double lambda2,
+    const double* __restrict__ betak2, const double* __restrict__ geo,
+    const double* __restrict__ Dg, const int* __restrict__ emap_g,
+    const int* __restrict__ cls, const int* __restrict__ btog,
+    const int* __restrict__ emask, const double* __restrict__ Bm,
+    const double* __restrict__ G, int nglobal, double* F, int stride, int nz,
+    int base_mode)
+{
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int e = blockIdx.x, k = solved_plane (blockIdx.y, nz, base_mode);
+  const int mode = k >> 1, lane = threadIdx.x;
+  __shared__ double D[256], buf[256], g[256], R[256], S[256], Hg[256];
+  __shared__ int emap[256];
+  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; emap[i] = emap_g[i]; }
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  double* f = F + (size_t) k * stride + (size_t) e * npnp;
+  for (int i = lane; i < npnp; i += 64) buf[i] = f[i] * ge[G_Q8 * npnp + i];
+  __syncthreads ();
+  if (emask[e]) {
+    for (int i = lane; i < npnp; i += 64) g[i] = 0.0;
+    __syncthreads ();
+    const int* bt = btog + (size_t) e * next;
+    for (int i = lane; i < next; i += 64) g[emap[i]] = G[(size_t) k * nglobal + bt[i]];
+    __syncthreads ();
+    helm_op_wave (np, D, ge, lambda2, betak2[mode], cyl, g, R, S, Hg, lane);
+    for (int i = lane; i < npnp; i += 64) buf[i] += Hg[i];
+    __syncthreads ();
+  }
+  // permuted, negated forcing: F = -(M f + H g) in emap order
+  for (int a = lane; a < npnp; a += 64) g[a] = -buf[emap[a]];
+  __syncthreads ();
+  const double* B = Bm + ((size_t) mode * nclass + cls[e]) * next * nint;
+  for (int a = lane; a < next; a += 64) {
+    double s = 0.0;
+    for (int q = 0; q < nint; q++) s += B[a * nint + q] * g[next + q];
+    f[a] = g[a] - s;
+  }
+  for (int a = next + lane; a < npnp; a += 64) f[a] = g[a];
+}
+
+__global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
+                            const int* __restrict__ gsrc,
+                            const int* __restrict__ nptr,
+                            const int* __restrict__ nidx,
+                            const double* __restrict__ nw,
+                            const double* __restrict__ F, int stride,
+                            const double* __restrict__ lines, int lstride,
+                            const int* __restrict__ nsolve, double* rhs,
+                            int nmax, int nz, int base_mode)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = solved_plane (blockIdx.y, nz, base_mode);
+  const int mode = k >> 1, rr = k & 1;
+  if (g >= nsolve[mode]) return;
+  const double* Fk = F + (size_t) k * stride;
+  double s = 0.0;
+  for (int c = gptr[g]; c < gptr[g + 1]; c++) s += Fk[gsrc[c]];
+  if (lines)
+    for (int c = nptr[g]; c < nptr[g + 1]; c++)
+      s += lines[(size_t) k * lstride + nidx[c]] * nw[c];
+  rhs[((size_t) mode * 2 + rr) * nmax + g] = s;
+}
+
+// FP64 banded solve U^T U x = b for both RHS of one mode per workgroup.
+// Blocks of 64 unknowns: 16 wavefronts stream the block's 64 contiguous
+// band columns (coalesced), form the dot products with the already solved
+// part of the vector (held in LDS), and stage the 64x64 triangle in LDS;
+// wavefront 0 then resolves the triangle with lane-broadcasts.  The second
+// pass runs the same sweep on the reversed copy R (backward substitution).
+template <int NU>
+__global__ __launch_bounds__(1024) void k_band_solve (
+    const double* __restrict__ bandU, const double* __restrict__ bandR,
+    size_t band_stride, int ld, const int* __restrict__ nsolve, double* rhs,
+    int nmax)
+{
+  extern __shared__ double smem[];
+  const int mode = blockIdx.x, n = nsolve[mode];
+  if (n == 0) return;
+  const int kd = ld - 1, vlen = n + kd;
+  double* yp0 = smem;
+  double* yp1 = yp0 + vlen;
+  double* tri = yp1 + vlen;          // [64][64]: tri[t*64+c] = U(j0+t, j0+c)
+  double* part = tri + 64 * 64;      // [64][2]
+  double* b0 = rhs + ((size_t) mode * 2) * nmax;
+  double* b1 = b0 + nmax;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + 63) >> 6;
+
+  for (int pass = 0; pass < 2; pass++) {
+    const double* A = (pass == 0 ? bandU : bandR) + (size_t) mode * band_stride;
+    for (int i = tid; i < 2 * vlen; i += blockDim.x) smem[i] = 0.0;
+    __syncthreads ();
+    for (int blk = 0; blk < nblk; blk++) {
+      const int j0 = blk << 6;
+      for (int i = tid; i < 64 * 64; i += blockDim.x) tri[i] = 0.0;
+      __syncthreads ();
+      double v[4][NU];
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) {
+        const int c = wave * 4 + cc, j = j0 + c;
+        const double* col = A + (size_t) j * ld;
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int o = lane + 64 * u;
+          v[cc][u] = (j < n && o < ld) ? col[o] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int cc = 0; cc < 4; cc++) {
+        const int c = wave * 4 + cc, j = j0 + c;
+        double a0 = 0.0, a1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int o = lane + 64 * u;
+          if (o < kd - c) {
+            const int idx = j + o;          // row j-kd+o, padded by kd
+            a0 += v[cc][u] * yp0[idx];
+            a1 += v[cc][u] * yp1[idx];
+          } else if (o <= kd && j < n) {
+            tri[(c - kd + o) * 64 + c] = v[cc][u];
+          }
+        }
+        a0 = wave_sum (a0);
+        a1 = wave_sum (a1);
+        if (lane == 0) { part[c * 2] = a0; part[c * 2 + 1] = a1; }
+      }
+      __syncthreads ();
+      if (wave == 0) {
+        const int c = lane, j = j0 + c;
+        const bool valid = j < n;
+        const int gi = (pass == 0) ? j : n - 1 - j;
+        double a0 = valid ? b0[gi] - part[c * 2] : 0.0;
+        double a1 = valid ? b1[gi] - part[c * 2 + 1] : 0.0;
+        const double dinv = valid ? 1.0 / tri[c * 64 + c] : 0.0;
+        const int tmax = min (64, n - j0);
+        for (int t = 0; t < tmax; t++) {
+          const double y0 = readlane_d (a0 * dinv, t);
+          const double y1 = readlane_d (a1 * dinv, t);
+          if (c > t) {
+            const double L = tri[t * 64 + c];
+            a0 -= L * y0;
+            a1 -= L * y1;
+          }
+        }
+        if (valid) {
+          const double y0 = a0 * dinv, y1 = a1 * dinv;
+          yp0[kd + j] = y0;
+          yp1[kd + j] = y1;
+          b0[gi] = y0;
+          b1[gi] = y1;
+        }
+      }
+      __syncthreads ();
+    }
+  }
+}
+
+// One wavefront per (element, solved plane): interior recovery and
+// essential-BC insertion (Element::global2localSC src/element.cpp:368-407,
+// Field::setEssential src/field.cpp:1105-1124).
+__global__ __launch_bounds__(64) void k_recover (
+    int np, int nclass, const int* __restrict__ emap_g,
+    const int* __restrict__ cls, const int* __restrict__ btog,
+    const int* __restrict__ bmask, const int* __restrict__ emask,
+    const double* __restrict__ Bm, const double* __restrict__ Hinv,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ F,
+    double* U, int stride, const int* __restrict__ nsolve,
+    const double* __restrict__ rhs, int nmax, int nz, int base_mode)
+{
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int e = blockIdx.x, k = solved_plane (blockIdx.y, nz, base_mode);
+  const int mode = k >> 1, rr = k & 1, lane = threadIdx.x;
+  __shared__ double ub[64], fi[256];
+  __shared__ int emap[256];
+  for (int i = lane; i < npnp; i += 64) emap[i] = emap_g[i];
+  const int* bt = btog + (size_t) e * next;
+  const int ns = nsolve[mode];
+  const double* x = rhs + ((size_t) mode * 2 + rr) * nmax;
+  const double* f = F + (size_t) k * stride + (size_t) e * npnp;
+  double* u = U + (size_t) k * stride + (size_t) e * npnp;
+  for (int i = lane; i < next; i += 64) { const int gg = bt[i]; ub[i] = gg < ns ? x[gg] : 0.0; }
+  for (int a = lane; a < nint; a += 64) fi[a] = f[next + a];
+  __syncthreads ();
+  const size_t mc = (size_t) mode * nclass + cls[e];
+  const double* B = Bm + mc * next * nint;
+  const double* Hi = Hinv + mc * nint * nint;
+  for (int a = lane; a < nint; a += 64) {
+    double s = 0.0;
+    for (int b = 0; b < nint; b++) s += Hi[b * nint + a] * fi[b];
+    for (int c = 0; c < next; c++) s -= B[c * nint + a] * ub[c];
+    u[emap[next + a]] = s;
+  }
+  for (int i = lane; i < next; i += 64) {
+    double v = ub[i];
+    if (emask[e] && bmask[(size_t) e * next + i]) v = G[(size_t) k * nglobal + bt[i]];
+    u[emap[i]] = v;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_helm_op (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const double* __restrict__ Uin, double* Uout, int stride, int nz,
+    int base_mode)
+{
+  const int npnp = np * np;
+  const int e = blockIdx.x, k = solved_plane (blockIdx.y, nz, base_mode);
+  const int lane = threadIdx.x;
+  __shared__ double D[256], src[256], R[256], S[256], out[256];
+  for (int i = lane; i < npnp; i += 64) {
+    D[i] = Dg[i];
+    src[i] = Uin[(size_t) k * stride + (size_t) e * npnp + i];
+  }
+  __syncthreads ();
+  helm_op_wave (np, D, geo + (size_t) e * NGEO * npnp, lambda2, betak2[k >> 1],
+                cyl, src, R, S, out, lane);
+  for (int i = lane; i < npnp; i += 64)
+    Uout[(size_t) k * stride + (size_t) e * npnp + i] = out[i];
+}
+
+// ---------------------------------------------------------------------------
+size_t solve_lds_bytes (int n, int ld) { return (2 * (size_t) (n + ld - 1) + 64 * 64 + 128) * sizeof (double); }
+constexpr size_t LDS_MAX = 160 * 1024;
+
+template <int NU>
+int launch_band_solve_nu (const sx_msys* s, hipStream_t st)
+{
+  const size_t lds = solve_lds_bytes (s->nmax, s->m->nband);
+  SX_HIP (hipFuncSetAttribute ((const void*) k_band_solve<NU>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_band_solve<NU>, dim3 (s->nmodes), dim3 (1024), lds, st,
+                      s->d_bandU, s->d_bandR, s->band_stride, s->m->nband,
+                      s->d_nsolve, s->d_rhs, s->nmax);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+int launch_band_solve (const sx_msys* s, hipStream_t st)
+{
+  const int nu = (s->m->nband + 63) / 64;
+  switch (nu) {
+  case 1: return launch_band_solve_nu<1> (s, st);
+  case 2: return launch_band_solve_nu<2> (s, st);
+  case 3: return launch_band_solve_nu<3> (s, st);
+  case 4: return launch_band_solve_nu<4> (s, st);
+  case 5: return launch_band_solve_nu<5> (s, st);
+  case 6: return launch_band_solve_nu<6> (s, st);
+  case 7: return launch_band_solve_nu<7> (s, st);
+  case 8: return launch_band_solve_nu<8> (s, st);
+  case 9: return launch_band_solve_nu<9> (s, st);
+  case 10: return launch_band_solve_nu<10> (s, st);
+  case 11: return launch_band_solve_nu<11> (s, st);
+  case 12: return launch_band_solve_nu<12> (s, st);
+  default:
+    set_error ("band width above 768 not supported by the band solver");
+    return SX_ERR_LIMIT;
+  }
+}
+
+int n_solved_planes (int nz, int base_mode) { return (base_mode == 0 && nz > 1) ? nz - 1 : nz; }
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" int sx_msys_destroy (sx_msys_t s)
+{
+  if (!s) return SX_OK;
+  void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Bm32,
+                  s->d_Hinv32, s->d_bandU, s->d_bandR, s->d_rhs, s->d_G,
+                  s->d_work};
+  for (void* p : ptrs) if (p) (void) hipFree (p);
+  delete s;
+  return SX_OK;
+}
+
+extern "C" int sx_msys_nsolve (sx_msys_t s, int mode)
+{
+  if (!s || mode < 0 || mode >= s->nmodes) return -1;
+  return s->nsolve[mode];
+}
+
+__global__ void k_to_f32 (const double* a, float* b, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float) a[i];
+}
+
+extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
+                               int base_mode, int num_modes, int precision,
+                               void* stream, sx_msys_t* out)
+{
+  if (!m || !out || num_modes < 1 || base_mode < 0 ||
+      (precision != SX_FP64 && precision != SX_MIXED)) {
+    set_error ("sx_msys_create: bad arguments");
+    return SX_ERR_ARG;
+  }
+  *out = nullptr;
+  hipStream_t st = (hipStream_t) stream;
+  sx_msys* s = new sx_msys;
+  s->m = m; s->nmodes = num_modes; s->base_mode = base_mode;
+  s->precision = precision; s->lambda2 = lambda2; s->beta = beta;
+  const int next = m->next, nint = m->nint, npnp = m->npnp, ld = m->nband;
+  const bool fixed = m->nglobal > m->nsolve;
+  for (int md = 0; md < num_modes; md++) {
+    const double bk = beta * (base_mode + md);
+    const double bk2 = bk * bk;
+    s->betak2.push_back (bk2);
+    const bool singular = (lambda2 + bk2) < EPSSP && !fixed;
+    s->nsolve.push_back (singular ? m->nsolve - 1 : m->nsolve);
+  }
+  s->nmax = *std::max_element (s->nsolve.begin (), s->nsolve.end ());
+  s->band_stride = (size_t) std::max (s->nmax, 1) * ld;
+  if (solve_lds_bytes (s->nmax, ld) > LDS_MAX) {
+    sx_msys_destroy (s);
+    set_error ("sx_msys_create: nsolve + nband too large for the LDS-resident band solver");
+    return SX_ERR_LIMIT;
+  }
+
+  int rc = SX_OK;
+  auto fail = [&] (int r) { sx_msys_destroy (s); return r; };
+#define TRY(call) do { hipError_t _e = (call); if (_e != hipSuccess) return fail (hip_fail (_e, #call)); } while (0)
+  const size_t nmc = (size_t) num_modes * m->nclass;
+  TRY (hipMalloc (&s->d_betak2, sizeof (double) * num_modes));
+  TRY (hipMalloc (&s->d_nsolve, sizeof (int) * num_modes));
+  TRY (hipMemcpy (s->d_betak2, s->betak2.data (), sizeof (double) * num_modes, hipMemcpyHostToDevice));
+  TRY (hipMemcpy (s->d_nsolve, s->nsolve.data (), sizeof (int) * num_modes, hipMemcpyHostToDevice));
+  TRY (hipMalloc (&s->d_Bm, sizeof (double) * std::max<size_t> (nmc * next * nint, 1)));
+  TRY (hipMalloc (&s->d_Hinv, sizeof (double) * std::max<size_t> (nmc * nint * nint, 1)));
+  TRY (hipMalloc (&s->d_bandU, sizeof (double) * num_modes * s->band_stride));
+  TRY (hipMalloc (&s->d_bandR, sizeof (double) * num_modes * s->band_stride));
+  TRY (hipMalloc (&s->d_rhs, sizeof (double) * num_modes * 2 * std::max (s->nmax, 1)));
+  TRY (hipMalloc (&s->d_G, sizeof (double) * 2 * num_modes * m->nglobal));
+  TRY (hipMemsetAsync (s->d_rhs, 0, sizeof (double) * num_modes * 2 * std::max (s->nmax, 1), st));
+  TRY (hipMemsetAsync (s->d_bandU, 0, sizeof (double) * num_modes * s->band_stride, st));
+
+  double *d_ws = nullptr, *d_S = nullptr;
+  int* d_status = nullptr;
+  const size_t ws_per = (size_t) npnp * npnp + (size_t) nint * nint;
+  TRY (hipMalloc (&d_ws, sizeof (double) * nmc * ws_per));
+  TRY (hipMalloc (&d_S, sizeof (double) * nmc * next * next));
+  TRY (hipMalloc (&d_status, sizeof (int) * (num_modes + 1)));
+  TRY (hipMemsetAsync (d_status, 0, sizeof (int) * (num_modes + 1), st));
+
+  hipLaunchKernelGGL (k_elem_sc, dim3 ((unsigned) nmc), dim3 (256), 0, st, m->np,
+                      m->nclass, m->cyl, lambda2, s->d_betak2, m->d_cgeo, m->d_D,
+                      m->d_emap, d_ws, d_S, s->d_Bm, s->d_Hinv, d_status + num_modes);
+  TRY (hipGetLastError ());
+  if (m->npairs)
+    hipLaunchKernelGGL (k_band_assemble, dim3 ((m->npairs + 255) / 256, num_modes),
+                        dim3 (256), 0, st, m->npairs, m->d_ppos, m->d_pmn, m->d_pptr,
+                        m->d_psrc, d_S, (size_t) m->nclass * next * next, s->d_bandU,
+                        s->band_stride, s->d_nsolve);
+  TRY (hipGetLastError ());
+  hipLaunchKernelGGL (k_band_factor, dim3 (num_modes), dim3 (1024),
+                      sizeof (double) * std::max (ld, 1), st, s->d_bandU,
+                      s->band_stride, ld, s->d_nsolve, d_status);
+  TRY (hipGetLastError ());
+  hipLaunchKernelGGL (k_band_reverse,
+                      dim3 ((unsigned) ((s->band_stride + 255) / 256), num_modes),
+                      dim3 (256), 0, st, s->d_bandU, s->d_bandR, s->band_stride, ld,
+                      s->d_nsolve);
+  TRY (hipGetLastError ());
+  std::vector<int> status (num_modes + 1);
+  TRY (hipMemcpyAsync (status.data (), d_status, sizeof (int) * (num_modes + 1),
+                       hipMemcpyDeviceToHost, st));
+  TRY (hipStreamSynchronize (st));
+  (void) hipFree (d_ws); (void) hipFree (d_S); (void) hipFree (d_status);
+  if (status[num_modes]) {
+    set_error ("sx_msys_create: matrix hii is not positive definite");
+    return fail (SX_ERR_FACTOR);
+  }
+  for (int md = 0; md < num_modes; md++)
+    if (status[md]) {
+      set_error ("sx_msys_create: failed to factor Helmholtz matrix of mode " +
+                 std::to_string (base_mode + md) + " (column " +
+                 std::to_string (status[md]) + ")");
+      return fail (SX_ERR_FACTOR);
+    }
+  if (precision == SX_MIXED) {
+    TRY (hipMalloc (&s->d_Bm32, sizeof (float) * std::max<size_t> (nmc * next * nint, 1)));
+    TRY (hipMalloc (&s->d_Hinv32, sizeof (float) * std::max<size_t> (nmc * nint * nint, 1)));
+    const size_t n1 = nmc * next * nint, n2 = nmc * nint * nint;
+    if (n1) hipLaunchKernelGGL (k_to_f32, dim3 ((unsigned) ((n1 + 255) / 256)), dim3 (256), 0, st, s->d_Bm, s->d_Bm32, n1);
+    if (n2) hipLaunchKernelGGL (k_to_f32, dim3 ((unsigned) ((n2 + 255) / 256)), dim3 (256), 0, st, s->d_Hinv, s->d_Hinv32, n2);
+    TRY (hipGetLastError ());
+    TRY (hipStreamSynchronize (st));
+  }
+#undef TRY
+  (void) rc;
+  *out = s;
+  return SX_OK;
+}
+
+static int check_solve_args (sx_msys_t s, int nz, int stride, const void* f,
+                             const void* u)
+{
+  if (!s || !f || !u || nz < 1 || stride < s->m->nel * s->m->npnp) {
+    set_error ("sx_field_solve: bad arguments");
+    return SX_ERR_ARG;
+  }
+  if (nz > 2 * s->nmodes && !(nz == 1)) {
+    set_error ("sx_field_solve: more planes than modes in the system");
+    return SX_ERR_ARG;
+  }
+  return SX_OK;
+}
+
+extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
+                               double* d_u, const double* d_lines, void* stream)
+{
+  int rc = check_solve_args (s, nz, stride, d_f, d_u);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t) stream;
+  const sx_mesh* m = s->m;
+  const int npl = n_solved_planes (nz, s->base_mode);
+  const int lstride = m->nbound * m->np;
+  const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
+  hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                      st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+  hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
+                      m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
+                      m->d_emap, m->d_cls, m->d_btog, m->d_emask, s->d_Bm, s->d_G,
+                      m->nglobal, d_f, stride, nz, s->base_mode);
+  hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                      st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
+                      d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                      s->base_mode);
+  SX_HIP (hipGetLastError ());
+  if ((rc = launch_band_solve (s, st))) return rc;
+  hipLaunchKernelGGL (k_recover, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
+                      m->d_emap, m->d_cls, m->d_btog, m->d_bmask, m->d_emask, s->d_Bm,
+                      s->d_Hinv, s->d_G, m->nglobal, d_f, d_u, stride, s->d_nsolve,
+                      s->d_rhs, s->nmax, nz, s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_f,
+                                     double* d_u, const double* d_lines,
+                                     double refine_tol, int max_iter, int* iters,
+                                     void* stream)
+{
+  (void) s; (void) nz; (void) stride; (void) d_f; (void) d_u; (void) d_lines;
+  (void) refine_tol; (void) max_iter; (void) iters; (void) stream;
+  set_error ("sx_field_solve_mixed: not implemented yet");
+  return SX_ERR_LIMIT;
+}
+
+extern "C" int sx_field_solve_host (sx_msys_t s, int nz, int stride, const double* f,
+                                    double* u, const double* lines)
+{
+  int rc = check_solve_args (s, nz, stride, f, u);
+  if (rc) return rc;
+  const size_t nb = sizeof (double) * (size_t) nz * stride;
+  const size_t lb = sizeof (double) * (size_t) nz * s->m->nbound * s->m->np;
+  double *df = nullptr, *du = nullptr, *dl = nullptr;
+  SX_HIP (hipMalloc (&df, nb));
+  SX_HIP (hipMalloc (&du, nb));
+  SX_HIP (hipMemcpy (df, f, nb, hipMemcpyHostToDevice));
+  SX_HIP (hipMemcpy (du, u, nb, hipMemcpyHostToDevice));
+  if (lines && lb) {
+    SX_HIP (hipMalloc (&dl, lb));
+    SX_HIP (hipMemcpy (dl, lines, lb, hipMemcpyHostToDevice));
+  }
+  rc = sx_field_solve (s, nz, stride, df, du, dl, nullptr);
+  if (rc == SX_OK) {
+    SX_HIP (hipMemcpy (u, du, nb, hipMemcpyDeviceToHost));
+  }
+  (void) hipFree (df); (void) hipFree (du); if (dl) (void) hipFree (dl);
+  return rc;
+}
+
+extern "C" int sx_helmholtz_op (sx_msys_t s, int nz, int stride, const double* d_u,
+                                double* d_hu, void* stream)
+{
+  int rc = check_solve_args (s, nz, stride, d_u, d_hu);
+  if (rc) return rc;
+  const sx_mesh* m = s->m;
+  hipLaunchKernelGGL (k_helm_op, dim3 (m->nel, n_solved_planes (nz, s->base_mode)),
+                      dim3 (64), 0, (hipStream_t) stream, m->np, m->cyl, s->lambda2,
+                      s->d_betak2, m->d_geo, m->d_D, d_u, d_hu, stride, nz,
+                      s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}

@@ -1,0 +1,136 @@
+"""Host-side mirror of semtex's Helmholtz solve interface over the C ABI.
+
+    M = ModalMatrixSys(lambda2, beta, base_mode, num_modes, mesh)   # src/matrix.h:61-66
+    u = Field(mesh, nz);  u.solve(f, M)                              # src/field.h:27
+
+Field data are torch tensors on the GPU laid out as the reference's
+AuxField: [nz][plane_size] doubles, element-major, row-major np x np per
+element (src/auxfield.h:109-114).  torch supplies device memory and the
+stream only; all arithmetic runs in the HIP library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh2D
+
+SX_FP64, SX_MIXED = _lib.SX_FP64, _lib.SX_MIXED
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None and a.size else None
+
+
+class GpuMesh:
+    """Element geometry + NumberSys + BoundarySys resident on the GPU."""
+
+    def __init__(self, mesh: Mesh2D):
+        L = _lib.load()
+        self.mesh = mesh
+        keep = [np.ascontiguousarray(mesh.x, np.float64),
+                np.ascontiguousarray(mesh.y, np.float64),
+                np.ascontiguousarray(mesh.btog, np.int32),
+                np.ascontiguousarray(mesh.bmask, np.int32),
+                np.ascontiguousarray(mesh.emask, np.int32),
+                np.ascontiguousarray(mesh.bnd_elmt, np.int32),
+                np.ascontiguousarray(mesh.bnd_side, np.int32),
+                np.ascontiguousarray(mesh.bnd_kind, np.int32)]
+        x, y, bt, bm, em, be, bs, bk = keep
+        h = C.c_void_p()
+        _lib.check(L.sx_mesh_create(mesh.np, mesh.nel, mesh.cyl, _ptr(x), _ptr(y),
+                                    _ptr(bt), _ptr(bm), _ptr(em), mesh.nglobal,
+                                    mesh.nsolve, mesh.nband, mesh.nbound, _ptr(be),
+                                    _ptr(bs), _ptr(bk), C.byref(h)), "sx_mesh_create")
+        self.handle = h
+        self.nclass = L.sx_mesh_nclass(h)
+
+    @property
+    def plane_size(self):
+        return self.mesh.plane_size
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.load().sx_mesh_destroy(self.handle)
+            self.handle = None
+
+
+class ModalMatrixSys:
+    """Per-Fourier-mode DIRECT systems (src/matrix.cpp:33-105), factorised on
+    the GPU.  mode k uses betak2 = (beta * (base_mode + k))^2."""
+
+    def __init__(self, lambda2, beta, base_mode, num_modes, gmesh: GpuMesh,
+                 precision=SX_FP64):
+        L = _lib.load()
+        self.gmesh = gmesh
+        self.lambda2, self.beta = float(lambda2), float(beta)
+        self.base_mode, self.num_modes = int(base_mode), int(num_modes)
+        self.precision = precision
+        h = C.c_void_p()
+        _lib.check(L.sx_msys_create(gmesh.handle, self.lambda2, self.beta,
+                                    self.base_mode, self.num_modes, precision,
+                                    _stream(), C.byref(h)), "sx_msys_create")
+        self.handle = h
+
+    def nsolve(self, mode):
+        return _lib.load().sx_msys_nsolve(self.handle, mode)
+
+    def __len__(self):
+        return self.num_modes
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.load().sx_msys_destroy(self.handle)
+            self.handle = None
+
+
+class Field:
+    """A scalar field with boundary line data (src/field.h)."""
+
+    def __init__(self, gmesh: GpuMesh, nz: int, device="cuda"):
+        import torch
+        m = gmesh.mesh
+        self.gmesh, self.nz = gmesh, nz
+        self.data = torch.zeros((nz, m.plane_size), dtype=torch.float64, device=device)
+        self.lines = torch.zeros((nz, max(m.nbound * m.np, 1)), dtype=torch.float64,
+                                 device=device)
+
+    def solve(self, f, M: ModalMatrixSys, mixed_tol=None, max_iter=20):
+        """Field::solve (src/field.cpp:598-709): f ([nz][plane_size] CUDA
+        float64 tensor) is destroyed; the solution overwrites self.data."""
+        import torch
+        assert f.dtype == torch.float64 and f.is_cuda and f.is_contiguous()
+        assert tuple(f.shape) == tuple(self.data.shape)
+        L = _lib.load()
+        lines = C.c_void_p(self.lines.data_ptr()) if self.gmesh.mesh.nbound else None
+        if M.precision == SX_MIXED:
+            it = C.c_int(0)
+            _lib.check(L.sx_field_solve_mixed(M.handle, self.nz, f.shape[1],
+                                              C.c_void_p(f.data_ptr()),
+                                              C.c_void_p(self.data.data_ptr()), lines,
+                                              float(mixed_tol or 0.0), int(max_iter),
+                                              C.byref(it), _stream()),
+                       "sx_field_solve_mixed")
+            self.last_iters = it.value
+        else:
+            _lib.check(L.sx_field_solve(M.handle, self.nz, f.shape[1],
+                                        C.c_void_p(f.data_ptr()),
+                                        C.c_void_p(self.data.data_ptr()), lines,
+                                        _stream()), "sx_field_solve")
+        return self
+
+
+def helmholtz_op(M: ModalMatrixSys, u, out):
+    """out = H u element by element (Element::HelmholtzOp), per plane."""
+    _lib.check(_lib.load().sx_helmholtz_op(M.handle, u.shape[0], u.shape[1],
+                                           C.c_void_p(u.data_ptr()),
+                                           C.c_void_p(out.data_ptr()), _stream()),
+               "sx_helmholtz_op")
+    return out
