@@ -113,6 +113,16 @@ int sx_field_solve_host (sx_msys_t s, int nz, int plane_stride,
 int sx_helmholtz_op (sx_msys_t s, int nz, int plane_stride, const double* d_u,
                      double* d_hu, void* stream);
 
+/* Per-kernel timing of sx_field_solve with HIP events recorded on the
+ * solve's stream (no counterpart in the reference, whose only timer is the CPU
+ * clock dclock(), veclib/util.c:68).  Phases: 0 essential BCs,
+ * 1 condense, 2 assemble, 3 band solve, 4 recover.  sx_timing_read
+ * synchronises the recorded events, fills ms[p] (total milliseconds) and
+ * count[p] (launches) for p < nphase, and clears the record. */
+#define SX_NPHASE 5
+int sx_timing_enable (int on);
+int sx_timing_read (double* ms, int* count, int nphase);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,7 +32,11 @@ SIGNATURES = {
                                   C.POINTER(_i), _vp]),
     "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
+    "sx_timing_enable": (_i, [_i]),
+    "sx_timing_read": (_i, [_dp, _ip, _i]),
 }
+SX_NPHASE = 5
+PHASES = ("essential", "condense", "assemble", "band_solve", "recover")
 
 _lib = None
 

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "sx_internal.h"
 
@@ -577,7 +578,62 @@ int launch_band_solve (const sx_msys* s, hipStream_t st)
 
 int n_solved_planes (int nz, int base_mode) { return (base_mode == 0 && nz > 1) ? nz - 1 : nz; }
 
+// Optional per-phase event timing of the solve (sx_timing_enable).
+struct PhaseRec { int phase; hipEvent_t a, b; };
+struct Timing {
+  bool on = false;
+  std::vector<PhaseRec> recs;
+  std::vector<hipEvent_t> pool;
+  hipEvent_t get ()
+  {
+    if (!pool.empty ()) { hipEvent_t e = pool.back (); pool.pop_back (); return e; }
+    hipEvent_t e;
+    (void) hipEventCreate (&e);
+    return e;
+  }
+} g_timing;
+
+struct PhaseScope {
+  PhaseRec r;
+  hipStream_t st;
+  bool on;
+  PhaseScope (int phase, hipStream_t s) : st (s), on (g_timing.on)
+  {
+    if (!on) return;
+    r.phase = phase; r.a = g_timing.get (); r.b = g_timing.get ();
+    (void) hipEventRecord (r.a, st);
+  }
+  ~PhaseScope ()
+  {
+    if (!on) return;
+    (void) hipEventRecord (r.b, st);
+    g_timing.recs.push_back (r);
+  }
+};
+
 }  // namespace
+
+extern "C" int sx_timing_enable (int on)
+{
+  g_timing.on = on != 0;
+  return SX_OK;
+}
+
+extern "C" int sx_timing_read (double* ms, int* count, int nphase)
+{
+  if (!ms || !count || nphase < 0) { set_error ("sx_timing_read: bad arguments"); return SX_ERR_ARG; }
+  for (int p = 0; p < nphase; p++) { ms[p] = 0.0; count[p] = 0; }
+  for (auto& r : g_timing.recs) {
+    SX_HIP (hipEventSynchronize (r.b));
+    float t = 0.0f;
+    SX_HIP (hipEventElapsedTime (&t, r.a, r.b));
+    if (r.phase < nphase) { ms[r.phase] += t; count[r.phase]++; }
+    g_timing.pool.push_back (r.a);
+    g_timing.pool.push_back (r.b);
+  }
+  g_timing.recs.clear ();
+  return SX_OK;
+}
 
 // ===========================================================================
 // C ABI
@@ -735,22 +791,37 @@ extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
   const int npl = n_solved_planes (nz, s->base_mode);
   const int lstride = m->nbound * m->np;
   const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
-  hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                      st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
-  hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
-                      m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
-                      m->d_emap, m->d_cls, m->d_btog, m->d_emask, s->d_Bm, s->d_G,
-                      m->nglobal, d_f, stride, nz, s->base_mode);
-  hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                      st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
-                      d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
-                      s->base_mode);
+  {
+    PhaseScope ps (0, st);
+    hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                        st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+  }
+  {
+    PhaseScope ps (1, st);
+    hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
+                        m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
+                        m->d_emap, m->d_cls, m->d_btog, m->d_emask, s->d_Bm, s->d_G,
+                        m->nglobal, d_f, stride, nz, s->base_mode);
+  }
+  {
+    PhaseScope ps (2, st);
+    hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                        st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
+                        d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                        s->base_mode);
+  }
   SX_HIP (hipGetLastError ());
-  if ((rc = launch_band_solve (s, st))) return rc;
-  hipLaunchKernelGGL (k_recover, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
-                      m->d_emap, m->d_cls, m->d_btog, m->d_bmask, m->d_emask, s->d_Bm,
-                      s->d_Hinv, s->d_G, m->nglobal, d_f, d_u, stride, s->d_nsolve,
-                      s->d_rhs, s->nmax, nz, s->base_mode);
+  {
+    PhaseScope ps (3, st);
+    if ((rc = launch_band_solve (s, st))) return rc;
+  }
+  {
+    PhaseScope ps (4, st);
+    hipLaunchKernelGGL (k_recover, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
+                        m->d_emap, m->d_cls, m->d_btog, m->d_bmask, m->d_emask, s->d_Bm,
+                        s->d_Hinv, s->d_G, m->nglobal, d_f, d_u, stride, s->d_nsolve,
+                        s->d_rhs, s->nmax, nz, s->base_mode);
+  }
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
