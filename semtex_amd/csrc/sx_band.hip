@@ -1,0 +1,421 @@
+// sx_band.hip -- FP64 banded Cholesky factorisation and solve on gfx950,
+// one workgroup per Fourier mode (LAPACK dpbtrf/dpbtrs('U') semantics,
+// called by the reference at src/matrix.cpp:252 and src/field.cpp:693).
+//
+// The factor is assembled and factorised in LAPACK 'U' band storage
+// (column j holds U(j-kd..j, j) at offsets 0..kd, ld = kd+1,
+// veclib/lapack.h:107).  The solve U^T U x = b is then two identical
+// forward sweeps:
+//   forward  U^T y = b   on the factor itself,
+//   backward U x = y     as a forward sweep on the reversed factor V,
+//                        V(i', j') = U(n-1-j', n-1-i'), with reversed vectors.
+// Each sweep processes blocks of B columns (B <= kd+1).  In block J
+//   r_J = b_J - sum_{I<J} W_IJ^T y_I     (the band rows above the block)
+//   y_J = W_JJ^-T r_J                    (W_JJ^-1 precomputed in set-up)
+// so no sequential substitution remains inside a block.  For the sweep
+// each block of the band is stored offset-major, X[J][q][c] = column j0+c
+// at band offset q (q < kd - c: rows above the block; q >= kd - c: the
+// inverted triangle), so a wavefront reads one coalesced row q for all B
+// columns and lane c accumulates column c's dot product by itself -- no
+// cross-lane reductions.  Eight wavefronts split the offsets and meet
+// through LDS.  The solved values live in a circular LDS window of W >=
+// kd+B rows, so LDS does not bound n or the number of right-hand sides.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <string>
+#include <cstdlib>
+
+#include "sx_internal.h"
+
+namespace sx {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Factorisation (set-up): right-looking, the pivot row staged in LDS, the
+// trailing window updated one column per wavefront.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_band_factor (
+    double* band, size_t band_stride, int ld, const int* __restrict__ nsolve,
+    int* status)
+{
+  extern __shared__ double row[];
+  __shared__ double s_dinv;
+  __shared__ int s_bad;
+  const int mode = blockIdx.x, n = nsolve[mode], kd = ld - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6,
+            nw = blockDim.x >> 6;
+  double* A = band + (size_t) mode * band_stride;
+  if (tid == 0) s_bad = 0;
+  for (int j = 0; j < n; j++) {
+    if (tid == 0) {
+      const double d = A[(size_t) j * ld + kd];
+      if (!(d > 0.0)) { s_bad = j + 1; s_dinv = 0.0; }
+      else { const double r = std::sqrt (d); A[(size_t) j * ld + kd] = r; s_dinv = 1.0 / r; }
+    }
+    __syncthreads ();
+    if (s_bad) break;
+    const double dinv = s_dinv;
+    const int last = min (n - 1, j + kd), w = last - j;
+    for (int t = tid; t < w; t += blockDim.x) {
+      const int i = j + 1 + t;
+      const size_t a = (size_t) i * ld + kd + j - i;
+      const double v = A[a] * dinv;
+      A[a] = v;
+      row[t] = v;
+    }
+    __syncthreads ();
+    for (int cl = wave; cl < w; cl += nw) {
+      const int l = j + 1 + cl;
+      const double rl = row[cl];
+      double* col = A + (size_t) l * ld + kd - l;
+      for (int ii = lane; ii <= cl; ii += 64) col[j + 1 + ii] -= row[ii] * rl;
+    }
+    __syncthreads ();
+  }
+  if (tid == 0 && s_bad) status[mode] = s_bad;
+}
+
+// Invert the BxB upper-triangular diagonal blocks in place; one wavefront
+// per (block, mode), lane c computes column c by back substitution.
+__global__ __launch_bounds__(64) void k_band_tinv (
+    double* band, size_t band_stride, int ld, int B,
+    const int* __restrict__ nsolve)
+{
+  __shared__ double T[64 * 64];
+  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
+  const int j0 = blockIdx.x * B;
+  if (j0 >= n) return;
+  const int bs = min (B, n - j0), c = threadIdx.x;
+  double* A = band + (size_t) mode * band_stride;
+  if (c < bs)
+    for (int t = 0; t <= c; t++) T[t * 64 + c] = A[(size_t) (j0 + c) * ld + kd - c + t];
+  __syncthreads ();
+  if (c < bs) {
+    // column c of inv(T) overwrites column c of the band (T stays in LDS);
+    // the lane re-reads only its own writes
+    double* X = A + (size_t) (j0 + c) * ld + kd - c;   // X[t] = inv(T)[t][c]
+    X[c] = 1.0 / T[c * 64 + c];
+    for (int t = c - 1; t >= 0; t--) {
+      double s = 0.0;
+      for (int k = t + 1; k <= c; k++) s += T[t * 64 + k] * X[k];
+      X[t] = -s / T[t * 64 + t];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: reversed band and offset-major sweep blocks
+// ---------------------------------------------------------------------------
+// V[i'][o] = U(i, i+kd-o), i = n-1-i'  (the backward sweep U x = y is a
+// forward sweep on V with reversed vectors).
+__global__ void k_band_reverse (const double* __restrict__ U, double* V,
+                                size_t band_stride, int ld,
+                                const int* __restrict__ nsolve)
+{
+  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) n * ld) return;
+  const int ip = (int) (idx / ld), o = (int) (idx % ld);
+  const int i = n - 1 - ip, jj = i + kd - o;
+  const double* Um = U + (size_t) mode * band_stride;
+  V[(size_t) mode * band_stride + idx] = (jj < n) ? Um[(size_t) jj * ld + o] : 0.0;
+}
+
+// X[J][q][c] = band column J*B+c at offset q (zero past the last column).
+__global__ void k_band_blockify (const double* __restrict__ U, double* X,
+                                 size_t band_stride, size_t sw_stride, int ld,
+                                 int B, const int* __restrict__ nsolve)
+{
+  const int mode = blockIdx.y, n = nsolve[mode];
+  const int nblk = (n + B - 1) / B;
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) nblk * ld * B) return;
+  const int J = (int) (idx / ((size_t) ld * B));
+  const int rem = (int) (idx % ((size_t) ld * B)), q = rem / B, c = rem % B;
+  const int j = J * B + c;
+  X[(size_t) mode * sw_stride + idx] =
+      (j < n) ? U[(size_t) mode * band_stride + (size_t) j * ld + q] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Sweep
+// ---------------------------------------------------------------------------
+constexpr int SW_NW = 8;   // wavefronts per sweep workgroup
+
+template <int B, int QS, int NR>
+struct Sweep {
+  static constexpr int NT = SW_NW * 64;
+  double v[QS];            // this wave's rows q = w*QS + k of the block, lane = column
+  double bin[NR];          // right-hand side of column lane
+};
+
+template <int B, int QS, int NR>
+__device__ __forceinline__ void sweep_load (Sweep<B, QS, NR>& S,
+                                            const double* __restrict__ X,
+                                            const double* __restrict__ in,
+                                            int nmax, int ld, int n, int J,
+                                            int reversed, int w, int lane)
+{
+  const int c = lane & (B - 1);
+  const double* blk = X + (size_t) J * ld * B + (size_t) w * QS * B + c;
+  if (w * QS + QS <= ld) {          // wave-uniform: all rows inside the band
+#pragma unroll
+    for (int k = 0; k < QS; k++) S.v[k] = blk[k * B];
+  } else {                          // last wave: rows past the band re-read row ld-1
+#pragma unroll
+    for (int k = 0; k < QS; k++) S.v[k] = blk[min (k, ld - 1 - w * QS) * B];
+  }
+  const int j = min (J * B + c, n - 1);
+  const int gi = reversed ? n - 1 - j : j;
+#pragma unroll
+  for (int r = 0; r < NR; r++) S.bin[r] = in[(size_t) r * nmax + gi];
+}
+
+template <int B, int QS, int NR>
+__device__ __forceinline__ void sweep_step (const Sweep<B, QS, NR>& S, double* win,
+                                            int W, double* part, double* rpriv,
+                                            double* part2, double* out, int nmax,
+                                            int ld, int n, int J, int reversed,
+                                            int w, int lane)
+{
+  const int kd = ld - 1, c = lane, j0 = J * B, Wm = W - 1;
+  // far part: rows above the block, q < kd - c.  The window holds every
+  // row twice (slots s and s+W), so the QS consecutive rows of this wave
+  // are read at consecutive addresses from slot (first row mod W).
+  double acc[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc[r] = 0.0;
+  const int s0 = (j0 + c - kd + w * QS) & Wm;         // row j0+c-kd+q0 (may be < 0)
+  const double* wr = win + s0;
+  if (w * QS + QS - 1 < kd - (B - 1)) {               // wave-uniform: all rows far
+#pragma unroll
+    for (int k = 0; k < QS; k++)
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc[r] += S.v[k] * wr[r * 2 * W + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < QS; k++) {
+      const int q = w * QS + k;
+      const double u = (q < kd - c) ? S.v[k] : 0.0;
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc[r] += u * wr[r * 2 * W + k];
+    }
+  }
+  if (c < B)
+#pragma unroll
+    for (int r = 0; r < NR; r++) part[(w * B + c) * NR + r] = acc[r];
+  __syncthreads ();
+  // triangle rows q in [kd-B+1, kd]: the waves holding them form the full
+  // r vector privately, then apply their rows of inv(W_JJ)^T
+  const bool tri = (w * QS + QS - 1 >= kd - (B - 1)) && (w * QS <= kd);
+  double acc2[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) acc2[r] = 0.0;
+  if (tri) {
+    if (c < B)
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        double rr = S.bin[r];
+#pragma unroll
+        for (int ww = 0; ww < SW_NW; ww++) rr -= part[(ww * B + c) * NR + r];
+        rpriv[(w * B + c) * NR + r] = rr;
+      }
+#pragma unroll
+    for (int k = 0; k < QS; k++) {
+      const int q = w * QS + k, t = c - kd + q;
+      const bool ok = t >= 0 && q <= kd && c < B;
+      const double u = ok ? S.v[k] : 0.0;
+      const int tc = min (max (t, 0), B - 1);
+#pragma unroll
+      for (int r = 0; r < NR; r++) acc2[r] += u * rpriv[(w * B + tc) * NR + r];
+    }
+  }
+  if (c < B)
+#pragma unroll
+    for (int r = 0; r < NR; r++) part2[(w * B + c) * NR + r] = acc2[r];
+  __syncthreads ();
+  if (w == 0 && c < B && j0 + c < n) {
+    const int gi = reversed ? n - 1 - (j0 + c) : j0 + c;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      double y = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < SW_NW; ww++) y += part2[(ww * B + c) * NR + r];
+      const int slot = (j0 + c) & Wm;
+      win[r * 2 * W + slot] = y;
+      win[r * 2 * W + slot + W] = y;
+      out[(size_t) r * nmax + gi] = y;
+    }
+  }
+  __syncthreads ();
+}
+
+template <int B, int QS, int NR>
+__global__ __launch_bounds__(SW_NW * 64) void k_band_sweeps (
+    const double* __restrict__ XF, const double* __restrict__ XR,
+    size_t sw_stride, int ld, int W, const int* __restrict__ nsolve,
+    double* rhs, int nmax)
+{
+  extern __shared__ double sm[];
+  const int mode = blockIdx.x, n = nsolve[mode];
+  if (n == 0) return;
+  double* win = sm;                          // [NR][2W] (each row stored twice)
+  double* part = win + 2 * NR * W;           // [NW][B][NR]
+  double* rpriv = part + SW_NW * B * NR;     // [NW][B][NR]
+  double* part2 = rpriv + SW_NW * B * NR;    // [NW][B][NR]
+  double* bg = rhs + (size_t) mode * NR * nmax;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane (tid >> 6);
+  const int nblk = (n + B - 1) / B;
+
+  for (int pass = 0; pass < 2; pass++) {
+    const double* X = (pass == 0 ? XF : XR) + (size_t) mode * sw_stride;
+    for (int i = tid; i < 2 * NR * W; i += SW_NW * 64) win[i] = 0.0;
+    __syncthreads ();
+    // the next block's rows are loaded while the current one is processed;
+    // prefetches are unconditional (block index clamped) so waiting for the
+    // current block never drains them
+    Sweep<B, QS, NR> sa, sb;
+    sweep_load (sa, X, bg, nmax, ld, n, 0, pass, w, lane);
+    for (int J = 0; J < nblk; J += 2) {
+      sweep_load (sb, X, bg, nmax, ld, n, min (J + 1, nblk - 1), pass, w, lane);
+      sweep_step (sa, win, W, part, rpriv, part2, bg, nmax, ld, n, J, pass, w, lane);
+      sweep_load (sa, X, bg, nmax, ld, n, min (J + 2, nblk - 1), pass, w, lane);
+      if (J + 1 < nblk)
+        sweep_step (sb, win, W, part, rpriv, part2, bg, nmax, ld, n, J + 1, pass, w, lane);
+    }
+  }
+}
+
+template <int B>
+size_t sweep_lds (int W, int NR)
+{
+  return sizeof (double) * (2 * (size_t) NR * W + 3 * (size_t) SW_NW * B * NR);
+}
+
+int window (int nband, int B)
+{
+  int W = 1;
+  while (W < nband - 1 + B) W <<= 1;
+  return W;
+}
+
+template <int B, int QS>
+int launch_sweeps (const sx_msys* s, hipStream_t st)
+{
+  const int W = window (s->m->nband, B);
+  const size_t lds = sweep_lds<B> (W, 2);
+  auto kern = k_band_sweeps<B, QS, 2>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (s->nmodes), dim3 (SW_NW * 64), lds, st, s->d_swF,
+                      s->d_swR, s->sw_stride, s->m->nband, W, s->d_nsolve, s->d_rhs,
+                      s->nmax);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// rows per wavefront, rounded up to a multiple of 4
+#define SX_QS_CASES(B) \
+  switch (qs) { \
+  case 4: return launch_sweeps<B, 4> (s, st);   case 8: return launch_sweeps<B, 8> (s, st); \
+  case 12: return launch_sweeps<B, 12> (s, st); case 16: return launch_sweeps<B, 16> (s, st); \
+  case 20: return launch_sweeps<B, 20> (s, st); case 24: return launch_sweeps<B, 24> (s, st); \
+  case 28: return launch_sweeps<B, 28> (s, st); case 32: return launch_sweeps<B, 32> (s, st); \
+  case 36: return launch_sweeps<B, 36> (s, st); case 40: return launch_sweeps<B, 40> (s, st); \
+  case 44: return launch_sweeps<B, 44> (s, st); case 48: return launch_sweeps<B, 48> (s, st); \
+  case 52: return launch_sweeps<B, 52> (s, st); case 56: return launch_sweeps<B, 56> (s, st); \
+  case 60: return launch_sweeps<B, 60> (s, st); case 64: return launch_sweeps<B, 64> (s, st); \
+  case 72: return launch_sweeps<B, 72> (s, st); case 80: return launch_sweeps<B, 80> (s, st); \
+  case 88: return launch_sweeps<B, 88> (s, st); case 96: return launch_sweeps<B, 96> (s, st); \
+  default: break; }
+
+}  // namespace
+
+// Block size: the largest power of two <= 64 that keeps the inverted
+// diagonal triangles inside the band (B <= kd+1).  SX_BAND_BLOCK in the
+// environment caps it (used by the tuner).
+int band_block (int nband)
+{
+  int cap = 64;
+  if (const char* e = std::getenv ("SX_BAND_BLOCK")) cap = std::max (1, std::atoi (e));
+  int B = 64;
+  while (B > 1 && (B > nband || B > cap)) B >>= 1;
+  return B;
+}
+
+static int rows_per_wave (int nband)
+{
+  int qs = (nband + SW_NW - 1) / SW_NW;
+  qs = (qs + 3) / 4 * 4;
+  if (qs > 64) qs = (qs + 7) / 8 * 8;
+  return qs;
+}
+
+size_t band_sweep_bytes_per_mode (int nmax, int nband)
+{
+  const int B = band_block (nband);
+  return (size_t) ((nmax + B - 1) / B) * nband * B;
+}
+
+int band_factor (sx_msys* s, hipStream_t st, int* d_status)
+{
+  const int ld = s->m->nband;
+  hipLaunchKernelGGL (k_band_factor, dim3 (s->nmodes), dim3 (1024),
+                      sizeof (double) * std::max (ld, 1), st, s->d_bandU,
+                      s->band_stride, ld, s->d_nsolve, d_status);
+  SX_HIP (hipGetLastError ());
+  const int B = band_block (ld);
+  const int nblk = (s->nmax + B - 1) / B;
+  double* V = nullptr;
+  SX_HIP (hipMalloc (&V, sizeof (double) * s->nmodes * s->band_stride));
+  SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * s->nmodes * s->band_stride, st));
+  hipLaunchKernelGGL (k_band_reverse,
+                      dim3 ((unsigned) ((s->band_stride + 255) / 256), s->nmodes),
+                      dim3 (256), 0, st, s->d_bandU, V, s->band_stride, ld, s->d_nsolve);
+  if (nblk > 0) {
+    hipLaunchKernelGGL (k_band_tinv, dim3 (nblk, s->nmodes), dim3 (64), 0, st,
+                        s->d_bandU, s->band_stride, ld, B, s->d_nsolve);
+    hipLaunchKernelGGL (k_band_tinv, dim3 (nblk, s->nmodes), dim3 (64), 0, st, V,
+                        s->band_stride, ld, B, s->d_nsolve);
+  }
+  SX_HIP (hipGetLastError ());
+  s->sw_block = B;
+  s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
+  SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
+  SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * s->nmodes * s->sw_stride));
+  const unsigned gx = (unsigned) ((s->sw_stride + 255) / 256);
+  hipLaunchKernelGGL (k_band_blockify, dim3 (gx, s->nmodes), dim3 (256), 0, st,
+                      s->d_bandU, s->d_swF, s->band_stride, s->sw_stride, ld, B,
+                      s->d_nsolve);
+  hipLaunchKernelGGL (k_band_blockify, dim3 (gx, s->nmodes), dim3 (256), 0, st, V,
+                      s->d_swR, s->band_stride, s->sw_stride, ld, B, s->d_nsolve);
+  SX_HIP (hipGetLastError ());
+  SX_HIP (hipStreamSynchronize (st));
+  (void) hipFree (V);
+  return SX_OK;
+}
+
+int band_solve (const sx_msys* s, hipStream_t st)
+{
+  const int qs = rows_per_wave (s->m->nband);
+  switch (s->sw_block) {
+  case 64: SX_QS_CASES (64) break;
+  case 32:
+    if (qs == 4) return launch_sweeps<32, 4> (s, st);
+    if (qs == 8) return launch_sweeps<32, 8> (s, st);
+    break;
+  case 16: return launch_sweeps<16, 4> (s, st);
+  case 8: return launch_sweeps<8, 4> (s, st);
+  case 4: return launch_sweeps<4, 4> (s, st);
+  case 2: return launch_sweeps<2, 4> (s, st);
+  case 1: return launch_sweeps<1, 4> (s, st);
+  default: break;
+  }
+  set_error ("band solver: band width " + std::to_string (s->m->nband) +
+             " outside the instantiated range (nband <= 768)");
+  return SX_ERR_LIMIT;
+}
+
+}  // namespace sx

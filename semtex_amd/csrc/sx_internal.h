@@ -38,6 +38,18 @@ void side_area (int np, const double* x, const double* y, int side,
 
 }  // namespace sx
 
+struct sx_msys;
+namespace sx {
+// Banded FP64 Cholesky of every mode's band, the offset-major sweep copies
+// of the factor and of its reversal (with inverted diagonal blocks), and
+// the two-sweep solve of the right-hand sides in d_rhs ([mode][2][nmax])
+// -- sx_band.hip.
+int    band_block (int nband);
+size_t band_sweep_bytes_per_mode (int nmax, int nband);
+int    band_factor (sx_msys* s, hipStream_t st, int* d_status);
+int    band_solve (const sx_msys* s, hipStream_t st);
+}  // namespace sx
+
 struct sx_mesh {
   int np = 0, nel = 0, cyl = 0, npnp = 0, next = 0, nint = 0;
   int nglobal = 0, nsolve = 0, nband = 0, kd = 0, nbound = 0, nclass = 0;
@@ -80,7 +92,11 @@ struct sx_msys {
   float* d_Bm32 = nullptr;     // FP32 copies for SX_MIXED
   float* d_Hinv32 = nullptr;
   double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband
-  double* d_bandR = nullptr;   // [nmodes][band_stride] reversed row copy
+                               // (assembly + factorisation; freed after set-up)
+  int sw_block = 0;            // sweep block size B
+  size_t sw_stride = 0;        // doubles per mode of each sweep copy
+  double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
+  double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks
   double* d_rhs = nullptr;     // [nmodes][2][nmax] solve workspace
   double* d_G = nullptr;       // [2*nmodes][nglobal] essential values
   double* d_work = nullptr;    // mixed-precision workspace

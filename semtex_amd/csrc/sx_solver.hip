@@ -5,13 +5,12 @@
 //   k_elem_sc       elemental Helmholtz matrix + static condensation per
 //                   (mode, element class)  [Element::HelmholtzSC]
 //   k_band_assemble deterministic direct-stiffness assembly of the band
-//   k_band_factor   FP64 banded Cholesky, one workgroup per mode [dpbtrf]
-//   k_band_reverse  reversed row copy of the factor for the backward sweep
+//   band_factor     FP64 banded Cholesky + inverted diagonal blocks (sx_band.hip)
 // Solve (the timed path, per call):
 //   k_essential     essential BC values into global numbering
 //   k_condense      -(M f + H g), emap gather, F_b -= hbi hii^-1 F_i
 //   k_assemble      direct-stiffness sum of F_b (+ natural BC integrals)
-//   k_band_solve    FP64 U^T U x = b, one workgroup per mode, both RHS
+//   band_solve      FP64 U^T U x = b, one workgroup per mode (sx_band.hip)
 //   k_recover       u_i = hii^-1 F_i - (hbi hii^-1)^T u_b, essential insert
 #include <algorithm>
 #include <cmath>
@@ -204,65 +203,6 @@ __global__ void k_band_assemble (int npairs, const long long* __restrict__ ppos,
   band[(size_t) mode * band_stride + ppos[p]] = s;
 }
 
-// Right-looking banded Cholesky ('U', dpbtrf semantics), one workgroup per
-// mode; rows of the pivot are staged in LDS, the trailing window is
-// updated column by column with one wavefront per column.
-__global__ __launch_bounds__(1024) void k_band_factor (
-    double* band, size_t band_stride, int ld, const int* __restrict__ nsolve,
-    int* status)
-{
-  extern __shared__ double row[];
-  __shared__ double s_dinv;
-  __shared__ int s_bad;
-  const int mode = blockIdx.x, n = nsolve[mode], kd = ld - 1;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6,
-            nw = blockDim.x >> 6;
-  double* A = band + (size_t) mode * band_stride;
-  if (tid == 0) s_bad = 0;
-  for (int j = 0; j < n; j++) {
-    if (tid == 0) {
-      const double d = A[(size_t) j * ld + kd];
-      if (!(d > 0.0)) { s_bad = j + 1; s_dinv = 0.0; }
-      else { const double r = std::sqrt (d); A[(size_t) j * ld + kd] = r; s_dinv = 1.0 / r; }
-    }
-    __syncthreads ();
-    if (s_bad) break;
-    const double dinv = s_dinv;
-    const int last = min (n - 1, j + kd), w = last - j;
-    for (int t = tid; t < w; t += blockDim.x) {
-      const int i = j + 1 + t;
-      const size_t a = (size_t) i * ld + kd + j - i;
-      const double v = A[a] * dinv;
-      A[a] = v;
-      row[t] = v;
-    }
-    __syncthreads ();
-    for (int cl = wave; cl < w; cl += nw) {
-      const int l = j + 1 + cl;
-      const double rl = row[cl];
-      double* col = A + (size_t) l * ld + kd - l;
-      for (int ii = lane; ii <= cl; ii += 64) col[j + 1 + ii] -= row[ii] * rl;
-    }
-    __syncthreads ();
-  }
-  if (tid == 0 && s_bad) status[mode] = s_bad;
-}
-
-// R[i'][o] = U(i, i+kd-o), i = n-1-i': the backward sweep U x = y becomes
-// a forward sweep on R with reversed vectors.
-__global__ void k_band_reverse (const double* __restrict__ U, double* R,
-                                size_t band_stride, int ld,
-                                const int* __restrict__ nsolve)
-{
-  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
-  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t) n * ld) return;
-  const int ip = (int) (idx / ld), o = (int) (idx % ld);
-  const int i = n - 1 - ip, jj = i + kd - o;
-  const double* Um = U + (size_t) mode * band_stride;
-  R[(size_t) mode * band_stride + idx] = (jj < n) ? Um[(size_t) jj * ld + o] : 0.0;
-}
-
 // ---------------------------------------------------------------------------
 // Solve path
 // ---------------------------------------------------------------------------
@@ -380,100 +320,6 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
   rhs[((size_t) mode * 2 + rr) * nmax + g] = s;
 }
 
-// FP64 banded solve U^T U x = b for both RHS of one mode per workgroup.
-// Blocks of 64 unknowns: 16 wavefronts stream the block's 64 contiguous
-// band columns (coalesced), form the dot products with the already solved
-// part of the vector (held in LDS), and stage the 64x64 triangle in LDS;
-// wavefront 0 then resolves the triangle with lane-broadcasts.  The second
-// pass runs the same sweep on the reversed copy R (backward substitution).
-template <int NU>
-__global__ __launch_bounds__(1024) void k_band_solve (
-    const double* __restrict__ bandU, const double* __restrict__ bandR,
-    size_t band_stride, int ld, const int* __restrict__ nsolve, double* rhs,
-    int nmax)
-{
-  extern __shared__ double smem[];
-  const int mode = blockIdx.x, n = nsolve[mode];
-  if (n == 0) return;
-  const int kd = ld - 1, vlen = n + kd;
-  double* yp0 = smem;
-  double* yp1 = yp0 + vlen;
-  double* tri = yp1 + vlen;          // [64][64]: tri[t*64+c] = U(j0+t, j0+c)
-  double* part = tri + 64 * 64;      // [64][2]
-  double* b0 = rhs + ((size_t) mode * 2) * nmax;
-  double* b1 = b0 + nmax;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nblk = (n + 63) >> 6;
-
-  for (int pass = 0; pass < 2; pass++) {
-    const double* A = (pass == 0 ? bandU : bandR) + (size_t) mode * band_stride;
-    for (int i = tid; i < 2 * vlen; i += blockDim.x) smem[i] = 0.0;
-    __syncthreads ();
-    for (int blk = 0; blk < nblk; blk++) {
-      const int j0 = blk << 6;
-      for (int i = tid; i < 64 * 64; i += blockDim.x) tri[i] = 0.0;
-      __syncthreads ();
-      double v[4][NU];
-#pragma unroll
-      for (int cc = 0; cc < 4; cc++) {
-        const int c = wave * 4 + cc, j = j0 + c;
-        const double* col = A + (size_t) j * ld;
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
-          const int o = lane + 64 * u;
-          v[cc][u] = (j < n && o < ld) ? col[o] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int cc = 0; cc < 4; cc++) {
-        const int c = wave * 4 + cc, j = j0 + c;
-        double a0 = 0.0, a1 = 0.0;
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
-          const int o = lane + 64 * u;
-          if (o < kd - c) {
-            const int idx = j + o;          // row j-kd+o, padded by kd
-            a0 += v[cc][u] * yp0[idx];
-            a1 += v[cc][u] * yp1[idx];
-          } else if (o <= kd && j < n) {
-            tri[(c - kd + o) * 64 + c] = v[cc][u];
-          }
-        }
-        a0 = wave_sum (a0);
-        a1 = wave_sum (a1);
-        if (lane == 0) { part[c * 2] = a0; part[c * 2 + 1] = a1; }
-      }
-      __syncthreads ();
-      if (wave == 0) {
-        const int c = lane, j = j0 + c;
-        const bool valid = j < n;
-        const int gi = (pass == 0) ? j : n - 1 - j;
-        double a0 = valid ? b0[gi] - part[c * 2] : 0.0;
-        double a1 = valid ? b1[gi] - part[c * 2 + 1] : 0.0;
-        const double dinv = valid ? 1.0 / tri[c * 64 + c] : 0.0;
-        const int tmax = min (64, n - j0);
-        for (int t = 0; t < tmax; t++) {
-          const double y0 = readlane_d (a0 * dinv, t);
-          const double y1 = readlane_d (a1 * dinv, t);
-          if (c > t) {
-            const double L = tri[t * 64 + c];
-            a0 -= L * y0;
-            a1 -= L * y1;
-          }
-        }
-        if (valid) {
-          const double y0 = a0 * dinv, y1 = a1 * dinv;
-          yp0[kd + j] = y0;
-          yp1[kd + j] = y1;
-          b0[gi] = y0;
-          b1[gi] = y1;
-        }
-      }
-      __syncthreads ();
-    }
-  }
-}
-
 // One wavefront per (element, solved plane): interior recovery and
 // essential-BC insertion (Element::global2localSC src/element.cpp:368-407,
 // Field::setEssential src/field.cpp:1105-1124).
@@ -538,44 +384,6 @@ __global__ __launch_bounds__(64) void k_helm_op (
 }
 
 // ---------------------------------------------------------------------------
-size_t solve_lds_bytes (int n, int ld) { return (2 * (size_t) (n + ld - 1) + 64 * 64 + 128) * sizeof (double); }
-constexpr size_t LDS_MAX = 160 * 1024;
-
-template <int NU>
-int launch_band_solve_nu (const sx_msys* s, hipStream_t st)
-{
-  const size_t lds = solve_lds_bytes (s->nmax, s->m->nband);
-  SX_HIP (hipFuncSetAttribute ((const void*) k_band_solve<NU>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  hipLaunchKernelGGL (k_band_solve<NU>, dim3 (s->nmodes), dim3 (1024), lds, st,
-                      s->d_bandU, s->d_bandR, s->band_stride, s->m->nband,
-                      s->d_nsolve, s->d_rhs, s->nmax);
-  SX_HIP (hipGetLastError ());
-  return SX_OK;
-}
-
-int launch_band_solve (const sx_msys* s, hipStream_t st)
-{
-  const int nu = (s->m->nband + 63) / 64;
-  switch (nu) {
-  case 1: return launch_band_solve_nu<1> (s, st);
-  case 2: return launch_band_solve_nu<2> (s, st);
-  case 3: return launch_band_solve_nu<3> (s, st);
-  case 4: return launch_band_solve_nu<4> (s, st);
-  case 5: return launch_band_solve_nu<5> (s, st);
-  case 6: return launch_band_solve_nu<6> (s, st);
-  case 7: return launch_band_solve_nu<7> (s, st);
-  case 8: return launch_band_solve_nu<8> (s, st);
-  case 9: return launch_band_solve_nu<9> (s, st);
-  case 10: return launch_band_solve_nu<10> (s, st);
-  case 11: return launch_band_solve_nu<11> (s, st);
-  case 12: return launch_band_solve_nu<12> (s, st);
-  default:
-    set_error ("band width above 768 not supported by the band solver");
-    return SX_ERR_LIMIT;
-  }
-}
-
 int n_solved_planes (int nz, int base_mode) { return (base_mode == 0 && nz > 1) ? nz - 1 : nz; }
 
 // Optional per-phase event timing of the solve (sx_timing_enable).
@@ -642,7 +450,7 @@ extern "C" int sx_msys_destroy (sx_msys_t s)
 {
   if (!s) return SX_OK;
   void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Bm32,
-                  s->d_Hinv32, s->d_bandU, s->d_bandR, s->d_rhs, s->d_G,
+                  s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
                   s->d_work};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete s;
@@ -686,11 +494,6 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
   }
   s->nmax = *std::max_element (s->nsolve.begin (), s->nsolve.end ());
   s->band_stride = (size_t) std::max (s->nmax, 1) * ld;
-  if (solve_lds_bytes (s->nmax, ld) > LDS_MAX) {
-    sx_msys_destroy (s);
-    set_error ("sx_msys_create: nsolve + nband too large for the LDS-resident band solver");
-    return SX_ERR_LIMIT;
-  }
 
   int rc = SX_OK;
   auto fail = [&] (int r) { sx_msys_destroy (s); return r; };
@@ -703,7 +506,6 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
   TRY (hipMalloc (&s->d_Bm, sizeof (double) * std::max<size_t> (nmc * next * nint, 1)));
   TRY (hipMalloc (&s->d_Hinv, sizeof (double) * std::max<size_t> (nmc * nint * nint, 1)));
   TRY (hipMalloc (&s->d_bandU, sizeof (double) * num_modes * s->band_stride));
-  TRY (hipMalloc (&s->d_bandR, sizeof (double) * num_modes * s->band_stride));
   TRY (hipMalloc (&s->d_rhs, sizeof (double) * num_modes * 2 * std::max (s->nmax, 1)));
   TRY (hipMalloc (&s->d_G, sizeof (double) * 2 * num_modes * m->nglobal));
   TRY (hipMemsetAsync (s->d_rhs, 0, sizeof (double) * num_modes * 2 * std::max (s->nmax, 1), st));
@@ -727,20 +529,14 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                         m->d_psrc, d_S, (size_t) m->nclass * next * next, s->d_bandU,
                         s->band_stride, s->d_nsolve);
   TRY (hipGetLastError ());
-  hipLaunchKernelGGL (k_band_factor, dim3 (num_modes), dim3 (1024),
-                      sizeof (double) * std::max (ld, 1), st, s->d_bandU,
-                      s->band_stride, ld, s->d_nsolve, d_status);
-  TRY (hipGetLastError ());
-  hipLaunchKernelGGL (k_band_reverse,
-                      dim3 ((unsigned) ((s->band_stride + 255) / 256), num_modes),
-                      dim3 (256), 0, st, s->d_bandU, s->d_bandR, s->band_stride, ld,
-                      s->d_nsolve);
-  TRY (hipGetLastError ());
+  if ((rc = band_factor (s, st, d_status))) return fail (rc);
   std::vector<int> status (num_modes + 1);
   TRY (hipMemcpyAsync (status.data (), d_status, sizeof (int) * (num_modes + 1),
                        hipMemcpyDeviceToHost, st));
   TRY (hipStreamSynchronize (st));
   (void) hipFree (d_ws); (void) hipFree (d_S); (void) hipFree (d_status);
+  (void) hipFree (s->d_bandU);
+  s->d_bandU = nullptr;
   if (status[num_modes]) {
     set_error ("sx_msys_create: matrix hii is not positive definite");
     return fail (SX_ERR_FACTOR);
@@ -813,7 +609,7 @@ extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
   SX_HIP (hipGetLastError ());
   {
     PhaseScope ps (3, st);
-    if ((rc = launch_band_solve (s, st))) return rc;
+    if ((rc = band_solve (s, st))) return rc;
   }
   {
     PhaseScope ps (4, st);
