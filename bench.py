@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: batched per-Fourier-mode Helmholtz solve on one MI355X.
+"""Benchmark: batched per-Fourier-mode Helmholtz solve and DNS step on one
+MI355X.
 
-Workload (default): 16x16 quadrilateral elements on [0,2pi]x[-1,1], N_P=10,
+Helmholtz workload: 16x16 quadrilateral elements on [0,2pi]x[-1,1], N_P=10,
 N_Z=64 planes = 32 Fourier modes (beta=1), lambda2=1 so mode k solves with
 1+k^2; u=0 on y=+-1, periodic in x; seeded Gaussian forcing with amplitude
 (1+k)^-2 per mode.  One step = one Field::solve of all 63 planes (32
 mode-solves: mode 0 real, modes 1..31 Re+Im; the Nyquist plane is not
-solved), inputs resident in HBM.  FP64 throughout.
+solved), inputs resident in HBM.  Run as the FP64 path and as the mixed
+path (FP32 elemental work, FP64 band solve, FP64 residual refinement).
 
-Prints ONE JSON line with the metric, a roofline entry for the dominant
-kernel (the banded FP64 triangular sweeps, HBM-bound) measured with HIP
-events on the solve stream, and the CPU oracle timed on a bounded sample of
-the same workload.
+DNS workload: forced periodic channel, 12x24 elements on [0,4pi]x[-1,1],
+N_P=8, N_Z=128 (Lz=2pi), Re=2000, dt=1e-3, second-order stiffly-stable
+splitting with skew-symmetric advection, Poiseuille + seeded 1e-2
+perturbation; 100 steps timed.
+
+Prints ONE JSON line: the Helmholtz rate as `value`, both precision
+variants, the DNS rate, a roofline entry for the dominant kernel (the
+banded FP64 triangular sweeps, HBM-bound) measured with HIP events on the
+solve stream, and the CPU oracle timed on bounded samples of the same
+workloads.
 """
 from __future__ import annotations
 
@@ -40,6 +48,12 @@ def parse():
                    help="bounded CPU-baseline sample length")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-parity", action="store_true")
+    p.add_argument("--dns-steps", type=int, default=100)
+    p.add_argument("--dns-warmup", type=int, default=3)
+    p.add_argument("--no-dns", action="store_true")
+    p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--mixed-tol", type=float, default=1e-11,
+                   help="relative correction size that ends the mixed refinement")
     return p.parse_args()
 
 
@@ -71,6 +85,114 @@ def cpu_baseline(P, seconds):
                       f"{el:.1f} s, same mesh/numbering/forcing, 1 thread"}
 
 
+def cpu_baseline_dns(P, steps=3, timed_from=1):
+    """The DNS oracle (numpy + the C solve restatement) on one core: the
+    same channel, initial field and step; systems are factorised first and
+    excluded, step 1 (first-order start) is untimed, steps 2..`steps` are
+    timed.  Returns the baseline entry and the oracle's fields after each
+    step for the per-step GPU parity check."""
+    from oracle.dns_oracle import DnsOracle
+    O = DnsOracle(P["mesh_u"], P["mesh_p"], P["nz"], P["beta"], P["kinvis"], P["dt"],
+                  const_force=P["const_force"], advection="skew")
+    t_setup = time.perf_counter()
+    for md in range(P["nz"] // 2):
+        O.system("p", 0.0, md)
+        for q in (1, 2):
+            O.system("u", {1: 1.0, 2: 1.5}[q] / (P["dt"] * P["kinvis"]), md)
+    t_setup = time.perf_counter() - t_setup
+    O.set_initial(P["u0"])
+    states, t_steps = [], []
+    for s in range(steps):
+        t0 = time.perf_counter()
+        O.step()
+        t_steps.append(time.perf_counter() - t0)
+        states.append([c.copy() for c in O.u] + [O.p.copy()])
+    el = sum(t_steps[timed_from:])
+    nt = steps - timed_from
+    return ({"value": nt / el, "unit": "DNS steps/s", "cores": 1, "kind": "port",
+             "sample": f"DNS oracle, full channel workload, steps {timed_from + 1}-{steps} "
+                       f"timed ({el:.1f} s); {3 * P['nz'] // 2} mode systems factorised "
+                       f"first ({t_setup:.1f} s, excluded); 1 thread"}, states)
+
+
+def helmholtz_leg(a, P, gm, precision, L, _lib, torch):
+    """Set up the modal systems, warm up, time `steps` solves; returns the
+    rate, the last solution and per-phase times (FP64 path only)."""
+    from semtex_amd.solver import SX_MIXED, Field, ModalMatrixSys
+    m, nz, nmodes = P["mesh"], P["nz"], P["nmodes"]
+    t0 = time.perf_counter()
+    M = ModalMatrixSys(P["lambda2"], P["beta"], 0, nmodes, gm, precision)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    F_src = torch.from_numpy(P["forcing"]).cuda()
+    u = Field(gm, nz)
+    fs = [F_src.clone() for _ in range(a.warmup + a.steps)]
+    torch.cuda.synchronize()
+    kw = dict(mixed_tol=a.mixed_tol, max_iter=20) if precision == SX_MIXED else {}
+    for i in range(a.warmup):
+        u.solve(fs[i], M, **kw)
+    torch.cuda.synchronize()
+    sol = u.data.clone()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t_wall = time.perf_counter()
+    e0.record()
+    for i in range(a.steps):
+        u.solve(fs[a.warmup + i], M, **kw)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = e0.elapsed_time(e1)
+    res = {"M": M, "u": u, "solution": sol, "ms_per_step": ms / a.steps,
+           "value": nmodes * a.steps / (ms / 1e3), "setup_s": setup_s, "wall_s": wall,
+           "iters": getattr(u, "last_iters", None)}
+    if precision != SX_MIXED:
+        fs_t = [F_src.clone() for _ in range(a.steps)]
+        torch.cuda.synchronize()
+        L.sx_timing_enable(1)
+        for i in range(a.steps):
+            u.solve(fs_t[i], M)
+        torch.cuda.synchronize()
+        pm = (C.c_double * _lib.SX_NPHASE)()
+        pc = (C.c_int * _lib.SX_NPHASE)()
+        _lib.check(L.sx_timing_read(pm, pc, _lib.SX_NPHASE), "sx_timing_read")
+        L.sx_timing_enable(0)
+        res["phases"] = {name: pm[i] / max(pc[i], 1) for i, name in enumerate(_lib.PHASES)}
+    return res
+
+
+def dns_leg(a, torch):
+    """DNS channel: set-up, `dns_warmup` untimed steps, `dns_steps` timed
+    steps from the same initial field."""
+    from semtex_amd.problems import dns_channel
+    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh
+    P = dns_channel()
+    mu = P["mesh_u"]
+    n = mu.nel * mu.npnp
+    print(f"# DNS mesh {mu.info['nx']}x{mu.info['ny']} np={mu.np} nz={P['nz']} "
+          f"nband u/p {mu.nband}/{P['mesh_p'].nband}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    gu, gp = GpuMesh(mu), GpuMesh(P["mesh_p"])
+    D = DNS(gu, gp, P["nz"], P["beta"], P["kinvis"], P["dt"], const_force=P["const_force"],
+            advection=SX_SKEW)
+    u0 = torch.zeros((3, P["nz"], D.stride), dtype=torch.float64)
+    u0[:, :, :n] = torch.from_numpy(P["u0"])
+    u0 = u0.cuda()
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    D.set_state(u0)
+    D.step(a.dns_warmup)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    D.step(a.dns_steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ok = all(bool(torch.isfinite(D.field(i)).all()) for i in range(4))
+    return {"P": P, "D": D, "u0": u0, "ms_per_step": ms / max(a.dns_steps, 1),
+            "value": a.dns_steps / (ms / 1e3), "setup_s": setup_s, "finite": ok}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -80,82 +202,65 @@ def main():
 
     from semtex_amd import _lib
     from semtex_amd.problems import helmholtz_fourier
-    from semtex_amd.solver import SX_FP64, Field, GpuMesh, ModalMatrixSys
+    from semtex_amd.solver import SX_FP64, SX_MIXED, GpuMesh
 
     assert torch.cuda.is_available(), "bench needs a GPU"
     L = _lib.load()
     P = helmholtz_fourier()
     m, nz, nmodes = P["mesh"], P["nz"], P["nmodes"]
+    n = m.nel * m.npnp
     print(f"# mesh {m.info['nx']}x{m.info['ny']} np={m.np} nz={nz} nglobal={m.nglobal} "
           f"nsolve={m.nsolve} nband={m.nband} ({m.info['ordering']})", file=sys.stderr, flush=True)
-
-    t0 = time.perf_counter()
     gm = GpuMesh(m)
-    M = ModalMatrixSys(P["lambda2"], P["beta"], 0, nmodes, gm, SX_FP64)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    print(f"# setup (geometry, condensation, band Cholesky of {nmodes} modes): "
-          f"{setup_s:.3f} s", file=sys.stderr, flush=True)
-
-    F_src = torch.from_numpy(P["forcing"]).cuda()
-    u = Field(gm, nz)
-    nbuf = a.warmup + a.steps
-    fs = [F_src.clone() for _ in range(nbuf)]
-    fs_t = [F_src.clone() for _ in range(a.steps)]
-    torch.cuda.synchronize()
-
-    for i in range(a.warmup):
-        u.solve(fs[i], M)
-    torch.cuda.synchronize()
+    f64 = helmholtz_leg(a, P, gm, SX_FP64, L, _lib, torch)
+    print(f"# FP64: {f64['value']:.1f} mode-solves/s, {f64['ms_per_step']:.4f} ms/step, "
+          f"setup {f64['setup_s']:.3f} s", file=sys.stderr, flush=True)
+    planes = [k for k in range(nz) if k != 1]
+    u64 = f64["solution"].cpu().numpy()[planes, :n]
 
     rel_err = None
     if not a.no_parity:
         from oracle import oracle as O
-        n = m.nel * m.npnp
-        ug = u.data.cpu().numpy()
         om = O.OracleMesh(m)
         errs = []
         for md in (0, 1, nmodes - 1):
             S = O.OracleMsys(om, P["lambda2"], (P["beta"] * md) ** 2)
             for k in ([0] if md == 0 else [2 * md, 2 * md + 1]):
                 uo = S.solve_plane(P["forcing"][k, :n])
-                errs.append(np.linalg.norm(ug[k, :n] - uo) / np.linalg.norm(uo))
+                errs.append(np.linalg.norm(f64["solution"][k, :n].cpu().numpy() - uo)
+                            / np.linalg.norm(uo))
         rel_err = float(max(errs))
         print(f"# parity vs oracle (modes 0,1,{nmodes-1}): max rel L2 {rel_err:.3e}",
               file=sys.stderr, flush=True)
 
-    # timed region: K solves, inputs resident in HBM
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t_wall = time.perf_counter()
-    e0.record()
-    for i in range(a.steps):
-        u.solve(fs[a.warmup + i], M)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall
-    ms = e0.elapsed_time(e1)
-    ms_per_step = ms / a.steps
-    value = nmodes * a.steps / (ms / 1e3)
-
-    # per-kernel timing (HIP events on the solve stream)
-    L.sx_timing_enable(1)
-    for i in range(a.steps):
-        u.solve(fs_t[i], M)
-    torch.cuda.synchronize()
-    pm = (C.c_double * _lib.SX_NPHASE)()
-    pc = (C.c_int * _lib.SX_NPHASE)()
-    _lib.check(L.sx_timing_read(pm, pc, _lib.SX_NPHASE), "sx_timing_read")
-    L.sx_timing_enable(0)
-    phases = {name: pm[i] / max(pc[i], 1) for i, name in enumerate(_lib.PHASES)}
+    variants = {"fp64": {"value": round(f64["value"], 2),
+                         "ms_per_step": round(f64["ms_per_step"], 5),
+                         "rel_err_vs_oracle": rel_err}}
+    value, chosen = f64["value"], "fp64"
+    if not a.no_mixed:
+        mx = helmholtz_leg(a, P, gm, SX_MIXED, L, _lib, torch)
+        umx = mx["solution"].cpu().numpy()[planes, :n]
+        err_mx = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
+        variants["mixed"] = {"value": round(mx["value"], 2),
+                             "ms_per_step": round(mx["ms_per_step"], 5),
+                             "refinement_iters": mx["iters"], "refine_tol": a.mixed_tol,
+                             "rel_err_vs_fp64": err_mx,
+                             "speedup_vs_fp64": round(mx["value"] / f64["value"], 3)}
+        print(f"# mixed: {mx['value']:.1f} mode-solves/s, {mx['iters']} refinement passes, "
+              f"rel err vs FP64 {err_mx:.2e}", file=sys.stderr, flush=True)
+        if err_mx <= 1e-10 and mx["value"] > value:
+            value, chosen = mx["value"], "mixed"
+        del mx
 
     # algorithmic bytes of one band-solve launch: every mode's factor once
     # (8 nsolve nband) + its right-hand sides read and solutions written
+    M = f64["M"]
     band_bytes = 0
     for md in range(nmodes):
         ns = M.nsolve(md)
         nrhs = 1 if md == 0 else 2
         band_bytes += 8 * ns * m.nband + 16 * ns * nrhs
+    phases = f64["phases"]
     band_ms = phases["band_solve"]
     achieved = band_bytes / (band_ms * 1e-3) / 1e9
 
@@ -166,7 +271,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": round(nmodes / value * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -175,13 +280,14 @@ def main():
         "config": {
             "workload": "batched 2.5D Helmholtz, 16x16 elements on [0,2pi]x[-1,1], N_P=10, "
                         "N_Z=64 (32 modes, beta=1, lambda2=1+k^2), Dirichlet y=+-1, periodic x, "
-                        "seeded Gaussian forcing ~(1+k)^-2; FP64 static condensation + "
+                        "seeded Gaussian forcing ~(1+k)^-2; static condensation + "
                         "banded Cholesky",
-            "precision": "fp64",
+            "precision": chosen,
             "mode_solves_per_step": nmodes,
             "nglobal": m.nglobal, "nsolve": m.nsolve, "nband": m.nband,
             "ordering": m.info["ordering"],
         },
+        "variants": variants,
         "roofline": {
             "kernel": "k_band_solve",
             "bound": "hbm",
@@ -194,10 +300,44 @@ def main():
             "avg_ms": round(band_ms, 5),
         },
         "phases_ms": {k: round(v, 5) for k, v in phases.items()},
-        "setup_s": round(setup_s, 3),
+        "setup_s": round(f64["setup_s"], 3),
         "rel_err_vs_oracle": rel_err,
-        "wall_s_timed": round(wall, 4),
+        "wall_s_timed": round(f64["wall_s"], 4),
     }
+    del f64
+
+    if not a.no_dns:
+        dn = dns_leg(a, torch)
+        dns = {"value": round(dn["value"], 3), "unit": "DNS steps/s",
+               "ms_per_step": round(dn["ms_per_step"], 4), "steps": a.dns_steps,
+               "warmup": a.dns_warmup, "setup_s": round(dn["setup_s"], 3),
+               "finite": dn["finite"],
+               "config": "forced channel 12x24 elements on [0,4pi]x[-1,1], N_P=8, N_Z=128, "
+                         "Lz=2pi, Re=2000, dt=1e-3, SS2 splitting, skew-symmetric advection, "
+                         "Poiseuille + U(-1e-2,1e-2) perturbation (seeded), FP64"}
+        if not a.no_cpu_baseline:
+            cb, states = cpu_baseline_dns(dn["P"])
+            cb["value"] = round(cb["value"], 4)
+            dns["cpu_baseline"] = cb
+            dns["gpu_over_cpu"] = round(dn["value"] / cb["value"], 1)
+            if not a.no_parity:
+                D, S = dn["D"], dn["D"].stride
+                nn = dn["P"]["mesh_u"].nel * dn["P"]["mesh_u"].npnp
+                D.set_state(dn["u0"])
+                pl = [k for k in range(dn["P"]["nz"]) if k != 1]
+                worst = 0.0
+                for st in states:
+                    D.step(1)
+                    for i in range(4):
+                        g = D.field(i).cpu().numpy()[pl, :nn]
+                        o = st[i][pl]
+                        worst = max(worst, float(np.linalg.norm(g - o) / np.linalg.norm(o)))
+                dns["rel_err_vs_oracle_per_step"] = worst
+                dns["parity_steps"] = len(states)
+        out["dns"] = dns
+        print(f"# DNS: {dn['value']:.2f} steps/s ({dn['ms_per_step']:.3f} ms/step)",
+              file=sys.stderr, flush=True)
+
     if not a.no_cpu_baseline:
         cb = cpu_baseline(P, a.cpu_seconds)
         cb["value"] = round(cb["value"], 2)

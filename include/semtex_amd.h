@@ -113,6 +113,39 @@ int sx_field_solve_host (sx_msys_t s, int nz, int plane_stride,
 int sx_helmholtz_op (sx_msys_t s, int nz, int plane_stride, const double* d_u,
                      double* d_hu, void* stream);
 
+/* ------------------------------------------------------------------------
+ * DNS time step: integrate() (dns/integrate.cpp:66-201) for Cartesian
+ * 2D (nz == 1: u, v, p) or 2.5D (nz even >= 2: u, v, w, p) flows.
+ * mesh_u: velocity numbering (its essential segments are no-slip walls
+ * with zero velocity), mesh_p: pressure numbering, same boundary segments
+ * in the same order; natural segments of mesh_p carry the computed
+ * high-order pressure condition (BCmgr::maintainFourier/evaluateCNBCp,
+ * src/bcmgr.cpp:1037-1335).  advection: SX_SKEW (skewSymmetric, "-S") or
+ * SX_ALTSKEW (altSkewSymmetric, the driver default), dns/nonlinear.cpp.
+ * const_force: ConstForce CONST_X/Y/Z added to mode 0 (may be NULL).
+ * The first nord-1 steps use the first-order direct solve (the limit of
+ * the reference's start-up Jacobi-PCG, dns/integrate.cpp:340-353).
+ * ---------------------------------------------------------------------- */
+#define SX_SKEW    0
+#define SX_ALTSKEW 1
+typedef struct sx_dns* sx_dns_t;
+int sx_dns_create (sx_mesh_t mesh_u, sx_mesh_t mesh_p, int nz, double beta,
+                   double kinvis, double dt, const double* const_force,
+                   int advection, int nord, void* stream, sx_dns_t* out);
+int sx_dns_destroy (sx_dns_t d);
+/* Domain::restart (src/domain.cpp:123-157): physical velocity
+ * [ncom][nz][plane_stride] (device) -> forward transform, Nyquist plane
+ * zeroed; pressure and history cleared, step = 0. */
+int sx_dns_set_state (sx_dns_t d, const double* d_u_phys, void* stream);
+/* Advance nsteps steps of the integrate() loop body. */
+int sx_dns_step (sx_dns_t d, int nsteps, void* stream);
+/* Copy field `which` (0..ncom-1 velocity, ncom pressure) in Fourier space
+ * [nz][plane_stride] to device memory d_out.  Returns the current step
+ * count through *step (may be NULL). */
+int sx_dns_get (sx_dns_t d, int which, double* d_out, int* step, void* stream);
+/* Plane stride of the DNS fields (Geometry::planeSize()). */
+int sx_dns_plane_stride (sx_dns_t d);
+
 /* Per-kernel timing of sx_field_solve with HIP events recorded on the
  * solve's stream (no counterpart in the reference, whose only timer is the CPU
  * clock dclock(), veclib/util.c:68).  Phases: 0 essential BCs,

@@ -124,8 +124,9 @@ class DnsOracle:
         nc = self.ncom
         self.u = np.zeros((nc, nz, self.n))
         self.p = np.zeros((nz, self.n))
-        self.Us = np.zeros((nord, nc, nz, self.n))
-        self.Uf = np.zeros((nord, nc, nz, self.n))
+        # time levels as lists of arrays: rolling rotates the list (rollm)
+        self.Us = [np.zeros((nc, nz, self.n)) for _ in range(nord)]
+        self.Uf = [np.zeros((nc, nz, self.n)) for _ in range(nord)]
         self.un = np.zeros((nord, nz, self.nline))
         self.hopbc = np.zeros((nord, nz, self.nline))
 
@@ -335,7 +336,7 @@ class DnsOracle:
         for q in range(Je):
             H += -alpha[q + 1] * self.Us[q] + beta[q] * self.Uf[q]
         u_old = self.Us[0].copy()                 # D->u after swapData
-        self.Uf = np.roll(self.Uf, 1, axis=0)     # rollm(Uf)
+        self.Uf = self.Uf[-1:] + self.Uf[:-1]     # rollm(Uf)
         # setPForce: div(u^) / dt
         fp = self.grad_xy(H[0], 0) + self.grad_xy(H[1], 1)
         if self.ncom == 3:
@@ -352,7 +353,7 @@ class DnsOracle:
         self.Uf[0] = F
         # Us: store u^n and roll
         self.Us[0] = u_old
-        self.Us = np.roll(self.Us, 1, axis=0)
+        self.Us = self.Us[-1:] + self.Us[:-1]
         # viscous substep (first-order constant until the history is full)
         lam2 = SS[Je][0] / (dt * nu)
         for i in range(self.ncom):

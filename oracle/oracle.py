@@ -171,7 +171,7 @@ class _OrMsys(C.Structure):
     _fields_ = [("lambda2", C.c_double), ("betak2", C.c_double),
                 ("nsolve", C.c_int), ("nband", C.c_int), ("singular", C.c_int),
                 ("H", C.c_void_p), ("hbi", C.c_void_p), ("hii", C.c_void_p),
-                ("geo", C.c_void_p)]
+                ("geo", C.c_void_p), ("hbb_cache", C.c_void_p)]
 
 
 class OracleMesh:

@@ -373,19 +373,24 @@ void or_grad (int np, const double* geo, double* tgtX, double* tgtY)
 int or_pbtrf (int n, int kd, double* ab, int ldab)
 {
   int j, i, l;
+  double* row = malloc (sizeof (double) * (kd + 1));
 #define AB(r, c) ab[(size_t) (c) * ldab + kd + (r) - (c)]
   for (j = 0; j < n; j++) {
     const int last = (j + kd < n - 1) ? j + kd : n - 1;
     double d = AB (j, j);
-    if (d <= 0.0) return j + 1;
+    if (d <= 0.0) { free (row); return j + 1; }
     d = sqrt (d);
     AB (j, j) = d;
-    for (i = j + 1; i <= last; i++) AB (j, i) /= d;
-    for (i = j + 1; i <= last; i++) {
-      const double uji = AB (j, i);
-      for (l = i; l <= last; l++) AB (i, l) -= uji * AB (j, l);
+    for (i = j + 1; i <= last; i++) row[i - j] = (AB (j, i) /= d);
+    /* U(i,l) -= U(j,i) U(j,l) for j < i <= l <= last, column by column
+     * (each column of the band is contiguous in i)                      */
+    for (l = j + 1; l <= last; l++) {
+      const double ujl = row[l - j];
+      double* col = &AB (0, l);
+      for (i = j + 1; i <= l; i++) col[i] -= row[i - j] * ujl;
     }
   }
+  free (row);
   return 0;
 }
 
@@ -429,14 +434,44 @@ int or_msys_build (const or_mesh* M, double lambda2, double betak2, or_msys* S)
   S->hbi = calloc ((size_t) nel * next * (nint ? nint : 1), sizeof (double));
   S->hii = calloc ((size_t) nel * (nint ? nint * nint : 1), sizeof (double));
   S->geo = calloc ((size_t) nel * OR_NGEO * npnp, sizeof (double));
+  S->hbb_cache = calloc ((size_t) nel * next * next, sizeof (double));
 
   for (e = 0; e < nel; e++) {
     const int* bmap = M->btog + e * next;
     double* geo = S->geo + (size_t) e * OR_NGEO * npnp;
+    int twin = -1, f;
     if (or_mapping (np, M->cyl, M->x + e * npnp, M->y + e * npnp, geo)) rc = -2;
-    if (or_helmholtz_sc (np, M->cyl, geo, M->y + e * npnp, lambda2, betak2, hbb,
-                         S->hbi + (size_t) e * next * nint,
-                         S->hii + (size_t) e * nint * nint)) rc = -3;
+    /* Family storage (femlib/family.c:171-192, veclib/lxsame.c:37-49):
+     * the reference keeps one copy of each geometric-factor array, and an
+     * element whose array matches an earlier element's to EPSDP (absolute,
+     * entry by entry) adopts the earlier copy (src/element.cpp:82-93).
+     * The elemental matrices then depend only on adopted arrays; when
+     * Q1..Q4 (and, cylindrical, the radii) are the same storage as an
+     * earlier element's, its matrix packs are the same too
+     * (src/matrix.cpp:236-237), and are copied instead of recomputed.   */
+    for (k = 0; k < OR_NGEO; k++) {
+      double* ga = geo + (size_t) k * npnp;
+      for (f = 0; f < e; f++) {
+        const double* gb = S->geo + ((size_t) f * OR_NGEO + k) * npnp;
+        int same = 1;
+        for (i = 0; i < npnp && same; i++) same = fabs (ga[i] - gb[i]) <= EPSDP;
+        if (same) { memcpy (ga, gb, sizeof (double) * npnp); break; }
+      }
+    }
+    for (f = 0; f < e && twin < 0; f++)
+      if (!memcmp (geo, S->geo + (size_t) f * OR_NGEO * npnp, sizeof (double) * 4 * npnp) &&
+          (!M->cyl || !memcmp (M->y + e * npnp, M->y + f * npnp, sizeof (double) * npnp)))
+        twin = f;
+    if (twin >= 0) {
+      memcpy (hbb, S->hbb_cache + (size_t) twin * next * next, sizeof (double) * next * next);
+      memcpy (S->hbi + (size_t) e * next * nint, S->hbi + (size_t) twin * next * nint,
+              sizeof (double) * next * nint);
+      memcpy (S->hii + (size_t) e * nint * nint, S->hii + (size_t) twin * nint * nint,
+              sizeof (double) * nint * nint);
+    } else if (or_helmholtz_sc (np, M->cyl, geo, M->y + e * npnp, lambda2, betak2, hbb,
+                                S->hbi + (size_t) e * next * nint,
+                                S->hii + (size_t) e * nint * nint)) rc = -3;
+    memcpy (S->hbb_cache + (size_t) e * next * next, hbb, sizeof (double) * next * next);
     /* src/matrix.cpp:229-234 band assembly */
     for (i = 0; i < next; i++) {
       const int m = bmap[i];
@@ -449,6 +484,8 @@ int or_msys_build (const or_mesh* M, double lambda2, double betak2, or_msys* S)
     }
   }
   free (hbb);
+  free (S->hbb_cache);
+  S->hbb_cache = NULL;
   if (rc) return rc;
   if (S->nsolve && or_pbtrf (S->nsolve, S->nband - 1, S->H, S->nband)) return -4;
   return 0;

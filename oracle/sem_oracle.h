@@ -97,6 +97,7 @@ typedef struct {
   double* hbi;            /* [nel][next*nint]                               */
   double* hii;            /* [nel][nint*nint]                               */
   double* geo;            /* [nel][OR_NGEO*np*np]                           */
+  double* hbb_cache;      /* build-time scratch, NULL afterwards            */
 } or_msys;
 
 /* Build (and Cholesky-factor) one mode's system.  Returns 0 or an error. */

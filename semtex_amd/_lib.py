@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libsemtex_amd.so")
 
 SX_OK = 0
 SX_FP64, SX_MIXED = 0, 1
+SX_SKEW, SX_ALTSKEW = 0, 1
 
 # Every symbol include/semtex_amd.h declares, with (restype, argtypes).
 _vp, _i, _d, _ip, _dp = C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p
@@ -32,6 +33,12 @@ SIGNATURES = {
                                   C.POINTER(_i), _vp]),
     "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
+    "sx_dns_create": (_i, [_vp, _vp, _i, _d, _d, _d, _dp, _i, _i, _vp, C.POINTER(_vp)]),
+    "sx_dns_destroy": (_i, [_vp]),
+    "sx_dns_set_state": (_i, [_vp, _dp, _vp]),
+    "sx_dns_step": (_i, [_vp, _i, _vp]),
+    "sx_dns_get": (_i, [_vp, _i, _dp, C.POINTER(_i), _vp]),
+    "sx_dns_plane_stride": (_i, [_vp]),
     "sx_timing_enable": (_i, [_i]),
     "sx_timing_read": (_i, [_dp, _ip, _i]),
 }

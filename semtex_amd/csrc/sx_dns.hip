@@ -1,0 +1,758 @@
+// sx_dns.hip -- the DNS time step of semtex's dns driver on gfx950:
+// integrate() (dns/integrate.cpp:66-201) with skewSymmetric or
+// altSkewSymmetric advection (dns/nonlinear.cpp:38-381, Cartesian), the
+// computed pressure boundary conditions (BCmgr::maintainFourier /
+// evaluateCNBCp, src/bcmgr.cpp:1037-1335, Edge::curlCurl src/edge.cpp:98),
+// Field::smooth (src/field.cpp:282-327) and ConstForce
+// (dns/fieldforce.cpp:268).  All fields stay on the GPU; the four
+// Helmholtz solves per step go through sx_field_solve.
+//
+// Kernels:
+//   k_dft        packed real DFT along z as an (nz x nz) matrix applied to
+//                [nz][stride] planes (femlib/fourier.c packing, 1/nz on
+//                the forward transform), FP64, LDS-staged matrix rows
+//   k_grad       x / y derivative per element (Element::grad)
+//   k_gradz      Fourier z derivative (AuxField::gradient(2))
+//   k_ew_*       fused element-wise field algebra of the advection terms,
+//                waveProp, setPForce and project
+//   k_smooth_*   mass-averaged direct stiffness sum on element boundaries
+//   k_hopbc      -d(u.n)/dt + n.N - nu n.curlcurl u on boundary segments
+//   k_plines     extrapolated dp/dn line data for the pressure solve
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "sx_internal.h"
+
+using namespace sx;
+
+namespace {
+
+constexpr double EPSDP = 6.0e-14;
+
+// ---------------------------------------------------------------------------
+// z transforms: out[k][s] = sum_n M[k][n] in[n][s], s < S (plane stride).
+// One thread per column s and group of KG output planes; the KG rows of M
+// are staged in LDS.
+// ---------------------------------------------------------------------------
+constexpr int KG = 16;
+__global__ __launch_bounds__(256) void k_dft (const double* __restrict__ M,
+                                              const double* __restrict__ in,
+                                              double* out, int nz, int S)
+{
+  extern __shared__ double Ms[];   // [KG][nz]
+  const int k0 = blockIdx.y * KG, kg = min (KG, nz - k0);
+  for (int i = threadIdx.x; i < kg * nz; i += blockDim.x) Ms[i] = M[(size_t) k0 * nz + i];
+  __syncthreads ();
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double acc[KG];
+#pragma unroll
+  for (int k = 0; k < KG; k++) acc[k] = 0.0;
+  for (int n = 0; n < nz; n++) {
+    const double x = in[(size_t) n * S + s];
+#pragma unroll
+    for (int k = 0; k < KG; k++)
+      if (k < kg) acc[k] += Ms[k * nz + n] * x;
+  }
+#pragma unroll
+  for (int k = 0; k < KG; k++)
+    if (k < kg) out[(size_t) (k0 + k) * S + s] = acc[k];
+}
+
+// x or y derivative of every element of every plane (Element::grad,
+// src/element.cpp:521-565): d/dx = (u DT) drdx + (D u) dsdx.
+__global__ __launch_bounds__(64) void k_grad (int np, const double* __restrict__ geo,
+                                              const double* __restrict__ Dg,
+                                              const double* __restrict__ in,
+                                              double* out, int dir, int stride)
+{
+  const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  __shared__ double D[256], u[256];
+  const size_t off = (size_t) k * stride + (size_t) e * npnp;
+  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; u[i] = in[off + i]; }
+  __syncthreads ();
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  const double* gr = ge + (dir == 0 ? G_RX : G_RY) * npnp;
+  const double* gs = ge + (dir == 0 ? G_SX : G_SY) * npnp;
+  for (int ij = lane; ij < npnp; ij += 64) {
+    const int i = ij / np, j = ij % np;
+    double ur = 0.0, us = 0.0;
+    for (int q = 0; q < np; q++) {
+      ur += u[i * np + q] * D[j * np + q];
+      us += D[i * np + q] * u[q * np + j];
+    }
+    out[off + ij] = ur * gr[ij] + us * gs[ij];
+  }
+}
+
+// AuxField::gradient(2) on Fourier planes: planes 0 and 1 zero,
+// Re_k <- -beta k Im_k, Im_k <- beta k Re_k.
+__global__ void k_gradz (const double* __restrict__ in, double* out, int nz,
+                         int S, double beta)
+{
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) nz * S) return;
+  const int k = (int) (idx / S);
+  const size_t s = idx % S;
+  const int m = k >> 1;
+  double v = 0.0;
+  if (m >= 1) {
+    const double bk = beta * m;
+    v = (k & 1) ? bk * in[(size_t) (k - 1) * S + s] : -bk * in[(size_t) (k + 1) * S + s];
+  }
+  out[idx] = v;
+}
+
+// Element-wise kernels over n = nz*S entries.
+__global__ void k_ew_submul (double* N, const double* a, const double* b, size_t n)
+{   // N -= a*b  (AuxField::timesMinus)
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) N[i] -= a[i] * b[i];
+}
+__global__ void k_ew_mul (double* out, const double* a, const double* b, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] * b[i];
+}
+__global__ void k_ew_sub (double* N, const double* t, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) N[i] -= t[i];
+}
+__global__ void k_ew_scale (double* x, double a, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= a;
+}
+__global__ void k_ew_addc (double* x, double c, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += c;
+}
+// waveProp (dns/integrate.cpp:204-239): H = sum_q -alpha[q+1] Us[q] + beta[q] dt Uf[q]
+__global__ void k_ew_wave (double* H, const double* us0, const double* us1,
+                           const double* uf0, const double* uf1, double a1,
+                           double a2, double b0, double b1, int Je, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double h = 0.0;
+  h += -a1 * us0[i];
+  h += b0 * uf0[i];
+  if (Je > 1) { h += -a2 * us1[i]; h += b1 * uf1[i]; }
+  H[i] = h;
+}
+// setPForce (dns/integrate.cpp:242-257): f = (g0 + g1 [+ g2]) / dt
+__global__ void k_ew_div (double* f, const double* g0, const double* g1,
+                          const double* g2, double dt, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = g0[i] + g1[i];
+  if (g2) s += g2[i];
+  f[i] = s / dt;
+}
+// project (dns/integrate.cpp:260-289): F = H * alpha + beta * grad p
+__global__ void k_ew_project (double* F, const double* H, const double* gp,
+                              double alpha, double beta, size_t n)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f = H[i] * alpha;
+  f += beta * gp[i];
+  F[i] = f;
+}
+
+// Field::smooth: ds[k][g] = imass[g] sum Q4 f over the element-boundary
+// slots of global node g, then written back to every slot.
+__global__ void k_smooth_sum (int npnp, int nglobal, const int* __restrict__ gptr,
+                              const int* __restrict__ gsrc, const int* __restrict__ emap,
+                              const double* __restrict__ geo, const double* __restrict__ imass,
+                              const double* __restrict__ f, double* ds, int S)
+{
+  const int g = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (g >= nglobal) return;
+  double s = 0.0;
+  for (int c = gptr[g]; c < gptr[g + 1]; c++) {
+    const int slot = gsrc[c], e = slot / npnp, ij = emap[slot % npnp];
+    s += geo[((size_t) e * NGEO + G_Q4) * npnp + ij] * f[(size_t) k * S + (size_t) e * npnp + ij];
+  }
+  ds[(size_t) k * nglobal + g] = s * imass[g];
+}
+__global__ void k_smooth_put (int npnp, int next, int nglobal, const int* __restrict__ btog,
+                              const int* __restrict__ emap, const double* __restrict__ ds,
+                              double* f, int nel, int S)
+{
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+  if (idx >= nel * next) return;
+  const int e = idx / next, i = idx % next;
+  f[(size_t) k * S + (size_t) e * npnp + emap[i]] = ds[(size_t) k * nglobal + btog[idx]];
+}
+
+// Computed pressure BC terms on boundary segment b, Fourier mode m
+// (BCmgr::maintainFourier, src/bcmgr.cpp:1037-1296; Edge::curlCurl,
+// src/edge.cpp:98-220), Cartesian.  hop_new and un_new are level 0 of
+// the rolled storage; un1, un2 the previous two levels of u.n.
+__global__ __launch_bounds__(64) void k_hopbc (
+    int np, const int* __restrict__ seg_elmt, const int* __restrict__ seg_idx,
+    const double* __restrict__ seg_nx, const double* __restrict__ seg_ny,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const double* __restrict__ U0, const double* __restrict__ U1,
+    const double* __restrict__ U2, const double* __restrict__ N0,
+    const double* __restrict__ N1, double* hop_new, double* un_new,
+    const double* __restrict__ un1, const double* __restrict__ un2, int Je,
+    double a0, double a1, double a2, double inv_dt, double nu, double beta,
+    int nz, int nline, int S)
+{
+  const int npnp = np * np, b = blockIdx.x, m = blockIdx.y, lane = threadIdx.x;
+  const int e = seg_elmt[b];
+  __shared__ double D[256], ur[256], ui[256], vr[256], vi[256], wr[256], wi[256];
+  __shared__ double gwr[2][16], gwi[2][16], wzr[256], wzi[256];
+  __shared__ int idx[16];
+  __shared__ double nx[16], ny[16];
+  for (int i = lane; i < npnp; i += 64) D[i] = Dg[i];
+  if (lane < np) {
+    idx[lane] = seg_idx[b * np + lane];
+    nx[lane] = seg_nx[b * np + lane];
+    ny[lane] = seg_ny[b * np + lane];
+  }
+  const int kr = (m == 0) ? 0 : 2 * m, ki = kr + 1;
+  const bool three = (m > 0);
+  const size_t eoff = (size_t) e * npnp;
+  for (int i = lane; i < npnp; i += 64) {
+    ur[i] = U0[(size_t) kr * S + eoff + i];
+    vr[i] = U1[(size_t) kr * S + eoff + i];
+    if (three) {
+      ui[i] = U0[(size_t) ki * S + eoff + i];
+      vi[i] = U1[(size_t) ki * S + eoff + i];
+      wr[i] = U2[(size_t) kr * S + eoff + i];
+      wi[i] = U2[(size_t) ki * S + eoff + i];
+    }
+  }
+  __syncthreads ();
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  const double *rx = ge + G_RX * npnp, *ry = ge + G_RY * npnp,
+               *sx = ge + G_SX * npnp, *sy = ge + G_SY * npnp;
+  // vorticity w_z = dv/dx - du/dy over the element (Re, and Im for m > 0)
+  for (int ij = lane; ij < npnp; ij += 64) {
+    const int i = ij / np, j = ij % np;
+    double vr_r = 0, vr_s = 0, ur_r = 0, ur_s = 0, vi_r = 0, vi_s = 0, ui_r = 0, ui_s = 0;
+    for (int q = 0; q < np; q++) {
+      const double dj = D[j * np + q], di = D[i * np + q];
+      vr_r += vr[i * np + q] * dj; vr_s += di * vr[q * np + j];
+      ur_r += ur[i * np + q] * dj; ur_s += di * ur[q * np + j];
+      if (three) {
+        vi_r += vi[i * np + q] * dj; vi_s += di * vi[q * np + j];
+        ui_r += ui[i * np + q] * dj; ui_s += di * ui[q * np + j];
+      }
+    }
+    wzr[ij] = (vr_r * rx[ij] + vr_s * sx[ij]) - (ur_r * ry[ij] + ur_s * sy[ij]);
+    if (three) wzi[ij] = (vi_r * rx[ij] + vi_s * sx[ij]) - (ui_r * ry[ij] + ui_s * sy[ij]);
+  }
+  __syncthreads ();
+  // gradients of w_z at the side nodes
+  if (lane < np) {
+    const int ij = idx[lane], i = ij / np, j = ij % np;
+    double a_r = 0, a_s = 0, b_r = 0, b_s = 0;
+    for (int q = 0; q < np; q++) {
+      const double dj = D[j * np + q], di = D[i * np + q];
+      a_r += wzr[i * np + q] * dj; a_s += di * wzr[q * np + j];
+      if (three) { b_r += wzi[i * np + q] * dj; b_s += di * wzi[q * np + j]; }
+    }
+    gwr[0][lane] = a_r * rx[ij] + a_s * sx[ij];     // d(w_r)/dx
+    gwr[1][lane] = a_r * ry[ij] + a_s * sy[ij];     // d(w_r)/dy
+    if (three) {
+      gwi[0][lane] = b_r * rx[ij] + b_s * sx[ij];
+      gwi[1][lane] = b_r * ry[ij] + b_s * sy[ij];
+    }
+  }
+  __syncthreads ();
+  if (lane < np) {
+    const int ij = idx[lane], i = ij / np, j = ij % np, q0 = lane;
+    const int jl = b * np + q0;
+    // -- plane(s) of this mode: u.n, time derivative, n.N
+    for (int pl = 0; pl < (three ? 2 : 1); pl++) {
+      const int k = pl ? ki : kr;
+      const double un = U0[(size_t) k * S + eoff + ij] * nx[q0] + U1[(size_t) k * S + eoff + ij] * ny[q0];
+      double h = 0.0;
+      if (Je) {
+        double w = a0 * un;
+        w += a1 * un1[(size_t) k * nline + jl];
+        if (Je > 1) w += a2 * un2[(size_t) k * nline + jl];
+        h = -inv_dt * w;
+      }
+      double nn = 0.0;
+      nn += N0[(size_t) k * S + eoff + ij] * nx[q0];
+      nn += N1[(size_t) k * S + eoff + ij] * ny[q0];
+      h += nn;
+      un_new[(size_t) k * nline + jl] = un;
+      hop_new[(size_t) k * nline + jl] = h;
+    }
+    // -- -nu n.curl curl u
+    double xr = gwr[1][q0], yr = -gwr[0][q0];
+    if (!three) {
+      const size_t o = (size_t) kr * nline + jl;
+      hop_new[o] += -nu * xr * nx[q0];
+      hop_new[o] += -nu * yr * ny[q0];
+    } else {
+      const double bk = beta * m, bk2 = bk * bk;
+      double xi = gwi[1][q0], yi = -gwi[0][q0];
+      // semi-Fourier terms: grads of Wr, Wi at the node
+      double wr_x = 0, wr_y = 0, wi_x = 0, wi_y = 0;
+      {
+        double c_r = 0, c_s = 0, d_r = 0, d_s = 0;
+        for (int q = 0; q < np; q++) {
+          const double dj = D[j * np + q], di = D[i * np + q];
+          c_r += wr[i * np + q] * dj; c_s += di * wr[q * np + j];
+          d_r += wi[i * np + q] * dj; d_s += di * wi[q * np + j];
+        }
+        wr_x = c_r * rx[ij] + c_s * sx[ij]; wr_y = c_r * ry[ij] + c_s * sy[ij];
+        wi_x = d_r * rx[ij] + d_s * sx[ij]; wi_y = d_r * ry[ij] + d_s * sy[ij];
+      }
+      xi += bk * wr_x;
+      yi += bk * wr_y;
+      xr += -bk * wi_x;
+      yr += -bk * wi_y;
+      xr += bk2 * ur[ij];
+      xi += bk2 * ui[ij];
+      yr += bk2 * vr[ij];
+      yi += bk2 * vi[ij];
+      const size_t orr = (size_t) kr * nline + jl, oi = (size_t) ki * nline + jl;
+      hop_new[orr] += -nu * xr * nx[q0];
+      hop_new[oi] += -nu * xi * nx[q0];
+      hop_new[orr] += -nu * yr * ny[q0];
+      hop_new[oi] += -nu * yi * ny[q0];
+    }
+  }
+}
+
+// evaluateCNBCp: lines[k] = sum_q beta_q hop[q][k]; Nyquist plane and
+// essential pressure segments zero.
+__global__ void k_plines (double* lines, const double* h0, const double* h1,
+                          double b0, double b1, int Je, int nz, int nline, int np,
+                          const int* __restrict__ kind)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t) nz * nline) return;
+  const int k = (int) (i / nline), j = (int) (i % nline);
+  double v = 0.0;
+  if (!(nz > 1 && k == 1) && kind[j / np] == SX_NATURAL) {
+    v += b0 * h0[i];
+    if (Je > 1) v += b1 * h1[i];
+  }
+  lines[i] = v;
+}
+
+inline unsigned nb256 (size_t n) { return (unsigned) ((n + 255) / 256); }
+
+}  // namespace
+
+// ===========================================================================
+struct sx_dns {
+  sx_mesh *mu = nullptr, *mp = nullptr;
+  int nz = 1, ncom = 2, S = 0, advection = SX_SKEW, nord = 2, toggle = 1, step = 0;
+  double beta = 0, nu = 0, dt = 0, cf[3] = {0, 0, 0};
+  size_t n = 0;                        // nz * S
+  sx_msys *Mv[2] = {nullptr, nullptr}, *Mp = nullptr;
+  std::vector<double*> bufs;           // every device allocation
+  double *u[3] = {}, *p = nullptr;
+  double *Us[2][3] = {}, *Uf[2][3] = {};
+  double *phys[3] = {}, *tmp = nullptr, *tmp2 = nullptr, *H[3] = {}, *F[3] = {};
+  double *Cf = nullptr, *Ci = nullptr; // forward / inverse packed DFT matrices
+  double *imass = nullptr, *ds = nullptr;
+  int nb = 0, nline = 0, hop_head = 0, un_head = 0;
+  int *seg_elmt = nullptr, *seg_idx = nullptr, *seg_kind = nullptr;
+  double *seg_nx = nullptr, *seg_ny = nullptr;
+  double *hop[2] = {}, *un[2] = {}, *plines = nullptr;
+
+  double* alloc (size_t cnt)
+  {
+    double* q = nullptr;
+    if (hipMalloc (&q, sizeof (double) * std::max<size_t> (cnt, 1)) != hipSuccess) return nullptr;
+    bufs.push_back (q);
+    return q;
+  }
+};
+
+namespace {
+
+int dft (sx_dns* d, const double* M, const double* in, double* out, hipStream_t st)
+{
+  dim3 grid ((d->S + 255) / 256, (d->nz + KG - 1) / KG);
+  hipLaunchKernelGGL (k_dft, grid, dim3 (256), sizeof (double) * KG * d->nz, st, M, in,
+                      out, d->nz, d->S);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// in-place transform through the scratch buffer tmp2
+int transform (sx_dns* d, double* f, int sign, hipStream_t st)
+{
+  if (d->nz < 2) return SX_OK;
+  int rc = dft (d, sign > 0 ? d->Cf : d->Ci, f, d->tmp2, st);
+  if (rc) return rc;
+  SX_HIP (hipMemcpyAsync (f, d->tmp2, sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+  return SX_OK;
+}
+
+int grad (sx_dns* d, const double* in, double* out, int dir, hipStream_t st)
+{
+  const sx_mesh* m = d->mu;
+  if (dir == 2) {
+    hipLaunchKernelGGL (k_gradz, dim3 (nb256 (d->n)), dim3 (256), 0, st, in, out, d->nz,
+                        d->S, d->beta);
+  } else {
+    hipLaunchKernelGGL (k_grad, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
+                        m->d_D, in, out, dir, d->S);
+  }
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+int smooth (sx_dns* d, double* f, hipStream_t st)
+{
+  const sx_mesh* m = d->mu;
+  hipLaunchKernelGGL (k_smooth_sum, dim3 ((m->nglobal + 255) / 256, d->nz), dim3 (256), 0,
+                      st, m->npnp, m->nglobal, m->d_gptr, m->d_gsrc, m->d_emap, m->d_geo,
+                      d->imass, f, d->ds, d->S);
+  hipLaunchKernelGGL (k_smooth_put, dim3 ((m->nel * m->next + 255) / 256, d->nz),
+                      dim3 (256), 0, st, m->npnp, m->next, m->nglobal, m->d_btog, m->d_emap,
+                      d->ds, f, m->nel, d->S);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+#define EW(kern, ...) \
+  do { hipLaunchKernelGGL (kern, dim3 (nb256 (d->n)), dim3 (256), 0, st, __VA_ARGS__); } while (0)
+
+// skewSymmetric / altSkewSymmetric (dns/nonlinear.cpp): U (Fourier copy of
+// u) -> Us0, N -> Uf0.  d->u are transformed to physical space in place
+// (the reference transforms D->u in place too).
+int advection (sx_dns* d, double** U, double** N, hipStream_t st)
+{
+  const int nc = d->ncom;
+  int rc;
+  for (int i = 0; i < nc; i++) {
+    SX_HIP (hipMemcpyAsync (U[i], d->u[i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+    SX_HIP (hipMemsetAsync (N[i], 0, sizeof (double) * d->n, st));
+    SX_HIP (hipMemcpyAsync (d->phys[i], d->u[i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+    if ((rc = transform (d, d->phys[i], -1, st))) return rc;
+  }
+  const bool conv = d->advection == SX_SKEW || d->toggle == 1;
+  const bool cons = d->advection == SX_SKEW || d->toggle == 0;
+  for (int i = 0; i < nc; i++) {
+    for (int j = 0; j < nc; j++) {
+      if (conv) {
+        if (j == 2) {
+          if ((rc = grad (d, U[i], d->tmp, 2, st))) return rc;
+          if ((rc = transform (d, d->tmp, -1, st))) return rc;
+        } else if ((rc = grad (d, d->phys[i], d->tmp, j, st))) return rc;
+        EW (k_ew_submul, N[i], d->phys[j], d->tmp, d->n);
+      }
+      if (cons) {
+        EW (k_ew_mul, d->tmp, d->phys[i], d->phys[j], d->n);
+        if (j == 2) {
+          if ((rc = transform (d, d->tmp, +1, st))) return rc;
+          if ((rc = grad (d, d->tmp, d->tmp2, 2, st))) return rc;
+          SX_HIP (hipMemcpyAsync (d->tmp, d->tmp2, sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+          if ((rc = transform (d, d->tmp, -1, st))) return rc;
+          EW (k_ew_sub, N[i], d->tmp, d->n);
+        } else {
+          if ((rc = grad (d, d->tmp, d->tmp2, j, st))) return rc;
+          EW (k_ew_sub, N[i], d->tmp2, d->n);
+        }
+      }
+    }
+    if (d->advection == SX_SKEW) EW (k_ew_scale, N[i], 0.5, d->n);
+    if ((rc = transform (d, N[i], +1, st))) return rc;
+    if (std::fabs (d->cf[i]) > EPSDP)
+      hipLaunchKernelGGL (k_ew_addc, dim3 (nb256 (d->S)), dim3 (256), 0, st, N[i], d->cf[i],
+                          (size_t) d->S);
+    if ((rc = smooth (d, N[i], st))) return rc;
+  }
+  if (d->advection == SX_ALTSKEW) d->toggle = 1 - d->toggle;
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+const double SSc[4][4] = {{0, 0, 0, 0}, {1.0, -1.0, 0, 0}, {1.5, -2.0, 0.5, 0}, {0, 0, 0, 0}};
+const double EXc[4][3] = {{0, 0, 0}, {1.0, 0, 0}, {2.0, -1.0, 0}, {0, 0, 0}};
+
+int dns_step (sx_dns* d, hipStream_t st)
+{
+  const int nc = d->ncom, nord = d->nord;
+  int rc;
+  double** U = d->Us[0];
+  double** N = d->Uf[0];
+  if ((rc = advection (d, U, N, st))) return rc;
+  d->step += 1;
+  // -- HOPBC: maintainFourier + evaluateCNBCp
+  double* plines = nullptr;
+  if (d->nb) {
+    const int Je = std::min (d->step - 1, nord);
+    d->hop_head = (d->hop_head + nord - 1) % nord;     // roll: new level 0
+    double* hnew = d->hop[d->hop_head];
+    double* un1 = d->un[d->un_head];                   // previous level 0
+    double* un2 = d->un[(d->un_head + 1) % nord];
+    const int unnew_head = (d->un_head + nord - 1) % nord;
+    double* unnew = d->un[unnew_head];
+    SX_HIP (hipMemsetAsync (hnew, 0, sizeof (double) * d->nz * d->nline, st));
+    // un_new is written into the level that is about to be dropped (un2)
+    // only after un2 has been read: use a separate scratch when nord == 2
+    double* unscr = d->plines;   // reused as scratch before the lines are formed
+    SX_HIP (hipMemsetAsync (unscr, 0, sizeof (double) * d->nz * d->nline, st));
+    const int nmodes = std::max (1, d->nz / 2);
+    hipLaunchKernelGGL (k_hopbc, dim3 (d->nb, nmodes), dim3 (64), 0, st, d->mu->np,
+                        d->seg_elmt, d->seg_idx, d->seg_nx, d->seg_ny, d->mu->d_geo,
+                        d->mu->d_D, U[0], U[1], nc == 3 ? U[2] : U[0], N[0], N[1], hnew,
+                        unscr, un1, un2, Je, SSc[Je][0], SSc[Je][1], SSc[Je][2],
+                        1.0 / d->dt, d->nu, d->beta, d->nz, d->nline, d->S);
+    SX_HIP (hipGetLastError ());
+    SX_HIP (hipMemcpyAsync (unnew, unscr, sizeof (double) * d->nz * d->nline,
+                            hipMemcpyDeviceToDevice, st));
+    d->un_head = unnew_head;
+    const int Jb = std::min (d->step, nord);
+    hipLaunchKernelGGL (k_plines, dim3 (nb256 ((size_t) d->nz * d->nline)), dim3 (256), 0,
+                        st, d->plines, d->hop[d->hop_head], d->hop[(d->hop_head + 1) % nord],
+                        EXc[Jb][0], EXc[Jb][1], Jb, d->nz, d->nline, d->mu->np, d->seg_kind);
+    SX_HIP (hipGetLastError ());
+    plines = d->plines;
+  }
+  // -- waveProp
+  const int Je = std::min (d->step, nord);
+  for (int i = 0; i < nc; i++)
+    EW (k_ew_wave, d->H[i], d->Us[0][i], d->Us[1][i], d->Uf[0][i], d->Uf[1][i],
+        SSc[Je][1], SSc[Je][2], EXc[Je][0] * d->dt, EXc[Je][1] * d->dt, Je, d->n);
+  // D->u after swapData holds u^n = Us[0]
+  for (int i = 0; i < nc; i++)
+    SX_HIP (hipMemcpyAsync (d->u[i], d->Us[0][i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+  // rollm (Uf)
+  for (int i = 0; i < nc; i++) std::swap (d->Uf[0][i], d->Uf[1][i]);
+  // -- setPForce: div u^ / dt into F[0]
+  if ((rc = grad (d, d->H[0], d->F[0], 0, st))) return rc;
+  if ((rc = grad (d, d->H[1], d->F[1], 1, st))) return rc;
+  if (nc == 3 && (rc = grad (d, d->H[2], d->F[2], 2, st))) return rc;
+  EW (k_ew_div, d->tmp, d->F[0], d->F[1], nc == 3 ? d->F[2] : nullptr, d->dt, d->n);
+  // -- pressure
+  if ((rc = sx_field_solve (d->Mp, d->nz, d->S, d->tmp, d->p, plines, st))) return rc;
+  // -- project: F_i = -u^_i/(dt nu) + grad_i p / nu
+  const double alpha = -1.0 / (d->dt * d->nu), betap = 1.0 / d->nu;
+  for (int i = 0; i < nc; i++) {
+    if ((rc = grad (d, d->p, d->tmp, i, st))) return rc;
+    EW (k_ew_project, d->F[i], d->H[i], d->tmp, alpha, betap, d->n);
+  }
+  // -- Us: store u^n and roll
+  for (int i = 0; i < nc; i++) std::swap (d->Us[0][i], d->Us[1][i]);
+  // -- viscous substep
+  sx_msys* Mv = d->Mv[Je - 1];
+  for (int i = 0; i < nc; i++)
+    if ((rc = sx_field_solve (Mv, d->nz, d->S, d->F[i], d->u[i], nullptr, st))) return rc;
+  return SX_OK;
+}
+
+}  // namespace
+
+extern "C" int sx_dns_destroy (sx_dns_t d)
+{
+  if (!d) return SX_OK;
+  for (double* q : d->bufs) (void) hipFree (q);
+  void* ip[] = {d->seg_elmt, d->seg_idx, d->seg_kind};
+  for (void* q : ip) if (q) (void) hipFree (q);
+  for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mp}) if (M) sx_msys_destroy (M);
+  delete d;
+  return SX_OK;
+}
+
+extern "C" int sx_dns_plane_stride (sx_dns_t d) { return d ? d->S : 0; }
+
+extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
+                              double kinvis, double dt, const double* cf, int advection,
+                              int nord, void* stream, sx_dns_t* out)
+{
+  if (!mu || !mp || !out || nz < 1 || (nz > 1 && (nz & 1)) || kinvis <= 0 || dt <= 0 ||
+      nord != 2 || (advection != SX_SKEW && advection != SX_ALTSKEW) || mu->nel != mp->nel ||
+      mu->np != mp->np || mu->nbound != mp->nbound || mu->cyl || mp->cyl || mu->np > 16) {
+    set_error ("sx_dns_create: bad arguments (Cartesian, nz 1 or even, N_TIME = 2, "
+               "same mesh and boundary segments for velocity and pressure)");
+    return SX_ERR_ARG;
+  }
+  for (int b = 0; b < mu->nbound; b++)
+    if (mu->bnd_elmt[b] != mp->bnd_elmt[b] || mu->bnd_side[b] != mp->bnd_side[b]) {
+      set_error ("sx_dns_create: velocity and pressure boundary segments differ");
+      return SX_ERR_ARG;
+    }
+  *out = nullptr;
+  hipStream_t st = (hipStream_t) stream;
+  sx_dns* d = new sx_dns;
+  d->mu = mu; d->mp = mp; d->nz = nz; d->ncom = (nz == 1) ? 2 : 3;
+  d->beta = beta; d->nu = kinvis; d->dt = dt; d->nord = nord; d->advection = advection;
+  if (cf) for (int i = 0; i < 3; i++) d->cf[i] = cf[i];
+  const int npnp = mu->npnp;
+  d->S = mu->nel * npnp + ((mu->nel * npnp) & 1);
+  d->n = (size_t) nz * d->S;
+  int rc;
+  auto fail = [&] (int r) { sx_dns_destroy (d); return r; };
+  const int nmodes = std::max (1, nz / 2);
+  // Helmholtz systems: velocity for the first-order start and the
+  // second-order steps, pressure (preSolve, dns/integrate.cpp:292-325)
+  if ((rc = sx_msys_create (mu, 1.0 / (dt * kinvis), beta, 0, nmodes, SX_FP64, stream, &d->Mv[0])))
+    return fail (rc);
+  if ((rc = sx_msys_create (mu, 1.5 / (dt * kinvis), beta, 0, nmodes, SX_FP64, stream, &d->Mv[1])))
+    return fail (rc);
+  if ((rc = sx_msys_create (mp, 0.0, beta, 0, nmodes, SX_FP64, stream, &d->Mp)))
+    return fail (rc);
+  // fields
+  const int nc = d->ncom;
+  for (int i = 0; i < nc; i++) {
+    if (!(d->u[i] = d->alloc (d->n)) || !(d->phys[i] = d->alloc (d->n)) ||
+        !(d->H[i] = d->alloc (d->n)) || !(d->F[i] = d->alloc (d->n)))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    for (int q = 0; q < 2; q++)
+      if (!(d->Us[q][i] = d->alloc (d->n)) || !(d->Uf[q][i] = d->alloc (d->n)))
+        return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+  }
+  if (!(d->p = d->alloc (d->n)) || !(d->tmp = d->alloc (d->n)) || !(d->tmp2 = d->alloc (d->n)))
+    return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+  for (double* q : d->bufs)
+    if (hipMemsetAsync (q, 0, sizeof (double) * d->n, st) != hipSuccess)
+      return fail (hip_fail (hipErrorUnknown, "sx_dns_create memset"));
+  // packed real DFT matrices (femlib/fourier.c:40-165 conventions)
+  if (nz > 1) {
+    std::vector<double> Cf ((size_t) nz * nz), Ci ((size_t) nz * nz);
+    for (int n = 0; n < nz; n++) {
+      Cf[0 * nz + n] = 1.0 / nz;
+      Cf[1 * nz + n] = ((n & 1) ? -1.0 : 1.0) / nz;
+      Ci[(size_t) n * nz + 0] = 1.0;
+      Ci[(size_t) n * nz + 1] = (n & 1) ? -1.0 : 1.0;
+      for (int k = 1; k < nz / 2; k++) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L *
+                              (long double) ((long) k * n % nz) / nz;
+        const double c = (double) cosl (a), s = (double) sinl (a);
+        Cf[(size_t) (2 * k) * nz + n] = c / nz;
+        Cf[(size_t) (2 * k + 1) * nz + n] = -s / nz;
+        Ci[(size_t) n * nz + 2 * k] = 2.0 * c;
+        Ci[(size_t) n * nz + 2 * k + 1] = -2.0 * s;
+      }
+    }
+    if (!(d->Cf = d->alloc (Cf.size ())) || !(d->Ci = d->alloc (Ci.size ())))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    if (hipMemcpy (d->Cf, Cf.data (), sizeof (double) * Cf.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->Ci, Ci.data (), sizeof (double) * Ci.size (), hipMemcpyHostToDevice))
+      return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
+  }
+  // smoothing: inverse global boundary mass (BCmgr::buildnum imass)
+  {
+    std::vector<double> mass (mu->nglobal, 0.0), q4 ((size_t) mu->nel * npnp);
+    std::vector<double> geo (NGEO * npnp);
+    for (int e = 0; e < mu->nel; e++) {
+      element_geometry (mu->np, 0, &mu->hx[(size_t) e * npnp], &mu->hy[(size_t) e * npnp], geo.data ());
+      for (int i = 0; i < mu->next; i++)
+        mass[mu->btog[(size_t) e * mu->next + i]] += geo[G_Q4 * npnp + mu->emap[i]];
+    }
+    for (double& v : mass) v = 1.0 / v;
+    if (!(d->imass = d->alloc (mass.size ())) || !(d->ds = d->alloc ((size_t) nz * mu->nglobal)))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    if (hipMemcpy (d->imass, mass.data (), sizeof (double) * mass.size (), hipMemcpyHostToDevice))
+      return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
+  }
+  // boundary segments for the computed pressure conditions
+  d->nb = mu->nbound;
+  d->nline = d->nb * mu->np;
+  if (d->nb) {
+    const int np = mu->np;
+    std::vector<int> el (d->nb), ix ((size_t) d->nb * np), kind (d->nb);
+    std::vector<double> nxv ((size_t) d->nb * np), nyv ((size_t) d->nb * np), geo (NGEO * npnp);
+    for (int b = 0; b < d->nb; b++) {
+      const int e = mu->bnd_elmt[b], s = mu->bnd_side[b];
+      el[b] = e;
+      kind[b] = mp->bnd_kind[b];
+      element_geometry (np, 0, &mu->hx[(size_t) e * npnp], &mu->hy[(size_t) e * npnp], geo.data ());
+      for (int k = 0; k < np; k++) {
+        int node;
+        double nx, ny;
+        switch (s) {      // CCW traverse and outward normal (Element::sideGeom)
+        case 0: node = k; nx = -geo[G_SX * npnp + node]; ny = -geo[G_SY * npnp + node]; break;
+        case 1: node = k * np + np - 1; nx = geo[G_RX * npnp + node]; ny = geo[G_RY * npnp + node]; break;
+        case 2: node = (np - 1) * np + (np - 1 - k); nx = geo[G_SX * npnp + node]; ny = geo[G_SY * npnp + node]; break;
+        default: node = (np - 1 - k) * np; nx = -geo[G_RX * npnp + node]; ny = -geo[G_RY * npnp + node]; break;
+        }
+        const double len = std::hypot (nx, ny);
+        ix[(size_t) b * np + k] = node;
+        nxv[(size_t) b * np + k] = nx / len;
+        nyv[(size_t) b * np + k] = ny / len;
+      }
+    }
+    if (hipMalloc (&d->seg_elmt, sizeof (int) * d->nb) || hipMalloc (&d->seg_idx, sizeof (int) * ix.size ()) ||
+        hipMalloc (&d->seg_kind, sizeof (int) * d->nb))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    if (!(d->seg_nx = d->alloc (nxv.size ())) || !(d->seg_ny = d->alloc (nyv.size ())))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    for (int q = 0; q < 2; q++)
+      if (!(d->hop[q] = d->alloc ((size_t) nz * d->nline)) || !(d->un[q] = d->alloc ((size_t) nz * d->nline)))
+        return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    if (!(d->plines = d->alloc ((size_t) nz * d->nline)))
+      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    if (hipMemcpy (d->seg_elmt, el.data (), sizeof (int) * el.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->seg_idx, ix.data (), sizeof (int) * ix.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->seg_kind, kind.data (), sizeof (int) * kind.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->seg_nx, nxv.data (), sizeof (double) * nxv.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->seg_ny, nyv.data (), sizeof (double) * nyv.size (), hipMemcpyHostToDevice))
+      return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
+    for (int q = 0; q < 2; q++) {
+      if (hipMemsetAsync (d->hop[q], 0, sizeof (double) * nz * d->nline, st) ||
+          hipMemsetAsync (d->un[q], 0, sizeof (double) * nz * d->nline, st))
+        return fail (hip_fail (hipErrorUnknown, "sx_dns_create memset"));
+    }
+  }
+  if (hipStreamSynchronize (st) != hipSuccess) return fail (hip_fail (hipErrorUnknown, "sync"));
+  *out = d;
+  return SX_OK;
+}
+
+extern "C" int sx_dns_set_state (sx_dns_t d, const double* u_phys, void* stream)
+{
+  if (!d || !u_phys) { set_error ("sx_dns_set_state: bad arguments"); return SX_ERR_ARG; }
+  hipStream_t st = (hipStream_t) stream;
+  int rc;
+  for (int i = 0; i < d->ncom; i++) {
+    SX_HIP (hipMemcpyAsync (d->u[i], u_phys + (size_t) i * d->n, sizeof (double) * d->n,
+                            hipMemcpyDeviceToDevice, st));
+    if ((rc = transform (d, d->u[i], +1, st))) return rc;
+    if (d->nz > 1) SX_HIP (hipMemsetAsync (d->u[i] + d->S, 0, sizeof (double) * d->S, st));
+    for (int q = 0; q < 2; q++) {
+      SX_HIP (hipMemsetAsync (d->Us[q][i], 0, sizeof (double) * d->n, st));
+      SX_HIP (hipMemsetAsync (d->Uf[q][i], 0, sizeof (double) * d->n, st));
+    }
+  }
+  SX_HIP (hipMemsetAsync (d->p, 0, sizeof (double) * d->n, st));
+  for (int q = 0; q < 2 && d->nb; q++) {
+    SX_HIP (hipMemsetAsync (d->hop[q], 0, sizeof (double) * d->nz * d->nline, st));
+    SX_HIP (hipMemsetAsync (d->un[q], 0, sizeof (double) * d->nz * d->nline, st));
+  }
+  d->step = 0; d->toggle = 1; d->hop_head = 0; d->un_head = 0;
+  return SX_OK;
+}
+
+extern "C" int sx_dns_step (sx_dns_t d, int nsteps, void* stream)
+{
+  if (!d || nsteps < 0) { set_error ("sx_dns_step: bad arguments"); return SX_ERR_ARG; }
+  for (int s = 0; s < nsteps; s++) {
+    const int rc = dns_step (d, (hipStream_t) stream);
+    if (rc) return rc;
+  }
+  return SX_OK;
+}
+
+extern "C" int sx_dns_get (sx_dns_t d, int which, double* d_out, int* step, void* stream)
+{
+  if (!d || !d_out || which < 0 || which > d->ncom) {
+    set_error ("sx_dns_get: bad arguments");
+    return SX_ERR_ARG;
+  }
+  const double* src = (which == d->ncom) ? d->p : d->u[which];
+  SX_HIP (hipMemcpyAsync (d_out, src, sizeof (double) * d->n, hipMemcpyDeviceToDevice,
+                          (hipStream_t) stream));
+  if (step) *step = d->step;
+  return SX_OK;
+}

@@ -55,6 +55,8 @@ struct sx_mesh {
   int nglobal = 0, nsolve = 0, nband = 0, kd = 0, nbound = 0, nclass = 0;
   int npairs = 0;
   std::vector<int> btog, bmask, emask, cls, emap;
+  std::vector<double> hx, hy;  // host node coordinates [nel][npnp]
+  std::vector<int> bnd_elmt, bnd_side, bnd_kind;
 
   // device arrays
   double* d_geo = nullptr;     // [nel][NGEO][npnp]

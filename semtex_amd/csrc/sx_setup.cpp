@@ -249,6 +249,11 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   const int npnp = m->npnp, next = m->next;
 
   m->btog.assign (btog, btog + (size_t) nel * next);
+  m->hx.assign (x, x + (size_t) nel * npnp);
+  m->hy.assign (y, y + (size_t) nel * npnp);
+  m->bnd_elmt.assign (bnd_elmt, bnd_elmt + nbound);
+  m->bnd_side.assign (bnd_side, bnd_side + nbound);
+  m->bnd_kind.assign (bnd_kind, bnd_kind + nbound);
   m->bmask.assign (bmask, bmask + (size_t) nel * next);
   m->emask.assign (emask, emask + nel);
   for (int v : m->btog)

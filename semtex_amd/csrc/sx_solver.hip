@@ -622,17 +622,6 @@ extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
   return SX_OK;
 }
 
-extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_f,
-                                     double* d_u, const double* d_lines,
-                                     double refine_tol, int max_iter, int* iters,
-                                     void* stream)
-{
-  (void) s; (void) nz; (void) stride; (void) d_f; (void) d_u; (void) d_lines;
-  (void) refine_tol; (void) max_iter; (void) iters; (void) stream;
-  set_error ("sx_field_solve_mixed: not implemented yet");
-  return SX_ERR_LIMIT;
-}
-
 extern "C" int sx_field_solve_host (sx_msys_t s, int nz, int stride, const double* f,
                                     double* u, const double* lines)
 {

@@ -45,3 +45,39 @@ def helmholtz_fourier(nx=16, ny=16, np_=10, nz=64, seed=SEED, ordering="auto"):
                   bc=("D", "N", "D", "N"), periodic_x=True, ordering=ordering)
     return dict(mesh=m, lambda2=1.0, beta=1.0, nz=nz, nmodes=nz // 2,
                 forcing=fourier_forcing(m, nz, seed))
+
+
+def dns_channel(nx=12, ny=24, np_=8, nz=128, Re=2000.0, dt=1e-3, Lz=2 * np.pi,
+                amp=1e-2, seed=SEED, ordering="auto"):
+    """Forced periodic channel for the DNS step: nx x ny elements on
+    [0,4pi]x[-1,1], N_P, N_Z planes with Lz = 2pi (beta = 2pi/Lz = 1),
+    KINVIS = 1/Re, CONST_X = 2 KINVIS (holds u = 1 - y^2), no-slip walls
+    on y = +-1 with computed pressure BCs, periodic x.  Initial field:
+    laminar Poiseuille plus i.i.d. U(-amp, amp) in u, v, w at every node and
+    plane (seeded), made single-valued on shared element-boundary nodes.
+    Returns the velocity and pressure meshes and the physical-space
+    initial state [3][nz][nel*np*np]."""
+    from .mesh import edgemaps
+    bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
+    mu = rect_mesh(nx, ny, np_, 0.0, 4 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True,
+                   ordering=ordering)
+    mp = rect_mesh(nx, ny, np_, 0.0, 4 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True,
+                   ordering=ordering)
+    nu = 1.0 / Re
+    n = mu.nel * mu.npnp
+    rng = np.random.default_rng(seed)
+    y = mu.y.ravel()
+    u0 = np.zeros((3, nz, n))
+    em, _ = edgemaps(np_)
+    ext = em[:mu.next]
+    for i in range(3):
+        u0[i] = rng.uniform(-amp, amp, (nz, n))
+        if i == 0:
+            u0[i] += 1.0 - y ** 2
+        for k in range(nz):
+            f = u0[i, k].reshape(mu.nel, -1)
+            g = np.zeros(mu.nglobal)
+            g[mu.btog] = f[:, ext]            # last writer wins on shared nodes
+            f[:, ext] = g[mu.btog]
+    return dict(mesh_u=mu, mesh_p=mp, nz=nz, beta=2 * np.pi / Lz, kinvis=nu, dt=dt,
+                const_force=(2 * nu, 0.0, 0.0), u0=u0, Lz=Lz)

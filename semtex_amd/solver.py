@@ -134,3 +134,54 @@ def helmholtz_op(M: ModalMatrixSys, u, out):
                                            C.c_void_p(out.data_ptr()), _stream()),
                "sx_helmholtz_op")
     return out
+
+
+SX_SKEW, SX_ALTSKEW = _lib.SX_SKEW, _lib.SX_ALTSKEW
+
+
+class DNS:
+    """integrate() (dns/integrate.cpp:66-201) on the GPU: the velocity and
+    pressure fields and their time history live in HBM; each step runs the
+    advection terms, the computed pressure BCs, the pressure Poisson solve,
+    the projection and the viscous Helmholtz solves."""
+
+    def __init__(self, gmesh_u: GpuMesh, gmesh_p: GpuMesh, nz, beta, kinvis, dt,
+                 const_force=(0.0, 0.0, 0.0), advection=SX_SKEW, nord=2):
+        L = _lib.load()
+        self.gu, self.gp = gmesh_u, gmesh_p
+        self.nz = nz
+        self.ncom = 2 if nz == 1 else 3
+        cf = (C.c_double * 3)(*const_force)
+        h = C.c_void_p()
+        _lib.check(L.sx_dns_create(gmesh_u.handle, gmesh_p.handle, nz, float(beta),
+                                   float(kinvis), float(dt), C.cast(cf, C.c_void_p),
+                                   advection, nord, _stream(), C.byref(h)), "sx_dns_create")
+        self.handle = h
+        self.stride = L.sx_dns_plane_stride(h)
+
+    def set_state(self, u_phys):
+        """u_phys: [ncom][nz][stride] CUDA float64, physical space."""
+        import torch
+        assert u_phys.dtype == torch.float64 and u_phys.is_cuda and u_phys.is_contiguous()
+        assert tuple(u_phys.shape) == (self.ncom, self.nz, self.stride)
+        _lib.check(_lib.load().sx_dns_set_state(self.handle, C.c_void_p(u_phys.data_ptr()),
+                                                _stream()), "sx_dns_set_state")
+
+    def step(self, n=1):
+        _lib.check(_lib.load().sx_dns_step(self.handle, int(n), _stream()), "sx_dns_step")
+        return self
+
+    def field(self, which):
+        """Fourier-space copy of velocity component `which` (ncom: pressure)."""
+        import torch
+        out = torch.empty((self.nz, self.stride), dtype=torch.float64, device="cuda")
+        st = C.c_int(0)
+        _lib.check(_lib.load().sx_dns_get(self.handle, int(which), C.c_void_p(out.data_ptr()),
+                                          C.byref(st), _stream()), "sx_dns_get")
+        self.step_no = st.value
+        return out
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.load().sx_dns_destroy(self.handle)
+            self.handle = None

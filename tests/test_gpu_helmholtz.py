@@ -206,3 +206,56 @@ def test_full_fourier_batch_parity_and_residual(torch_cuda, oracle):
         resid = np.concatenate([rb[:m.nsolve], r[:, em[m.next:]].ravel()])
         scale = np.linalg.norm((q8 * F[k, :n].reshape(m.nel, m.npnp)).ravel())
         assert np.linalg.norm(resid) < 1e-11 * scale, k
+
+
+def test_mixed_precision_refinement_reaches_fp64(torch_cuda, oracle):
+    """SX_MIXED: FP32 elemental condensation/recovery + FP64 band solve +
+    FP64 residual refinement agrees with the FP64 path to <= 1e-10 relative
+    on the full 16x16 N_P=10 N_Z=64 batch, and with
+    boundary data on a small periodic problem."""
+    import torch
+    from semtex_amd.mesh import rect_mesh
+    from semtex_amd.problems import fourier_forcing, helmholtz_fourier
+    from semtex_amd.solver import SX_MIXED, Field, GpuMesh, ModalMatrixSys
+    print("case: mixed precision 16x16 np=10 nz=64", flush=True)
+    P = helmholtz_fourier()
+    m, nz = P["mesh"], P["nz"]
+    gm = GpuMesh(m)
+    M64 = ModalMatrixSys(P["lambda2"], P["beta"], 0, P["nmodes"], gm)
+    M32 = ModalMatrixSys(P["lambda2"], P["beta"], 0, P["nmodes"], gm, precision=SX_MIXED)
+    F = torch.from_numpy(P["forcing"]).cuda()
+    u64, u32 = Field(gm, nz), Field(gm, nz)
+    u64.solve(F.clone(), M64)
+    u32.solve(F.clone(), M32, mixed_tol=1e-13, max_iter=10)
+    torch.cuda.synchronize()
+    n = m.nel * m.npnp
+    a, b = u32.data.cpu().numpy()[:, :n], u64.data.cpu().numpy()[:, :n]
+    planes = [k for k in range(nz) if k != 1]
+    rel = np.linalg.norm(a[planes] - b[planes]) / np.linalg.norm(b[planes])
+    print("mixed iterations", u32.last_iters, "rel", rel, flush=True)
+    assert rel < 1e-10
+    assert 1 <= u32.last_iters <= 10
+    # boundary data (essential + natural) on a small periodic problem
+    m2 = rect_mesh(6, 4, 7, 0.0, 2 * np.pi, -1.0, 1.0, bc=("D", "N", "N", "N"), periodic_x=True)
+    gm2 = GpuMesh(m2)
+    nz2 = 8
+    F2 = fourier_forcing(m2, nz2, seed=3)
+    lines = np.zeros((nz2, m2.nbound * m2.np))
+    for k in range(nz2):
+        if k == 1:
+            continue
+        for bb in range(m2.nbound):
+            xs, ys = m2.side_xy(bb)
+            lines[k, bb * m2.np:(bb + 1) * m2.np] = np.cos((k + 1) * xs) + 0.5 * (m2.bnd_kind[bb] == 1)
+    ua, ub = Field(gm2, nz2), Field(gm2, nz2)
+    ua.lines.copy_(torch.from_numpy(lines))
+    ub.lines.copy_(torch.from_numpy(lines))
+    Ma = ModalMatrixSys(2.0, 1.0, 0, nz2 // 2, gm2)
+    Mb = ModalMatrixSys(2.0, 1.0, 0, nz2 // 2, gm2, precision=SX_MIXED)
+    ua.solve(torch.from_numpy(F2).cuda(), Ma)
+    ub.solve(torch.from_numpy(F2).cuda(), Mb, mixed_tol=1e-13)
+    torch.cuda.synchronize()
+    n2 = m2.nel * m2.npnp
+    a2, b2 = ub.data.cpu().numpy()[:, :n2], ua.data.cpu().numpy()[:, :n2]
+    pl = [k for k in range(nz2) if k != 1]
+    assert np.linalg.norm(a2[pl] - b2[pl]) / np.linalg.norm(b2[pl]) < 1e-10
