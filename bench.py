@@ -51,6 +51,8 @@ def parse():
     p.add_argument("--dns-steps", type=int, default=100)
     p.add_argument("--dns-warmup", type=int, default=3)
     p.add_argument("--no-dns", action="store_true")
+    p.add_argument("--dns-only", action="store_true",
+                   help="run only the DNS leg (for kernel profiles)")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--mixed-tol", type=float, default=1e-11,
                    help="relative correction size that ends the mixed refinement")
@@ -211,6 +213,12 @@ def main():
     n = m.nel * m.npnp
     print(f"# mesh {m.info['nx']}x{m.info['ny']} np={m.np} nz={nz} nglobal={m.nglobal} "
           f"nsolve={m.nsolve} nband={m.nband} ({m.info['ordering']})", file=sys.stderr, flush=True)
+    if a.dns_only:
+        dn = dns_leg(a, torch)
+        print(json.dumps({"dns_steps_per_s": round(dn["value"], 3),
+                          "ms_per_step": round(dn["ms_per_step"], 4),
+                          "steps": a.dns_steps, "finite": dn["finite"]}), flush=True)
+        return
     gm = GpuMesh(m)
     f64 = helmholtz_leg(a, P, gm, SX_FP64, L, _lib, torch)
     print(f"# FP64: {f64['value']:.1f} mode-solves/s, {f64['ms_per_step']:.4f} ms/step, "
