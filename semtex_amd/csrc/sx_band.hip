@@ -316,6 +316,320 @@ int launch_sweeps (const sx_msys* s, hipStream_t st)
   return SX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Right-looking panel sweeps (default path, band offsets kd <= 511)
+// ---------------------------------------------------------------------------
+// The factor is cut into 64x64 blocks: row block J couples to column blocks
+// J..J+D, D = floor((kd+63)/64).  Panel J holds the inverse of the diagonal
+// block, T_J = U_JJ^-1, and the D off-diagonal blocks
+// U_{J,J+d} = U(64J+t, 64(J+d)+c) (zero outside the band).  A block is
+// stored as row pairs [t/2][c][2] so that lane c reads rows t and t+1 of
+// its column with one 16-byte load and a wavefront reads 1 KB contiguously.
+// A forward sweep U^T y = b is
+//   y_J = T_J^T r_J,    r_K -= U_{J,K}^T y_J   for K = J+1..J+D,
+// with r_K initialised to b_K.  Wave w (w < D) owns the pending column
+// block J+w at step J and holds the next step's 64x64 block in registers,
+// each pair reloaded right after its use, so a whole step's loads are in
+// flight behind the compute.  Every wave issues the same loads in the same
+// order (waves without a block read a zero panel), which keeps the memory
+// wait counts exact; the triangle is shared by all waves, TRW rows each.
+// One extra zero panel follows the nb panels of each mode.
+constexpr int PB = 64;             // panel block
+constexpr int PMAXD = 8;           // off-diagonal blocks per panel supported
+
+__device__ __forceinline__ void lds_sync ()
+{
+  // LDS-only workgroup barrier: global loads stay in flight across it
+  asm volatile ("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+__device__ __forceinline__ size_t pidx (int t, int c)
+{
+  return ((size_t) (t >> 1) * PB + c) * 2 + (t & 1);
+}
+
+// T_J = inverse of the diagonal block (zero outside the band, identity
+// past row n-1), one wavefront per (J, mode); lane c back-substitutes
+// column c.  J = nb writes the zero panel.
+__global__ __launch_bounds__(64) void k_panel_tinv (
+    const double* __restrict__ band, size_t band_stride, int ld,
+    const int* __restrict__ nsolve, double* P, size_t pstride, int D)
+{
+  __shared__ double T[PB * (PB + 1)], X[PB * (PB + 1)];
+  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
+  const int J = blockIdx.x, j0 = J * PB, c = threadIdx.x;
+  double* out = P + (size_t) mode * pstride + (size_t) J * (D + 1) * PB * PB;
+  const int bs = max (0, min (PB, n - j0));
+  const double* A = band + (size_t) mode * band_stride;
+  for (int t = 0; t < PB; t++) {
+    double v = 0.0;
+    if (c < bs && t <= c && c - t <= kd) v = A[(size_t) (j0 + c) * ld + kd - c + t];
+    T[t * (PB + 1) + c] = v;
+  }
+  if (c >= bs) T[c * (PB + 1) + c] = 1.0;
+  __syncthreads ();
+  X[c * (PB + 1) + c] = 1.0 / T[c * (PB + 1) + c];
+  for (int t = c - 1; t >= 0; t--) {
+    double sum = 0.0;
+    for (int k = t + 1; k <= c; k++) sum += T[t * (PB + 1) + k] * X[k * (PB + 1) + c];
+    X[t * (PB + 1) + c] = -sum / T[t * (PB + 1) + t];
+  }
+  for (int t = 0; t < PB; t++)
+    out[pidx (t, c)] = (c < bs && t <= c) ? X[t * (PB + 1) + c] : 0.0;
+}
+
+// Off-diagonal panel blocks U(64J+t, 64(J+d)+c), d = 1..D, J = 0..nb
+// (J = nb: the zero panel).
+__global__ void k_panel_offdiag (const double* __restrict__ band, size_t band_stride,
+                                 int ld, const int* __restrict__ nsolve, double* P,
+                                 size_t pstride, int D, int nbp)
+{
+  const int mode = blockIdx.y, n = nsolve[mode], kd = ld - 1;
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) nbp * D * PB * PB) return;
+  const int J = (int) (idx / ((size_t) D * PB * PB));
+  const int rem = (int) (idx % ((size_t) D * PB * PB));
+  const int d = 1 + rem / (PB * PB), t = (rem % (PB * PB)) / PB, c = rem % PB;
+  const int i = J * PB + t, j = (J + d) * PB + c;
+  double v = 0.0;
+  if (j < n && j - i <= kd) v = band[(size_t) mode * band_stride + (size_t) j * ld + kd - (j - i)];
+  P[(size_t) mode * pstride + ((size_t) J * (D + 1) + d) * PB * PB + pidx (t, c)] = v;
+}
+
+// wavefronts per pending column block: two (rows 0-31 and 32-63) while
+// 2D waves fit the register file at two waves per SIMD, else one
+constexpr int panel_h (int D) { return D > 0 ? 2 : 2; }
+constexpr int panel_nw (int D) { return panel_h (D) * D < 4 ? 4 : panel_h (D) * D; }
+
+template <int NR, int D>
+struct PanelSweep {
+  static constexpr int H = panel_h (D);
+  static constexpr int NW = panel_nw (D);
+  static constexpr int TRW = ((PB + NW - 1) / NW + 1) & ~1;  // triangle rows per wave
+  static constexpr int HB = PB / H;                          // block rows per wave
+};
+
+template <int NR, int D>
+__global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax)
+{
+  constexpr int NW = PanelSweep<NR, D>::NW, TRW = PanelSweep<NR, D>::TRW;
+  constexpr int HB = PanelSweep<NR, D>::HB, H = PanelSweep<NR, D>::H;
+  extern __shared__ double sm[];
+  double* ybuf = sm;                                // [64][NR] y of the last block
+  double* rbuf = ybuf + PB * NR;                    // [2][D][64][NR] pending r, by half
+  double* part = rbuf + 2 * D * PB * NR;            // [NW][64][NR] triangle partials
+  const int mode = blockIdx.x, n = nsolve[mode];
+  if (n == 0) return;
+  const int nb = (n + PB - 1) / PB;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
+  const int slot = w / H, half = w % H;             // pending block J+slot, rows half*HB..
+  const bool owner = slot < D;
+  double* bg = rhs + (size_t) mode * NR * nmax;
+  const size_t PS = (size_t) (D + 1) * PB * PB;     // doubles per panel
+  double u[HB], tr[TRW], bp[NR];
+
+  for (int pass = 0; pass < 2; pass++) {
+    const double* P = (pass == 0 ? PF : PR) + (size_t) mode * pstride;
+    const double* zero = P + (size_t) nb * PS + (size_t) PB * PB;    // a zero block
+    auto gidx = [&] (int j) __attribute__ ((always_inline)) {
+      j = min (j, n - 1);
+      return pass ? n - 1 - j : j;
+    };
+    // this wave's rows of block U_{J, J+1+slot} (or zeros)
+    auto ublk = [&] (int J) __attribute__ ((always_inline)) {
+      return (owner ? P + (size_t) J * PS + (size_t) (slot + 1) * PB * PB : zero) +
+             (size_t) half * HB * PB + 2 * lane;
+    };
+    // triangle rows [w*TRW, w*TRW+TRW) of T_J (rows past 63 re-read row 63)
+    auto trow = [&] (int J, int k) __attribute__ ((always_inline)) {
+      const int t = min (w * TRW + k, PB - 2);
+      return P + (size_t) J * PS + pidx (t, lane);
+    };
+    auto load_b = [&] (int K) __attribute__ ((always_inline)) {
+#pragma unroll
+      for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * nmax + gidx (K * PB + lane)];
+    };
+    // triangle partials of block J, then prefetch of T_{J+1}
+    auto tri = [&] (int J) __attribute__ ((always_inline)) {
+      const double* r0 = rbuf + (size_t) (J % D) * PB * NR;
+      const double* r1 = r0 + (size_t) D * PB * NR;
+      const int Jn = min (J + 1, nb - 1);
+      double pp[NR];
+#pragma unroll
+      for (int r = 0; r < NR; r++) pp[r] = 0.0;
+#pragma unroll
+      for (int k = 0; k < TRW; k += 2) {
+        const double t0 = tr[k], t1 = tr[k + 1];
+        const int ta = min (w * TRW + k, PB - 2);
+        const double wt = (w * TRW + k < PB) ? 1.0 : 0.0;   // rows past 63 add nothing
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          pp[r] += wt * (t0 * (r0[ta * NR + r] + r1[ta * NR + r]) +
+                         t1 * (r0[(ta + 1) * NR + r] + r1[(ta + 1) * NR + r]));
+        // reload behind the use (offset tied to the sums, as for the blocks)
+        int off = (int) ((const char*) trow (Jn, k) - (const char*) P);
+#pragma unroll
+        for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(pp[r]));
+        const double2 v = *(const double2*) ((const char*) P + off);
+        tr[k] = v.x;
+        tr[k + 1] = v.y;
+      }
+#pragma unroll
+      for (int r = 0; r < NR; r++) part[(w * PB + lane) * NR + r] = pp[r];
+    };
+    auto finish = [&] (int J) __attribute__ ((always_inline)) {
+      if (w == 0) {
+        const int j = J * PB + lane;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          double y = 0.0;
+#pragma unroll
+          for (int g = 0; g < NW; g++) y += part[(g * PB + lane) * NR + r];
+          ybuf[lane * NR + r] = y;
+          if (j < n) bg[(size_t) r * nmax + gidx (j)] = y;
+        }
+      }
+    };
+
+    // r of blocks 0..D-1 from b (pass 1: the forward result, reversed);
+    // the second half's partial sums start at zero
+    if (owner) {
+      const bool in = slot * PB + lane < n;
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        rbuf[((size_t) (half * D + slot) * PB + lane) * NR + r] =
+            (in && half == 0) ? bg[(size_t) r * nmax + gidx (slot * PB + lane)] : 0.0;
+    }
+    // prefetch in the steady-state order: T_0 rows, U_{0,1+slot}, b of block D
+#pragma unroll
+    for (int k = 0; k < TRW; k += 2) {
+      const double2 v = *(const double2*) trow (0, k);
+      tr[k] = v.x;
+      tr[k + 1] = v.y;
+    }
+    {
+      const double* pu = ublk (0);
+#pragma unroll
+      for (int t = 0; t < HB; t += 2) {
+        const double2 v = *(const double2*) (pu + (size_t) t * PB);
+        u[t] = v.x;
+        u[t + 1] = v.y;
+      }
+    }
+    load_b (D);
+    // drain once per pass: the loop's wait counts then follow its own,
+    // fixed load order
+    __builtin_amdgcn_s_waitcnt (0);
+    __syncthreads ();
+
+    // step J: y_J from the triangle, then r_{J+1+slot} -= U_{J,J+1+slot}^T y_J
+    for (int J = 0; J < nb; J++) {
+      tri (J);
+      lds_sync ();
+      finish (J);
+      lds_sync ();
+      // U_{J+1,J+2+slot} is loaded pair by pair right behind the use: the
+      // load offset is tied to the running sums, so the loads cannot be
+      // hoisted above the FMAs (which would need a second register buffer)
+      {
+        const int K = J + 1 + slot;
+        const double* yb = ybuf + half * HB * NR;
+        double acc0[NR], acc1[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) acc0[r] = acc1[r] = 0.0;
+        const double* pn = ublk (J + 1);
+#pragma unroll
+        for (int t = 0; t < HB; t += 2) {
+          const double u0 = u[t], u1 = u[t + 1];
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            acc0[r] += u0 * yb[t * NR + r];
+            acc1[r] += u1 * yb[(t + 1) * NR + r];
+          }
+          int off = t * PB * (int) sizeof (double);
+#pragma unroll
+          for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(acc0[r]), "v"(acc1[r]));
+          const double2 v = *(const double2*) ((const char*) pn + off);
+          u[t] = v.x;
+          u[t + 1] = v.y;
+        }
+        double acc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          acc[r] = acc0[r] + acc1[r];
+          // bp is read only after the sums (its wait must not move up)
+          asm volatile ("" : "+v"(bp[r]) : "v"(acc[r]));
+        }
+        if (owner) {
+          double* rp = rbuf + ((size_t) (half * D + K % D) * PB + lane) * NR;
+          if (slot == D - 1) {                // block K enters the window now
+            const bool in = K * PB + lane < n && half == 0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) rp[r] = (in ? bp[r] : 0.0) - acc[r];
+          } else {
+#pragma unroll
+            for (int r = 0; r < NR; r++) rp[r] -= acc[r];
+          }
+        }
+        // b of the block entering next step, loaded after the use of bp
+        {
+          int off = 0;
+#pragma unroll
+          for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(acc[r]), "v"(bp[r]));
+          const int gi = gidx ((K + 1) * PB + lane) + off;
+#pragma unroll
+          for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * nmax + gi];
+        }
+      }
+      lds_sync ();
+    }
+    // the backward pass reads what the forward pass wrote to global memory
+    __threadfence ();
+    __syncthreads ();
+  }
+}
+
+int panel_D (int nband)
+{
+  const int kd = nband - 1;
+  return std::max (1, (kd + PB - 1) / PB);
+}
+
+template <int NR, int D>
+int launch_panel_d (const sx_msys* s, hipStream_t st)
+{
+  constexpr int NW = PanelSweep<NR, D>::NW;
+  const size_t lds = sizeof (double) * ((size_t) PB * NR * (1 + 2 * D + NW));
+  auto kern = k_panel_sweeps<NR, D>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (s->nmodes), dim3 (NW * 64), lds, st, s->d_swF, s->d_swR,
+                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+template <int NR>
+int launch_panel (const sx_msys* s, hipStream_t st)
+{
+  switch (s->sw_D) {
+  case 1: return launch_panel_d<NR, 1> (s, st);
+  case 2: return launch_panel_d<NR, 2> (s, st);
+  case 3: return launch_panel_d<NR, 3> (s, st);
+  case 4: return launch_panel_d<NR, 4> (s, st);
+  case 5: return launch_panel_d<NR, 5> (s, st);
+  case 6: return launch_panel_d<NR, 6> (s, st);
+  case 7: return launch_panel_d<NR, 7> (s, st);
+  case 8: return launch_panel_d<NR, 8> (s, st);
+  default: break;
+  }
+  set_error ("panel sweep: unsupported block count");
+  return SX_ERR_LIMIT;
+}
+
 // rows per wavefront, rounded up to a multiple of 4
 #define SX_QS_CASES(B) \
   switch (qs) { \
@@ -353,8 +667,17 @@ static int rows_per_wave (int nband)
   return qs;
 }
 
+static bool use_panels (int nband)
+{
+  if (const char* e = std::getenv ("SX_BAND_SWEEP"))
+    if (std::string (e) == "offset") return false;
+  return panel_D (nband) <= PMAXD;
+}
+
 size_t band_sweep_bytes_per_mode (int nmax, int nband)
 {
+  if (use_panels (nband))
+    return (size_t) ((nmax + PB - 1) / PB + 1) * (panel_D (nband) + 1) * PB * PB;
   const int B = band_block (nband);
   return (size_t) ((nmax + B - 1) / B) * nband * B;
 }
@@ -366,7 +689,8 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
                       sizeof (double) * std::max (ld, 1), st, s->d_bandU,
                       s->band_stride, ld, s->d_nsolve, d_status);
   SX_HIP (hipGetLastError ());
-  const int B = band_block (ld);
+  const bool panels = use_panels (ld);
+  const int B = panels ? PB : band_block (ld);
   const int nblk = (s->nmax + B - 1) / B;
   double* V = nullptr;
   SX_HIP (hipMalloc (&V, sizeof (double) * s->nmodes * s->band_stride));
@@ -374,6 +698,32 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
   hipLaunchKernelGGL (k_band_reverse,
                       dim3 ((unsigned) ((s->band_stride + 255) / 256), s->nmodes),
                       dim3 (256), 0, st, s->d_bandU, V, s->band_stride, ld, s->d_nsolve);
+  if (panels) {
+    const int D = panel_D (ld);
+    s->sw_panel = 1;
+    s->sw_D = D;
+    s->sw_block = PB;
+    s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
+    SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
+    SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * s->nmodes * s->sw_stride));
+    const double* src[2] = {s->d_bandU, V};
+    double* dst[2] = {s->d_swF, s->d_swR};
+    // panels 0..nblk (per-mode nb <= nblk; everything past a mode's own
+    // blocks is zero, including its zero panel at index nb)
+    for (int k = 0; k < 2; k++) {
+      hipLaunchKernelGGL (k_panel_tinv, dim3 (nblk + 1, s->nmodes), dim3 (64), 0, st, src[k],
+                          s->band_stride, ld, s->d_nsolve, dst[k], s->sw_stride, D);
+      const size_t tot = (size_t) (nblk + 1) * D * PB * PB;
+      hipLaunchKernelGGL (k_panel_offdiag, dim3 ((unsigned) ((tot + 255) / 256), s->nmodes),
+                          dim3 (256), 0, st, src[k], s->band_stride, ld, s->d_nsolve,
+                          dst[k], s->sw_stride, D, nblk + 1);
+    }
+    SX_HIP (hipGetLastError ());
+    SX_HIP (hipStreamSynchronize (st));
+    (void) hipFree (V);
+    return SX_OK;
+  }
+  s->sw_panel = 0;
   if (nblk > 0) {
     hipLaunchKernelGGL (k_band_tinv, dim3 (nblk, s->nmodes), dim3 (64), 0, st,
                         s->d_bandU, s->band_stride, ld, B, s->d_nsolve);
@@ -399,6 +749,7 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
 
 int band_solve (const sx_msys* s, hipStream_t st)
 {
+  if (s->sw_panel) return launch_panel<2> (s, st);
   const int qs = rows_per_wave (s->m->nband);
   switch (s->sw_block) {
   case 64: SX_QS_CASES (64) break;

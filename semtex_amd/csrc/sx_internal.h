@@ -96,6 +96,8 @@ struct sx_msys {
   double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband
                                // (assembly + factorisation; freed after set-up)
   int sw_block = 0;            // sweep block size B
+  int sw_panel = 0;            // 1: right-looking panel sweeps (sw_D off-diagonal blocks)
+  int sw_D = 0;
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks
