@@ -37,6 +37,7 @@ extern "C" {
 
 #define SX_FP64  0           /* solver precisions                          */
 #define SX_MIXED 1           /* FP32 elemental applies + FP64 refinement   */
+#define SX_JACPCG 2          /* Jacobi-preconditioned CG (SolverKind JACPCG) */
 
 typedef struct sx_mesh*  sx_mesh_t;   /* geometry + numbering, on the GPU  */
 typedef struct sx_msys*  sx_msys_t;   /* factorised per-mode systems       */
@@ -77,7 +78,11 @@ int sx_mesh_nclass (sx_mesh_t m);
  * base_mode+num_modes-1, with betak2 = (beta*mode)^2
  * (Field::modeConstant, Cartesian), all on the GPU.  A mode whose
  * lambda2+betak2 vanishes with no essential nodes is singular and drops its
- * last unknown (src/matrix.cpp:154-162).  precision: SX_FP64 or SX_MIXED. */
+ * last unknown (src/matrix.cpp:154-162).  precision: SX_FP64 or SX_MIXED
+ * (DIRECT), or SX_JACPCG: MatrixSys with method JACPCG
+ * (src/matrix.cpp:269-310), which holds only the Jacobi preconditioner
+ * (inverse of the assembled diagonal, Element::HelmholtzDiag
+ * src/element.cpp:1484-1533) and is solved with sx_field_solve_pcg. */
 int sx_msys_create (sx_mesh_t m, double lambda2, double beta, int base_mode,
                     int num_modes, int precision, void* stream,
                     sx_msys_t* out);
@@ -101,6 +106,16 @@ int sx_field_solve_mixed (sx_msys_t s, int nz, int plane_stride, double* d_f,
                           double refine_tol, int max_iter, int* iters,
                           void* stream);
 
+/* Field::solve, JACPCG branch (src/field.cpp:711-805), every solved plane
+ * at once, each with its own stopping test: d_u holds the initial guess on
+ * entry (the field's current values, as in the reference) and the solution
+ * on exit; iterations stop when |r|^2 <= (tol_rel |b|)^2 (TOL_REL) or the
+ * iteration index reaches step_max (STEP_MAX).  *iters (may be NULL)
+ * returns the largest iteration count over the planes. */
+int sx_field_solve_pcg (sx_msys_t s, int nz, int plane_stride, double* d_f, double* d_u,
+                        const double* d_lines, double tol_rel, int step_max, int* iters,
+                        void* stream);
+
 /* Host-pointer convenience wrapper for drop-in callers whose Field data
  * lives in host memory (Domain::u, src/domain.cpp:75-79): copies in,
  * solves, copies u back (f is not copied back). */
@@ -112,6 +127,15 @@ int sx_field_solve_host (sx_msys_t s, int nz, int plane_stride,
  * betak2 as in sx_msys_create.  Device planes [nz][plane_stride]. */
 int sx_helmholtz_op (sx_msys_t s, int nz, int plane_stride, const double* d_u,
                      double* d_hu, void* stream);
+
+/* Real transforms along z of every line of nz planes of stride
+ * plane_stride (AuxField::transform src/auxfield.cpp:1157-1194 -> dDFTr
+ * femlib/fourier.c:40-165): sign +1 physical -> Fourier, packed
+ * [Re X0, Re X_nz/2, Re X1, Im X1, ...] and scaled 1/nz; sign -1 the
+ * inverse, unscaled.  nz == 1 or even.  In place (d_in == d_out) when nz is
+ * a power of two in 8..1024; other nz need distinct buffers. */
+int sx_fourier_transform (int nz, int plane_stride, const double* d_in, double* d_out,
+                          int sign, void* stream);
 
 /* ------------------------------------------------------------------------
  * DNS time step: integrate() (dns/integrate.cpp:66-201) for Cartesian

@@ -47,6 +47,9 @@ def lib():
         L.or_pbtrf.restype = C.c_int
         L.or_pbtrs.argtypes = [C.c_int, C.c_int, dp, C.c_int, dp]
         L.or_msys_build.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+        L.or_pcg_diag.argtypes = [C.c_void_p, C.c_void_p, dp]
+        L.or_solve_plane_pcg.argtypes = [C.c_void_p, C.c_void_p, dp, dp, dp, C.c_double, C.c_int]
+        L.or_solve_plane_pcg.restype = C.c_int
         L.or_msys_build.restype = C.c_int
         L.or_msys_free.argtypes = [C.c_void_p]
         L.or_side_geom.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp, dp, dp]
@@ -216,6 +219,25 @@ class OracleMsys:
         u = np.zeros(m.nel * m.np * m.np)
         lib().or_solve_plane(C.byref(self.om.s), C.byref(self.s), f, line, u)
         return u
+
+    def solve_plane_pcg(self, f, line=None, u0=None, tol_rel=1e-8, step_max=500):
+        """JACPCG solve of one plane (src/field.cpp:711-805) from the initial
+        guess u0 (zero if None); returns (u, iterations)."""
+        m = self.om.m
+        f = np.ascontiguousarray(f, np.float64).reshape(-1).copy()
+        line = np.zeros(max(m.nbound * m.np, 1)) if line is None else \
+            np.ascontiguousarray(line, np.float64).reshape(-1)
+        u = np.zeros(m.nel * m.np * m.np) if u0 is None else \
+            np.ascontiguousarray(u0, np.float64).reshape(-1).copy()
+        it = lib().or_solve_plane_pcg(C.byref(self.om.s), C.byref(self.s), f, line, u,
+                                      float(tol_rel), int(step_max))
+        return u, it
+
+    def pcg_diag(self):
+        m = self.om.m
+        pc = np.zeros(m.nglobal + m.nel * (m.np - 2) ** 2)
+        lib().or_pcg_diag(C.byref(self.om.s), C.byref(self.s), pc)
+        return pc
 
     def __del__(self):
         try:

@@ -649,6 +649,201 @@ int or_solve_plane (const or_mesh* M, const or_msys* S, double* f,
 }
 
 /* ------------------------------------------------------------------------ */
+/* JACPCG path.                                                              */
+/* ------------------------------------------------------------------------ */
+/* src/element.cpp:1484-1533 Element::HelmholtzDiag, row-major (the emap
+ * gather is done by the caller).  The cross term reads the derivative
+ * matrix at linear indices j*j and i*i, as the reference does; it only
+ * contributes where Q3 is nonzero (non-affine elements).                  */
+static void helmholtz_diag (int np, int cyl, const double* geo, const double* y,
+                            double lambda2, double betak2, double* dg)
+{
+  const int npnp = np * np;
+  double *z = malloc (sizeof (double) * np), *w = malloc (sizeof (double) * np),
+         *D = malloc (sizeof (double) * npnp);
+  const double *Q1 = geo + OR_Q1 * npnp, *Q2 = geo + OR_Q2 * npnp,
+               *Q3 = geo + OR_Q3 * npnp, *Q4 = geo + OR_Q4 * npnp;
+  const int q3 = !is_zero (Q3, npnp);
+  int i, j, k;
+  or_gll (np, z, w, D, 0);
+  for (i = 0; i < np; i++)
+    for (j = 0; j < np; j++) {
+      const int ij = i * np + j;
+      double hcon, d = 0.0;
+      if (cyl) { const double r2 = y[ij] * y[ij]; hcon = (r2 > EPSDP) ? betak2 / r2 + lambda2 : 0.0; }
+      else hcon = lambda2 + betak2;
+      for (k = 0; k < np; k++) d += Q1[i * np + k] * sqr (D[k * np + j]);
+      for (k = 0; k < np; k++) d += Q2[k * np + j] * sqr (D[k * np + i]);
+      if (q3) d += 2.0 * Q3[ij] * D[j * j] * D[i * i];
+      d += hcon * Q4[ij];
+      dg[ij] = d;
+    }
+  free (z); free (w); free (D);
+}
+
+/* src/matrix.cpp:269-310: inverse of the assembled diagonal, globally
+ * numbered (nglobal boundary values, then nel blocks of nint interior
+ * values in emap order).                                                   */
+void or_pcg_diag (const or_mesh* M, const or_msys* S, double* pc)
+{
+  const int np = M->np, npnp = np * np, next = 4 * (np - 1), nint = npnp - next,
+            nglobal = M->nglobal, npts = nglobal + M->nel * nint;
+  double* dg = malloc (sizeof (double) * npnp);
+  int *emap = malloc (sizeof (int) * npnp), *pmap = malloc (sizeof (int) * npnp);
+  int e, i;
+  or_edgemaps (np, emap, pmap);
+  memset (pc, 0, sizeof (double) * npts);
+  for (e = 0; e < M->nel; e++) {
+    const int* bmap = M->btog + e * next;
+    helmholtz_diag (np, M->cyl, S->geo + (size_t) e * OR_NGEO * npnp, M->y + e * npnp,
+                    S->lambda2, S->betak2, dg);
+    for (i = 0; i < next; i++) pc[bmap[i]] += dg[emap[i]];
+    for (i = 0; i < nint; i++) pc[nglobal + e * nint + i] = dg[emap[next + i]];
+  }
+  for (i = 0; i < npts; i++) pc[i] = 1.0 / pc[i];
+  free (dg); free (emap); free (pmap);
+}
+
+/* src/field.cpp:855-940 Field::HelmholtzOperator (no mixed BCs): y = A x
+ * on globally numbered vectors, element by element.                       */
+static void global_helmholtz (const or_mesh* M, const or_msys* S, const int* emap,
+                              const double* x, double* y, double* P, double* work)
+{
+  const int np = M->np, npnp = np * np, next = 4 * (np - 1), nint = npnp - next,
+            nglobal = M->nglobal, npts = nglobal + M->nel * nint;
+  int e, i;
+  memset (y, 0, sizeof (double) * npts);
+  for (e = 0; e < M->nel; e++) {
+    const int* bmap = M->btog + e * next;
+    for (i = 0; i < next; i++) P[emap[i]] = x[bmap[i]];
+    for (i = 0; i < nint; i++) P[emap[next + i]] = x[nglobal + e * nint + i];
+    or_helmholtz_op (np, M->cyl, S->geo + (size_t) e * OR_NGEO * npnp, M->y + e * npnp,
+                     S->lambda2, S->betak2, P, work);
+    for (i = 0; i < next; i++) y[bmap[i]] += work[emap[i]];
+    for (i = 0; i < nint; i++) y[nglobal + e * nint + i] += work[emap[next + i]];
+  }
+}
+
+static double dot (int n, const double* a, const double* b)
+{
+  double s = 0.0;
+  int i;
+  for (i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+/* src/field.cpp:711-805 Field::solve, JACPCG branch, one plane.  f is
+ * destroyed; u holds the initial guess on entry and the solution on exit.
+ * Returns the iteration count i of the reference's loop.                  */
+int or_solve_plane_pcg (const or_mesh* M, const or_msys* S, double* f, const double* line,
+                        double* u, double tol_rel, int step_max)
+{
+  const int np = M->np, npnp = np * np, next = 4 * (np - 1), nint = npnp - next,
+            nel = M->nel, nglobal = M->nglobal, nsolve = S->nsolve,
+            npts = nglobal + nel * nint;
+  double *r = calloc (npts, sizeof (double)), *p = calloc (npts, sizeof (double)),
+         *q = calloc (npts, sizeof (double)), *x = calloc (npts, sizeof (double)),
+         *z = calloc (npts, sizeof (double)), *pc = malloc (sizeof (double) * npts),
+         *g = malloc (sizeof (double) * npnp), *work = malloc (sizeof (double) * npnp),
+         *P = malloc (sizeof (double) * npnp), *area = malloc (sizeof (double) * np),
+         *nx = malloc (sizeof (double) * np), *ny = malloc (sizeof (double) * np);
+  int *emap = malloc (sizeof (int) * npnp), *pmap = malloc (sizeof (int) * npnp);
+  int e, i, b, k, it, idx[64];
+  double epsb2, r2, rho1, rho2 = 1.0, alpha, beta, dotp;
+
+  or_edgemaps (np, emap, pmap);
+  or_pcg_diag (M, S, pc);
+
+  /* getEssential into x, constrain f <- -(M f + H g) */
+  get_essential (M, line, x);
+  for (e = 0; e < nel; e++) {
+    const double* geo = S->geo + (size_t) e * OR_NGEO * npnp;
+    double* fe = f + (size_t) e * npnp;
+    for (i = 0; i < npnp; i++) fe[i] *= geo[OR_Q8 * npnp + i];
+    if (M->emask[e]) {
+      const int* bmap = M->btog + e * next;
+      memset (g, 0, sizeof (double) * npnp);
+      for (i = 0; i < next; i++) g[emap[i]] = x[bmap[i]];
+      or_helmholtz_op (np, M->cyl, geo, M->y + e * npnp, S->lambda2, S->betak2, g, work);
+      for (i = 0; i < npnp; i++) fe[i] += work[i];
+    }
+    for (i = 0; i < npnp; i++) fe[i] = -fe[i];
+  }
+  /* buildRHS with element interiors (local2globalSum) + <h, w>, essential
+   * tail zeroed (src/field.cpp:943-1009)                                   */
+  for (e = 0; e < nel; e++) {
+    const int* bmap = M->btog + e * next;
+    const double* fe = f + (size_t) e * npnp;
+    for (i = 0; i < next; i++) r[bmap[i]] += fe[emap[i]];
+    for (i = 0; i < nint; i++) r[nglobal + e * nint + i] += fe[emap[next + i]];
+  }
+  for (b = 0; b < M->nbound; b++) {
+    const int el = M->bnd_elmt[b];
+    const int* bmap = M->btog + el * next;
+    if (M->bnd_kind[b] != OR_NATURAL) continue;
+    or_side_geom (np, S->geo + (size_t) el * OR_NGEO * npnp, M->x + el * npnp,
+                  M->y + el * npnp, M->bnd_side[b], area, nx, ny);
+    side_nodes (np, M->bnd_side[b], idx);
+    for (k = 0; k < np; k++) r[bmap[idx[k]]] += line[b * np + k] * area[k];
+  }
+  for (i = nsolve; i < nglobal; i++) r[i] = 0.0;
+
+  epsb2 = tol_rel * sqrt (dot (npts, r, r));
+  epsb2 *= epsb2;
+
+  /* initial guess from the field (local2global, last element writes) */
+  for (e = 0; e < nel; e++) {
+    const int* bmap = M->btog + e * next;
+    const double* ue = u + (size_t) e * npnp;
+    for (i = 0; i < next; i++) x[bmap[i]] = ue[emap[i]];
+    for (i = 0; i < nint; i++) x[nglobal + e * nint + i] = ue[emap[next + i]];
+  }
+  for (i = nsolve; i < nglobal; i++) x[i] = 0.0;
+  global_helmholtz (M, S, emap, x, p, P, work);
+  for (i = nsolve; i < nglobal; i++) { p[i] = 0.0; r[i] = 0.0; }
+  for (i = 0; i < npts; i++) r[i] -= p[i];
+  r2 = dot (npts, r, r);
+
+  it = 0;
+  while (r2 > epsb2 && ++it < step_max) {
+    for (i = 0; i < npts; i++) z[i] = pc[i] * r[i];
+    rho1 = dot (npts, r, z);
+    if (it == 1) memcpy (p, z, sizeof (double) * npts);
+    else {
+      beta = rho1 / rho2;
+      for (i = 0; i < npts; i++) p[i] = beta * p[i] + z[i];
+    }
+    global_helmholtz (M, S, emap, p, q, P, work);
+    for (i = nsolve; i < nglobal; i++) q[i] = 0.0;
+    dotp = dot (npts, p, q);
+    alpha = rho1 / dotp;
+    for (i = 0; i < npts; i++) x[i] += alpha * p[i];
+    for (i = 0; i < npts; i++) r[i] -= alpha * q[i];
+    rho2 = rho1;
+    r2 = dot (npts, r, r);
+  }
+
+  /* global2local, then the essential values (getEssential, setEssential) */
+  for (e = 0; e < nel; e++) {
+    const int* bmap = M->btog + e * next;
+    double* ue = u + (size_t) e * npnp;
+    for (i = 0; i < next; i++) ue[emap[i]] = x[bmap[i]];
+    for (i = 0; i < nint; i++) ue[emap[next + i]] = x[nglobal + e * nint + i];
+  }
+  get_essential (M, line, x);
+  for (e = 0; e < nel; e++) {
+    const int* bmap = M->btog + e * next;
+    const int* bm = M->bmask + e * next;
+    double* ue = u + (size_t) e * npnp;
+    if (!M->emask[e]) continue;
+    for (i = 0; i < next; i++) if (bm[i]) ue[emap[i]] = x[bmap[i]];
+  }
+  free (r); free (p); free (q); free (x); free (z); free (pc); free (g); free (work);
+  free (P); free (area); free (nx); free (ny); free (emap); free (pmap);
+  return it;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Real DFT along z (femlib/fourier.c:40-165).  Direct O(N^2) evaluation,
  * exp(-i) convention for the forward transform, scaled by 1/N.            */
 /* ------------------------------------------------------------------------ */

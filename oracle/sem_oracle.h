@@ -120,6 +120,14 @@ int or_solve_plane (const or_mesh* M, const or_msys* S,
 /* Real DFT of nlines interleaved lines of length nz along z with stride
  * `ntrn` (femlib/fourier.c:40-165 dDFTr): sign +1 forward (scaled 1/nz),
  * -1 inverse.  Packing [Re0, ReNyq, Re1, Im1, ...]. */
+/* JACPCG (src/field.cpp:711-805, src/matrix.cpp:269-310): inverse of the
+ * assembled diagonal (globally numbered, nglobal + nel*nint), and the
+ * preconditioned CG solve of one plane from the initial guess in u.
+ * Returns the iteration count.                                            */
+void or_pcg_diag (const or_mesh* M, const or_msys* S, double* pc);
+int  or_solve_plane_pcg (const or_mesh* M, const or_msys* S, double* f, const double* line,
+                         double* u, double tol_rel, int step_max);
+
 void or_dft (double* data, int nz, int ntrn, int sign);
 
 #ifdef __cplusplus

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsemtex_amd.so")
 
 SX_OK = 0
-SX_FP64, SX_MIXED = 0, 1
+SX_FP64, SX_MIXED, SX_JACPCG = 0, 1, 2
 SX_SKEW, SX_ALTSKEW = 0, 1
 
 # Every symbol include/semtex_amd.h declares, with (restype, argtypes).
@@ -33,6 +33,8 @@ SIGNATURES = {
                                   C.POINTER(_i), _vp]),
     "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
+    "sx_fourier_transform": (_i, [_i, _i, _dp, _dp, _i, _vp]),
+    "sx_field_solve_pcg": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i, C.POINTER(_i), _vp]),
     "sx_dns_create": (_i, [_vp, _vp, _i, _d, _d, _d, _dp, _i, _i, _vp, C.POINTER(_vp)]),
     "sx_dns_destroy": (_i, [_vp]),
     "sx_dns_set_state": (_i, [_vp, _dp, _vp]),

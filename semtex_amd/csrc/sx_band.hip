@@ -747,9 +747,22 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
   return SX_OK;
 }
 
-int band_solve (const sx_msys* s, hipStream_t st)
+int band_solve (const sx_msys* s, int nr, hipStream_t st)
 {
-  if (s->sw_panel) return launch_panel<2> (s, st);
+  if (s->sw_panel) {
+    switch (nr) {
+    case 2: return launch_panel<2> (s, st);
+    case 4: return launch_panel<4> (s, st);
+    case 6: return launch_panel<6> (s, st);
+    default: break;
+    }
+    set_error ("band solve: 2, 4 or 6 right-hand sides per mode");
+    return SX_ERR_ARG;
+  }
+  if (nr != 2) {
+    set_error ("band solve: the offset-major sweeps take 2 right-hand sides per mode");
+    return SX_ERR_ARG;
+  }
   const int qs = rows_per_wave (s->m->nband);
   switch (s->sw_block) {
   case 64: SX_QS_CASES (64) break;

@@ -30,36 +30,6 @@ namespace {
 
 constexpr double EPSDP = 6.0e-14;
 
-// ---------------------------------------------------------------------------
-// z transforms: out[k][s] = sum_n M[k][n] in[n][s], s < S (plane stride).
-// One thread per column s and group of KG output planes; the KG rows of M
-// are staged in LDS.
-// ---------------------------------------------------------------------------
-constexpr int KG = 16;
-__global__ __launch_bounds__(256) void k_dft (const double* __restrict__ M,
-                                              const double* __restrict__ in,
-                                              double* out, int nz, int S)
-{
-  extern __shared__ double Ms[];   // [KG][nz]
-  const int k0 = blockIdx.y * KG, kg = min (KG, nz - k0);
-  for (int i = threadIdx.x; i < kg * nz; i += blockDim.x) Ms[i] = M[(size_t) k0 * nz + i];
-  __syncthreads ();
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= S) return;
-  double acc[KG];
-#pragma unroll
-  for (int k = 0; k < KG; k++) acc[k] = 0.0;
-  for (int n = 0; n < nz; n++) {
-    const double x = in[(size_t) n * S + s];
-#pragma unroll
-    for (int k = 0; k < KG; k++)
-      if (k < kg) acc[k] += Ms[k * nz + n] * x;
-  }
-#pragma unroll
-  for (int k = 0; k < KG; k++)
-    if (k < kg) out[(size_t) (k0 + k) * S + s] = acc[k];
-}
-
 // x or y derivative of every element of every plane (Element::grad,
 // src/element.cpp:521-565): d/dx = (u DT) drdx + (D u) dsdx.
 __global__ __launch_bounds__(64) void k_grad (int np, const double* __restrict__ geo,
@@ -355,11 +325,11 @@ struct sx_dns {
   double beta = 0, nu = 0, dt = 0, cf[3] = {0, 0, 0};
   size_t n = 0;                        // nz * S
   sx_msys *Mv[2] = {nullptr, nullptr}, *Mp = nullptr;
+  bool group_solve = false;            // velocity solves share one band launch
   std::vector<double*> bufs;           // every device allocation
   double *u[3] = {}, *p = nullptr;
   double *Us[2][3] = {}, *Uf[2][3] = {};
   double *phys[3] = {}, *tmp = nullptr, *tmp2 = nullptr, *H[3] = {}, *F[3] = {};
-  double *Cf = nullptr, *Ci = nullptr; // forward / inverse packed DFT matrices
   double *imass = nullptr, *ds = nullptr;
   int nb = 0, nline = 0, hop_head = 0, un_head = 0;
   int *seg_elmt = nullptr, *seg_idx = nullptr, *seg_kind = nullptr;
@@ -377,23 +347,26 @@ struct sx_dns {
 
 namespace {
 
-int dft (sx_dns* d, const double* M, const double* in, double* out, hipStream_t st)
-{
-  dim3 grid ((d->S + 255) / 256, (d->nz + KG - 1) / KG);
-  hipLaunchKernelGGL (k_dft, grid, dim3 (256), sizeof (double) * KG * d->nz, st, M, in,
-                      out, d->nz, d->S);
-  SX_HIP (hipGetLastError ());
-  return SX_OK;
-}
-
-// in-place transform through the scratch buffer tmp2
+// z transform of a whole field, in place (tmp2 is the scratch when the
+// transform cannot run in place)
 int transform (sx_dns* d, double* f, int sign, hipStream_t st)
 {
   if (d->nz < 2) return SX_OK;
-  int rc = dft (d, sign > 0 ? d->Cf : d->Ci, f, d->tmp2, st);
+  if (fourier_in_place (d->nz)) return fourier_z (d->nz, d->S, f, f, sign, st);
+  int rc = fourier_z (d->nz, d->S, f, d->tmp2, sign, st);
   if (rc) return rc;
   SX_HIP (hipMemcpyAsync (f, d->tmp2, sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
   return SX_OK;
+}
+
+// out-of-place z transform (out may not alias in)
+int transform_to (sx_dns* d, const double* in, double* out, int sign, hipStream_t st)
+{
+  if (d->nz < 2) {
+    SX_HIP (hipMemcpyAsync (out, in, sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+    return SX_OK;
+  }
+  return fourier_z (d->nz, d->S, in, out, sign, st);
 }
 
 int grad (sx_dns* d, const double* in, double* out, int dir, hipStream_t st)
@@ -426,18 +399,16 @@ int smooth (sx_dns* d, double* f, hipStream_t st)
 #define EW(kern, ...) \
   do { hipLaunchKernelGGL (kern, dim3 (nb256 (d->n)), dim3 (256), 0, st, __VA_ARGS__); } while (0)
 
-// skewSymmetric / altSkewSymmetric (dns/nonlinear.cpp): U (Fourier copy of
-// u) -> Us0, N -> Uf0.  d->u are transformed to physical space in place
-// (the reference transforms D->u in place too).
+// skewSymmetric / altSkewSymmetric (dns/nonlinear.cpp): U holds the
+// Fourier velocity (the reference's Us[0] copy of D->u), N -> Uf0; the
+// physical-space velocity goes to phys.
 int advection (sx_dns* d, double** U, double** N, hipStream_t st)
 {
   const int nc = d->ncom;
   int rc;
   for (int i = 0; i < nc; i++) {
-    SX_HIP (hipMemcpyAsync (U[i], d->u[i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
     SX_HIP (hipMemsetAsync (N[i], 0, sizeof (double) * d->n, st));
-    SX_HIP (hipMemcpyAsync (d->phys[i], d->u[i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
-    if ((rc = transform (d, d->phys[i], -1, st))) return rc;
+    if ((rc = transform_to (d, U[i], d->phys[i], -1, st))) return rc;
   }
   const bool conv = d->advection == SX_SKEW || d->toggle == 1;
   const bool cons = d->advection == SX_SKEW || d->toggle == 0;
@@ -455,8 +426,7 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
         if (j == 2) {
           if ((rc = transform (d, d->tmp, +1, st))) return rc;
           if ((rc = grad (d, d->tmp, d->tmp2, 2, st))) return rc;
-          SX_HIP (hipMemcpyAsync (d->tmp, d->tmp2, sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
-          if ((rc = transform (d, d->tmp, -1, st))) return rc;
+          if ((rc = transform_to (d, d->tmp2, d->tmp, -1, st))) return rc;
           EW (k_ew_sub, N[i], d->tmp, d->n);
         } else {
           if ((rc = grad (d, d->tmp, d->tmp2, j, st))) return rc;
@@ -483,6 +453,10 @@ int dns_step (sx_dns* d, hipStream_t st)
 {
   const int nc = d->ncom, nord = d->nord;
   int rc;
+  // Us[0] <- u: the level about to be overwritten holds nothing needed, so
+  // the buffers are exchanged; u is rewritten by the viscous solves (its
+  // Nyquist plane, which they skip, is copied back from U below)
+  for (int i = 0; i < nc; i++) std::swap (d->u[i], d->Us[0][i]);
   double** U = d->Us[0];
   double** N = d->Uf[0];
   if ((rc = advection (d, U, N, st))) return rc;
@@ -524,9 +498,12 @@ int dns_step (sx_dns* d, hipStream_t st)
   for (int i = 0; i < nc; i++)
     EW (k_ew_wave, d->H[i], d->Us[0][i], d->Us[1][i], d->Uf[0][i], d->Uf[1][i],
         SSc[Je][1], SSc[Je][2], EXc[Je][0] * d->dt, EXc[Je][1] * d->dt, Je, d->n);
-  // D->u after swapData holds u^n = Us[0]
-  for (int i = 0; i < nc; i++)
-    SX_HIP (hipMemcpyAsync (d->u[i], d->Us[0][i], sizeof (double) * d->n, hipMemcpyDeviceToDevice, st));
+  // D->u after swapData holds u^n = Us[0]: only the Nyquist plane, which
+  // the solves leave alone, has to be carried over
+  if (d->nz > 1)
+    for (int i = 0; i < nc; i++)
+      SX_HIP (hipMemcpyAsync (d->u[i] + d->S, d->Us[0][i] + d->S, sizeof (double) * d->S,
+                              hipMemcpyDeviceToDevice, st));
   // rollm (Uf)
   for (int i = 0; i < nc; i++) std::swap (d->Uf[0][i], d->Uf[1][i]);
   // -- setPForce: div u^ / dt into F[0]
@@ -546,6 +523,7 @@ int dns_step (sx_dns* d, hipStream_t st)
   for (int i = 0; i < nc; i++) std::swap (d->Us[0][i], d->Us[1][i]);
   // -- viscous substep
   sx_msys* Mv = d->Mv[Je - 1];
+  if (d->group_solve) return field_solve_group (Mv, nc, d->nz, d->S, d->F, d->u, nullptr, st);
   for (int i = 0; i < nc; i++)
     if ((rc = sx_field_solve (Mv, d->nz, d->S, d->F[i], d->u[i], nullptr, st))) return rc;
   return SX_OK;
@@ -604,6 +582,15 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
     return fail (rc);
   // fields
   const int nc = d->ncom;
+  // the velocity components share their systems: one band launch solves
+  // all of them (falls back to one solve per component when the band is
+  // too wide for the panel sweeps)
+  d->group_solve = true;
+  for (int q = 0; q < 2; q++) {
+    rc = msys_reserve_rhs (d->Mv[q], 2 * nc);
+    if (rc == SX_ERR_LIMIT) d->group_solve = false;
+    else if (rc) return fail (rc);
+  }
   for (int i = 0; i < nc; i++) {
     if (!(d->u[i] = d->alloc (d->n)) || !(d->phys[i] = d->alloc (d->n)) ||
         !(d->H[i] = d->alloc (d->n)) || !(d->F[i] = d->alloc (d->n)))
@@ -617,30 +604,7 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
   for (double* q : d->bufs)
     if (hipMemsetAsync (q, 0, sizeof (double) * d->n, st) != hipSuccess)
       return fail (hip_fail (hipErrorUnknown, "sx_dns_create memset"));
-  // packed real DFT matrices (femlib/fourier.c:40-165 conventions)
-  if (nz > 1) {
-    std::vector<double> Cf ((size_t) nz * nz), Ci ((size_t) nz * nz);
-    for (int n = 0; n < nz; n++) {
-      Cf[0 * nz + n] = 1.0 / nz;
-      Cf[1 * nz + n] = ((n & 1) ? -1.0 : 1.0) / nz;
-      Ci[(size_t) n * nz + 0] = 1.0;
-      Ci[(size_t) n * nz + 1] = (n & 1) ? -1.0 : 1.0;
-      for (int k = 1; k < nz / 2; k++) {
-        const long double a = 2.0L * 3.14159265358979323846264338327950288L *
-                              (long double) ((long) k * n % nz) / nz;
-        const double c = (double) cosl (a), s = (double) sinl (a);
-        Cf[(size_t) (2 * k) * nz + n] = c / nz;
-        Cf[(size_t) (2 * k + 1) * nz + n] = -s / nz;
-        Ci[(size_t) n * nz + 2 * k] = 2.0 * c;
-        Ci[(size_t) n * nz + 2 * k + 1] = -2.0 * s;
-      }
-    }
-    if (!(d->Cf = d->alloc (Cf.size ())) || !(d->Ci = d->alloc (Ci.size ())))
-      return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
-    if (hipMemcpy (d->Cf, Cf.data (), sizeof (double) * Cf.size (), hipMemcpyHostToDevice) ||
-        hipMemcpy (d->Ci, Ci.data (), sizeof (double) * Ci.size (), hipMemcpyHostToDevice))
-      return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
-  }
+  if (int rc = fourier_prepare (nz)) return fail (rc);
   // smoothing: inverse global boundary mass (BCmgr::buildnum imass)
   {
     std::vector<double> mass (mu->nglobal, 0.0), q4 ((size_t) mu->nel * npnp);

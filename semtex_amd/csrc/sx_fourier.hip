@@ -1,0 +1,274 @@
+// sx_fourier.hip -- real transforms along z of every (element, node) line
+// of a field of nz planes, in the reference's packing and scaling
+// (AuxField::transform, src/auxfield.cpp:1157-1194 -> dDFTr,
+// femlib/fourier.c:40-165):
+//
+//   forward (sign +1):  plane 0 = Re X_0, plane 1 = Re X_{nz/2},
+//                       planes 2k, 2k+1 = Re X_k, Im X_k   (k = 1..nz/2-1),
+//                       X_k = (1/nz) sum_n x_n exp(-2 pi i k n / nz);
+//   inverse (sign -1):  x_n = X_0 + (-1)^n X_{nz/2}
+//                             + 2 sum_k Re(X_k exp(+2 pi i k n / nz)).
+//
+// For nz a power of two (8..1024) one workgroup transforms a tile of
+// columns in LDS: columns 2q and 2q+1 form the complex sequence
+// x_2q + i x_2q+1, which a radix-2 Stockham FFT transforms in natural order;
+// the two real spectra are separated (forward) or combined (inverse) from
+// Z_k and conj(Z_{nz-k}).  Each tile is read once and written once, so the
+// transform is HBM-bound (16 bytes per point per pass) and may run in place.
+// Other nz use the packed DFT matrix, one thread per column.
+#include <cmath>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "sx_internal.h"
+
+namespace {
+
+constexpr int FT_THREADS = 256;
+
+template <int N>
+struct FftTile {
+  static constexpr int TS = (4096 / N) < 4 ? 4 : 4096 / N;   // columns per tile
+  static constexpr int NQ = TS / 2;                          // complex sequences
+  static size_t lds () { return sizeof (double2) * (2 * (size_t) N * NQ + N / 2); }
+};
+
+__device__ __forceinline__ double2 cmul (double2 a, double2 b)
+{
+  return make_double2 (a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int N>
+__global__ __launch_bounds__(FT_THREADS) void k_fft_z (const double* in, double* out, int S,
+                                                       int sign, const double2* __restrict__ tw)
+{
+  constexpr int TS = FftTile<N>::TS, NQ = FftTile<N>::NQ;
+  extern __shared__ double2 lds[];
+  double2* A = lds;                   // [N][NQ]
+  double2* B = A + N * NQ;            // [N][NQ], also the raw tile [N][TS]
+  double2* W = B + N * NQ;            // exp(-2 pi i j / N), j < N/2
+  double* raw = (double*) B;
+  const int s0 = blockIdx.x * TS, tid = threadIdx.x;
+  for (int i = tid; i < N / 2; i += FT_THREADS) W[i] = tw[i];
+  for (int i = tid; i < N * TS; i += FT_THREADS) {
+    const int n = i / TS, c = i % TS, s = s0 + c;
+    raw[i] = s < S ? in[(size_t) n * S + s] : 0.0;
+  }
+  __syncthreads ();
+  if (sign > 0) {
+    // Z_q[n] = x_2q[n] + i x_2q+1[n]
+    for (int i = tid; i < N * NQ; i += FT_THREADS) {
+      const int n = i / NQ, q = i % NQ;
+      A[i] = make_double2 (raw[n * TS + 2 * q], raw[n * TS + 2 * q + 1]);
+    }
+  } else {
+    // Z_q[k] = Xa[k] + i Xb[k], spectra extended by X[N-k] = conj X[k]
+    for (int i = tid; i < N * NQ; i += FT_THREADS) {
+      const int k = i / NQ, q = i % NQ;
+      const int kk = k <= N / 2 ? k : N - k;
+      double ar, ai, br, bi;
+      if (kk == 0) {
+        ar = raw[2 * q]; ai = 0.0; br = raw[2 * q + 1]; bi = 0.0;
+      } else if (kk == N / 2) {
+        ar = raw[TS + 2 * q]; ai = 0.0; br = raw[TS + 2 * q + 1]; bi = 0.0;
+      } else {
+        ar = raw[2 * kk * TS + 2 * q];       ai = raw[(2 * kk + 1) * TS + 2 * q];
+        br = raw[2 * kk * TS + 2 * q + 1];   bi = raw[(2 * kk + 1) * TS + 2 * q + 1];
+      }
+      if (k > N / 2) { ai = -ai; bi = -bi; }
+      A[i] = make_double2 (ar - bi, ai + br);
+    }
+  }
+  __syncthreads ();
+  // radix-2 Stockham: y[2j-k] = a + w b, y[2j-k+m] = a - w b, w = W^(k N/2m)
+  double2 *x = A, *y = B;
+#pragma unroll 1
+  for (int m = 1; m < N; m <<= 1) {
+    const int ws = N / (2 * m);
+    for (int b = tid; b < (N / 2) * NQ; b += FT_THREADS) {
+      const int j = b / NQ, q = b % NQ, k = j & (m - 1);
+      double2 w = W[k * ws];
+      if (sign < 0) w.y = -w.y;
+      const double2 a = x[j * NQ + q];
+      const double2 t = cmul (x[(j + N / 2) * NQ + q], w);
+      y[(2 * j - k) * NQ + q] = make_double2 (a.x + t.x, a.y + t.y);
+      y[(2 * j - k + m) * NQ + q] = make_double2 (a.x - t.x, a.y - t.y);
+    }
+    __syncthreads ();
+    double2* sw = x; x = y; y = sw;
+  }
+  if (sign > 0) {
+    const double scale = 1.0 / N;
+    for (int i = tid; i < N * TS; i += FT_THREADS) {
+      const int p = i / TS, c = i % TS, s = s0 + c;
+      if (s >= S) continue;
+      const int q = c >> 1, k = p == 0 ? 0 : (p == 1 ? N / 2 : p >> 1);
+      const double2 zk = x[k * NQ + q], zr = x[((N - k) & (N - 1)) * NQ + q];
+      // Xa = (Z_k + conj Z_{N-k}) / 2,  Xb = (Z_k - conj Z_{N-k}) / 2i
+      double re, im;
+      if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
+      else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
+      out[(size_t) p * S + s] = ((p < 2 || (p & 1) == 0) ? re : im) * scale;
+    }
+  } else {
+    for (int i = tid; i < N * TS; i += FT_THREADS) {
+      const int n = i / TS, c = i % TS, s = s0 + c;
+      if (s >= S) continue;
+      const double2 z = x[n * NQ + (c >> 1)];
+      out[(size_t) n * S + s] = (c & 1) ? z.y : z.x;
+    }
+  }
+}
+
+// Packed DFT matrix product out[k][s] = sum_n M[k][n] in[n][s] (any nz);
+// one thread per column, KG output planes per pass (M rows in LDS).
+constexpr int KG = 16;
+__global__ __launch_bounds__(256) void k_dft_z (const double* __restrict__ M,
+                                                const double* __restrict__ in, double* out,
+                                                int nz, int S)
+{
+  extern __shared__ double Ms[];   // [KG][nz]
+  const int k0 = blockIdx.y * KG, kg = min (KG, nz - k0);
+  for (int i = threadIdx.x; i < kg * nz; i += blockDim.x) Ms[i] = M[(size_t) k0 * nz + i];
+  __syncthreads ();
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  double acc[KG];
+#pragma unroll
+  for (int k = 0; k < KG; k++) acc[k] = 0.0;
+  for (int n = 0; n < nz; n++) {
+    const double xv = in[(size_t) n * S + s];
+#pragma unroll
+    for (int k = 0; k < KG; k++)
+      if (k < kg) acc[k] += Ms[k * nz + n] * xv;
+  }
+#pragma unroll
+  for (int k = 0; k < KG; k++)
+    if (k < kg) out[(size_t) (k0 + k) * S + s] = acc[k];
+}
+
+// Per (device, nz) tables: FFT twiddles, or forward/inverse DFT matrices.
+struct Tables {
+  double2* tw = nullptr;
+  double *Cf = nullptr, *Ci = nullptr;
+};
+std::mutex g_mu;
+std::map<std::pair<int, int>, Tables> g_tables;
+
+bool pow2_fft (int nz) { return sx::fourier_in_place (nz); }
+
+int tables (int nz, Tables** out)
+{
+  int dev = 0;
+  SX_HIP (hipGetDevice (&dev));
+  std::lock_guard<std::mutex> lk (g_mu);
+  Tables& t = g_tables[{dev, nz}];
+  const long double twopi = 2.0L * 3.14159265358979323846264338327950288L;
+  if (pow2_fft (nz) && !t.tw) {
+    std::vector<double2> h (nz / 2);
+    for (int j = 0; j < nz / 2; j++) {
+      const long double a = twopi * j / nz;
+      h[j] = make_double2 ((double) cosl (a), (double) -sinl (a));
+    }
+    SX_HIP (hipMalloc (&t.tw, sizeof (double2) * h.size ()));
+    SX_HIP (hipMemcpy (t.tw, h.data (), sizeof (double2) * h.size (), hipMemcpyHostToDevice));
+  }
+  if (!pow2_fft (nz) && !t.Cf) {
+    std::vector<double> Cf ((size_t) nz * nz, 0.0), Ci ((size_t) nz * nz, 0.0);
+    for (int n = 0; n < nz; n++) {
+      Cf[n] = 1.0 / nz;
+      Ci[(size_t) n * nz] = 1.0;
+      if (nz > 1) {
+        Cf[(size_t) nz + n] = ((n & 1) ? -1.0 : 1.0) / nz;
+        Ci[(size_t) n * nz + 1] = (n & 1) ? -1.0 : 1.0;
+      }
+      for (int k = 1; k < nz / 2; k++) {
+        const long double a = twopi * (long double) ((long) k * n % nz) / nz;
+        const double c = (double) cosl (a), s = (double) sinl (a);
+        Cf[(size_t) (2 * k) * nz + n] = c / nz;
+        Cf[(size_t) (2 * k + 1) * nz + n] = -s / nz;
+        Ci[(size_t) n * nz + 2 * k] = 2.0 * c;
+        Ci[(size_t) n * nz + 2 * k + 1] = -2.0 * s;
+      }
+    }
+    SX_HIP (hipMalloc (&t.Cf, sizeof (double) * Cf.size ()));
+    SX_HIP (hipMalloc (&t.Ci, sizeof (double) * Ci.size ()));
+    SX_HIP (hipMemcpy (t.Cf, Cf.data (), sizeof (double) * Cf.size (), hipMemcpyHostToDevice));
+    SX_HIP (hipMemcpy (t.Ci, Ci.data (), sizeof (double) * Ci.size (), hipMemcpyHostToDevice));
+  }
+  *out = &t;
+  return SX_OK;
+}
+
+template <int N>
+int launch_fft (int S, const double* in, double* out, int sign, const double2* tw, hipStream_t st)
+{
+  const size_t lds = FftTile<N>::lds ();
+  auto kern = k_fft_z<N>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  const int TS = FftTile<N>::TS;
+  hipLaunchKernelGGL (kern, dim3 ((S + TS - 1) / TS), dim3 (FT_THREADS), lds, st, in, out, S,
+                      sign, tw);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+}  // namespace
+
+namespace sx {
+
+int fourier_prepare (int nz)
+{
+  Tables* t = nullptr;
+  return nz > 1 ? tables (nz, &t) : SX_OK;
+}
+
+int fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st)
+{
+  if (nz < 2 || S < 1) {
+    if (in != out && nz >= 1 && S >= 1)
+      SX_HIP (hipMemcpyAsync (out, in, sizeof (double) * nz * S, hipMemcpyDeviceToDevice, st));
+    return SX_OK;
+  }
+  Tables* t = nullptr;
+  int rc = tables (nz, &t);
+  if (rc) return rc;
+  if (pow2_fft (nz)) {
+    switch (nz) {
+    case 8: return launch_fft<8> (S, in, out, sign, t->tw, st);
+    case 16: return launch_fft<16> (S, in, out, sign, t->tw, st);
+    case 32: return launch_fft<32> (S, in, out, sign, t->tw, st);
+    case 64: return launch_fft<64> (S, in, out, sign, t->tw, st);
+    case 128: return launch_fft<128> (S, in, out, sign, t->tw, st);
+    case 256: return launch_fft<256> (S, in, out, sign, t->tw, st);
+    case 512: return launch_fft<512> (S, in, out, sign, t->tw, st);
+    case 1024: return launch_fft<1024> (S, in, out, sign, t->tw, st);
+    default: break;
+    }
+  }
+  if (in == out) {
+    set_error ("fourier transform: the matrix form (nz not a power of two in 8..1024) "
+               "needs distinct input and output");
+    return SX_ERR_ARG;
+  }
+  dim3 grid ((S + 255) / 256, (nz + KG - 1) / KG);
+  hipLaunchKernelGGL (k_dft_z, grid, dim3 (256), sizeof (double) * KG * nz, st,
+                      sign > 0 ? t->Cf : t->Ci, in, out, nz, S);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+}  // namespace sx
+
+extern "C" int sx_fourier_transform (int nz, int stride, const double* d_in, double* d_out,
+                                     int sign, void* stream)
+{
+  if (nz < 1 || stride < 1 || !d_in || !d_out || (sign != 1 && sign != -1) ||
+      (nz > 1 && (nz & 1))) {
+    sx::set_error ("sx_fourier_transform: bad arguments (nz must be 1 or even, sign +-1)");
+    return SX_ERR_ARG;
+  }
+  return sx::fourier_z (nz, stride, d_in, d_out, sign, (hipStream_t) stream);
+}

@@ -47,7 +47,20 @@ namespace sx {
 int    band_block (int nband);
 size_t band_sweep_bytes_per_mode (int nmax, int nband);
 int    band_factor (sx_msys* s, hipStream_t st, int* d_status);
-int    band_solve (const sx_msys* s, hipStream_t st);
+int    band_solve (const sx_msys* s, int nr, hipStream_t st);   // nr RHS per mode
+// Grow the per-mode right-hand-side workspace to nr (2, 4 or 6) columns.
+int    msys_reserve_rhs (sx_msys* s, int nr);
+// Field::solve for nf fields sharing one system (the DNS velocity
+// components): one band launch for all 2*nf right-hand sides per mode.
+int    field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_f,
+                          double* const* d_u, const double* const* d_lines, hipStream_t st);
+// Real z transforms of nz planes of stride S in the reference's packing
+// (sx_fourier.hip); in place when fourier_in_place (nz).
+int    fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st);
+int    fourier_prepare (int nz);   // builds the per-device tables (synchronous)
+// JACPCG set-up: the Jacobi preconditioner of every mode (sx_pcg.hip).
+int    pcg_setup (sx_msys* s, hipStream_t st);
+inline bool fourier_in_place (int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }
 }  // namespace sx
 
 struct sx_mesh {
@@ -77,7 +90,14 @@ struct sx_mesh {
   int* d_pmn = nullptr;        // [npairs][2] (m, n) global rows
   int* d_pptr = nullptr;       // [npairs+1] CSR: entry -> condensed-matrix slots
   int* d_psrc = nullptr;       //   slot = cls*next*next + i*next + k
+  // element tiles for the batched condensation / recovery: elements sorted
+  // by class, tiles of at most ETILE elements of one class
+  int ntile = 0, ness = 0;
+  int* d_tile = nullptr;       // [ntile][3] class, first index into d_elist, count
+  int* d_elist = nullptr;      // [nel] element numbers, class-sorted
+  int* d_ess_elem = nullptr;   // [ness] elements with essential nodes (emask)
 };
+constexpr int ETILE = 16;
 
 struct sx_msys {
   sx_mesh* m = nullptr;
@@ -101,8 +121,10 @@ struct sx_msys {
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks
-  double* d_rhs = nullptr;     // [nmodes][2][nmax] solve workspace
-  double* d_G = nullptr;       // [2*nmodes][nglobal] essential values
-  double* d_work = nullptr;    // mixed-precision workspace
+  double* d_rhs = nullptr;     // [nmodes][nr][nmax] solve workspace (nr <= rhs_cap)
+  double* d_G = nullptr;       // [fields][2*nmodes][nglobal] essential values
+  int rhs_cap = 2;             // right-hand sides per mode d_rhs / d_G can hold
+  double* d_work = nullptr;    // mixed-precision / PCG workspace
+  double* d_pc = nullptr;      // SX_JACPCG: [nmodes][npts] inverse assembled diagonal
   size_t work_doubles = 0;
 };

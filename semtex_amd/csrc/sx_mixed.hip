@@ -277,7 +277,7 @@ int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
                       stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
                       s->base_mode);
   SX_HIP (hipGetLastError ());
-  int rc = band_solve (s, st);
+  int rc = band_solve (s, 2, st);
   if (rc) return rc;
   hipLaunchKernelGGL (k_recover32, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
                       m->d_emap, m->d_cls, m->d_btog, s->d_Bm32, s->d_Hinv32, Fp, V, stride,

@@ -219,7 +219,7 @@ extern "C" int sx_mesh_destroy (sx_mesh_t m)
   void* ptrs[] = {m->d_geo, m->d_cgeo, m->d_D, m->d_cls, m->d_btog, m->d_bmask,
                   m->d_emask, m->d_emap, m->d_gptr, m->d_gsrc, m->d_ess,
                   m->d_nptr, m->d_nidx, m->d_nw, m->d_ppos, m->d_pmn,
-                  m->d_pptr, m->d_psrc};
+                  m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete m;
   return SX_OK;
@@ -391,6 +391,24 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   }
   m->npairs = (int) ppos.size ();
 
+  // class-sorted element tiles and the elements with essential nodes
+  std::vector<int> elist, tile, ess_elem;
+  for (int c = 0; c < m->nclass; c++) {
+    const int first = (int) elist.size ();
+    for (int e = 0; e < nel; e++)
+      if (m->cls[e] == c) elist.push_back (e);
+    for (int t = first; t < (int) elist.size (); t += ETILE) {
+      tile.push_back (c);
+      tile.push_back (t);
+      tile.push_back (std::min (ETILE, (int) elist.size () - t));
+    }
+  }
+  for (int e = 0; e < nel; e++)
+    if (m->emask[e]) ess_elem.push_back (e);
+  m->ntile = (int) tile.size () / 3;
+  m->ness = (int) ess_elem.size ();
+  if (ess_elem.empty ()) ess_elem.push_back (0);
+
   int rc = SX_OK;
   std::vector<int> cls (m->cls);
   if ((rc = upload (&m->d_geo, geo)) || (rc = upload (&m->d_cgeo, cgeo)) ||
@@ -401,7 +419,9 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       (rc = upload (&m->d_ess, ess)) || (rc = upload (&m->d_nptr, nptr)) ||
       (rc = upload (&m->d_nidx, nidx)) || (rc = upload (&m->d_nw, nw)) ||
       (rc = upload (&m->d_ppos, ppos)) || (rc = upload (&m->d_pmn, pmn)) ||
-      (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc))) {
+      (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc)) ||
+      (rc = upload (&m->d_tile, tile)) || (rc = upload (&m->d_elist, elist)) ||
+      (rc = upload (&m->d_ess_elem, ess_elem))) {
     sx_mesh_destroy (m);
     return rc;
   }

@@ -297,6 +297,181 @@ __global__ __launch_bounds__(64) void k_condense (
   for (int a = next + lane; a < npnp; a += 64) f[a] = g[a];
 }
 
+// ---------------------------------------------------------------------------
+// Batched condensation / recovery: one workgroup per (tile of same-class
+// elements, Fourier mode).  The mode's condensed matrices of that class
+// are staged in LDS once and applied to all element-planes of the tile
+// (both planes of the mode); each thread accumulates four element-planes
+// per matrix entry it reads.
+// ---------------------------------------------------------------------------
+constexpr int CT_THREADS = 256, CT_NB = 4;
+
+// the solved planes of mode m: k0 and their count (1 or 2)
+__device__ __forceinline__ int mode_planes (int m, int nz, int base_mode, int* k0)
+{
+  if (nz == 1 || (base_mode == 0 && m == 0)) { *k0 = 0; return 1; }
+  *k0 = 2 * m;
+  return 2;
+}
+
+// Constrain on the elements with essential nodes (Field::constrain,
+// src/field.cpp:812-852): f <- -(M f + H g), row-major, in place.
+__global__ __launch_bounds__(64) void k_weak_ess (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ ess_elem, const double* __restrict__ G, int nglobal,
+    double* F, int stride, int nz, int base_mode)
+{
+  const int npnp = np * np, next = 4 * (np - 1);
+  const int e = ess_elem[blockIdx.x], k = solved_plane (blockIdx.y, nz, base_mode);
+  const int lane = threadIdx.x;
+  __shared__ double D[256], g[256], R[256], S[256], Hg[256];
+  __shared__ int emap[256];
+  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; emap[i] = emap_g[i]; g[i] = 0.0; }
+  __syncthreads ();
+  const int* bt = btog + (size_t) e * next;
+  for (int i = lane; i < next; i += 64) g[emap[i]] = G[(size_t) k * nglobal + bt[i]];
+  __syncthreads ();
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  helm_op_wave (np, D, ge, lambda2, betak2[k >> 1], cyl, g, R, S, Hg, lane);
+  double* f = F + (size_t) k * stride + (size_t) e * npnp;
+  for (int i = lane; i < npnp; i += 64) f[i] = -(f[i] * ge[G_Q8 * npnp + i] + Hg[i]);
+}
+
+// f (row-major; already weak on essential elements, else -M f is formed
+// here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i.
+__global__ __launch_bounds__(CT_THREADS) void k_condense_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ emask,
+    const double* __restrict__ geo, const double* __restrict__ Bm, double* F, int stride,
+    int nz, int base_mode)
+{
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Bs = sm;                                   // [next][nint]
+  double* wp = Bs + next * nint;                     // [nep][npnp], emap order
+  int* emap = (int*) (wp + (size_t) ETILE * 2 * npnp);
+  const int tid = threadIdx.x;
+  const double* B = Bm + ((size_t) mode * nclass + c) * next * nint;
+  for (int i = tid; i < next * nint; i += CT_THREADS) Bs[i] = B[i];
+  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  __syncthreads ();
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    const int r = emap[a];
+    const double v = F[(size_t) k * stride + (size_t) e * npnp + r];
+    wp[i] = emask[e] ? v : -v * geo[((size_t) e * NGEO + G_Q8) * npnp + r];
+  }
+  __syncthreads ();
+  const int ng = (nep + CT_NB - 1) / CT_NB;
+  for (int task = tid; task < next * ng; task += CT_THREADS) {
+    const int a = task % next, g0 = (task / next) * CT_NB;
+    double acc[CT_NB];
+#pragma unroll
+    for (int j = 0; j < CT_NB; j++) acc[j] = 0.0;
+    for (int q = 0; q < nint; q++) {
+      const double b = Bs[a * nint + q];
+#pragma unroll
+      for (int j = 0; j < CT_NB; j++)
+        if (g0 + j < nep) acc[j] += b * wp[(g0 + j) * npnp + next + q];
+    }
+#pragma unroll
+    for (int j = 0; j < CT_NB; j++) {
+      const int ep = g0 + j;
+      if (ep < nep) {
+        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+        F[(size_t) k * stride + (size_t) e * npnp + a] = wp[ep * npnp + a] - acc[j];
+      }
+    }
+  }
+  for (int i = tid; i < nep * nint; i += CT_THREADS) {
+    const int ep = i / nint, q = i % nint, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    F[(size_t) k * stride + (size_t) e * npnp + next + q] = wp[ep * npnp + next + q];
+  }
+}
+
+// u_i = Hinv^T F_i - Bm^T u_b with u_b from the band solution, then the
+// essential values (Element::global2localSC src/element.cpp:368-407,
+// Field::setEssential src/field.cpp:1105-1124).
+__global__ __launch_bounds__(CT_THREADS) void k_recover_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ bmask, const int* __restrict__ emask,
+    const double* __restrict__ Bm, const double* __restrict__ Hinv,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
+    int nz, int base_mode, int rnr, int roff)
+{
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Hs = sm;                                   // [nint][nint]
+  double* Bs = Hs + nint * nint;                     // [next][nint]
+  double* xs = Bs + next * nint;                     // [nep][npnp]: u_b | F_i
+  int* emap = (int*) (xs + (size_t) ETILE * 2 * npnp);
+  const int tid = threadIdx.x, ns = nsolve[mode];
+  const size_t mc = (size_t) mode * nclass + c;
+  for (int i = tid; i < nint * nint; i += CT_THREADS) Hs[i] = Hinv[mc * nint * nint + i];
+  for (int i = tid; i < next * nint; i += CT_THREADS) Bs[i] = Bm[mc * next * nint + i];
+  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    double v;
+    if (a < next) {
+      const int gg = btog[(size_t) e * next + a];
+      v = gg < ns ? rhs[((size_t) mode * rnr + roff + (k & 1)) * nmax + gg] : 0.0;
+    } else {
+      v = F[(size_t) k * stride + (size_t) e * npnp + a];
+    }
+    xs[i] = v;
+  }
+  __syncthreads ();
+  const int ng = (nep + CT_NB - 1) / CT_NB;
+  for (int task = tid; task < nint * ng; task += CT_THREADS) {
+    const int a = task % nint, g0 = (task / nint) * CT_NB;
+    double acc[CT_NB];
+#pragma unroll
+    for (int j = 0; j < CT_NB; j++) acc[j] = 0.0;
+    for (int b = 0; b < nint; b++) {
+      const double h = Hs[b * nint + a];
+#pragma unroll
+      for (int j = 0; j < CT_NB; j++)
+        if (g0 + j < nep) acc[j] += h * xs[(g0 + j) * npnp + next + b];
+    }
+    for (int q = 0; q < next; q++) {
+      const double b = Bs[q * nint + a];
+#pragma unroll
+      for (int j = 0; j < CT_NB; j++)
+        if (g0 + j < nep) acc[j] -= b * xs[(g0 + j) * npnp + q];
+    }
+#pragma unroll
+    for (int j = 0; j < CT_NB; j++) {
+      const int ep = g0 + j;
+      if (ep < nep) {
+        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+        U[(size_t) k * stride + (size_t) e * npnp + emap[next + a]] = acc[j];
+      }
+    }
+  }
+  for (int i = tid; i < nep * next; i += CT_THREADS) {
+    const int ep = i / next, a = i % next, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    double v = xs[ep * npnp + a];
+    if (emask[e] && bmask[(size_t) e * next + a])
+      v = G[(size_t) k * nglobal + btog[(size_t) e * next + a]];
+    U[(size_t) k * stride + (size_t) e * npnp + emap[a]] = v;
+  }
+}
+
 __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
                             const int* __restrict__ gsrc,
                             const int* __restrict__ nptr,
@@ -305,11 +480,11 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
                             const double* __restrict__ F, int stride,
                             const double* __restrict__ lines, int lstride,
                             const int* __restrict__ nsolve, double* rhs,
-                            int nmax, int nz, int base_mode)
+                            int nmax, int nz, int base_mode, int rnr, int roff)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = solved_plane (blockIdx.y, nz, base_mode);
-  const int mode = k >> 1, rr = k & 1;
+  const int mode = k >> 1, rr = roff + (k & 1);
   if (g >= nsolve[mode]) return;
   const double* Fk = F + (size_t) k * stride;
   double s = 0.0;
@@ -317,7 +492,7 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
   if (lines)
     for (int c = nptr[g]; c < nptr[g + 1]; c++)
       s += lines[(size_t) k * lstride + nidx[c]] * nw[c];
-  rhs[((size_t) mode * 2 + rr) * nmax + g] = s;
+  rhs[((size_t) mode * rnr + rr) * nmax + g] = s;
 }
 
 // One wavefront per (element, solved plane): interior recovery and
@@ -330,17 +505,17 @@ __global__ __launch_bounds__(64) void k_recover (
     const double* __restrict__ Bm, const double* __restrict__ Hinv,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F,
     double* U, int stride, const int* __restrict__ nsolve,
-    const double* __restrict__ rhs, int nmax, int nz, int base_mode)
+    const double* __restrict__ rhs, int nmax, int nz, int base_mode, int rnr, int roff)
 {
   const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
   const int e = blockIdx.x, k = solved_plane (blockIdx.y, nz, base_mode);
-  const int mode = k >> 1, rr = k & 1, lane = threadIdx.x;
+  const int mode = k >> 1, rr = roff + (k & 1), lane = threadIdx.x;
   __shared__ double ub[64], fi[256];
   __shared__ int emap[256];
   for (int i = lane; i < npnp; i += 64) emap[i] = emap_g[i];
   const int* bt = btog + (size_t) e * next;
   const int ns = nsolve[mode];
-  const double* x = rhs + ((size_t) mode * 2 + rr) * nmax;
+  const double* x = rhs + ((size_t) mode * rnr + rr) * nmax;
   const double* f = F + (size_t) k * stride + (size_t) e * npnp;
   double* u = U + (size_t) k * stride + (size_t) e * npnp;
   for (int i = lane; i < next; i += 64) { const int gg = bt[i]; ub[i] = gg < ns ? x[gg] : 0.0; }
@@ -451,7 +626,7 @@ extern "C" int sx_msys_destroy (sx_msys_t s)
   if (!s) return SX_OK;
   void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Bm32,
                   s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
-                  s->d_work};
+                  s->d_work, s->d_pc};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete s;
   return SX_OK;
@@ -474,7 +649,7 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                                void* stream, sx_msys_t* out)
 {
   if (!m || !out || num_modes < 1 || base_mode < 0 ||
-      (precision != SX_FP64 && precision != SX_MIXED)) {
+      (precision != SX_FP64 && precision != SX_MIXED && precision != SX_JACPCG)) {
     set_error ("sx_msys_create: bad arguments");
     return SX_ERR_ARG;
   }
@@ -503,6 +678,12 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
   TRY (hipMalloc (&s->d_nsolve, sizeof (int) * num_modes));
   TRY (hipMemcpy (s->d_betak2, s->betak2.data (), sizeof (double) * num_modes, hipMemcpyHostToDevice));
   TRY (hipMemcpy (s->d_nsolve, s->nsolve.data (), sizeof (int) * num_modes, hipMemcpyHostToDevice));
+  if (precision == SX_JACPCG) {      // MatrixSys JACPCG: the preconditioner only
+    TRY (hipMalloc (&s->d_G, sizeof (double) * 2 * num_modes * m->nglobal));
+    if ((rc = pcg_setup (s, st))) return fail (rc);
+    *out = s;
+    return SX_OK;
+  }
   TRY (hipMalloc (&s->d_Bm, sizeof (double) * std::max<size_t> (nmc * next * nint, 1)));
   TRY (hipMalloc (&s->d_Hinv, sizeof (double) * std::max<size_t> (nmc * nint * nint, 1)));
   TRY (hipMalloc (&s->d_bandU, sizeof (double) * num_modes * s->band_stride));
@@ -577,6 +758,72 @@ static int check_solve_args (sx_msys_t s, int nz, int stride, const void* f,
   return SX_OK;
 }
 
+namespace {
+
+size_t tiled_lds (const sx_mesh* m, bool recover)
+{
+  const size_t next = m->next, nint = m->nint, npnp = m->npnp;
+  return sizeof (double) * ((recover ? nint * nint : 0) + next * nint + 2 * ETILE * npnp) +
+         sizeof (int) * npnp;
+}
+
+bool tiled_ok (const sx_mesh* m) { return tiled_lds (m, true) <= 160 * 1024; }
+
+int n_mode_blocks (int nz) { return nz == 1 ? 1 : nz / 2; }
+
+// constrain + condense of one field (F: forcing in, condensed out)
+int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const int npl = n_solved_planes (nz, s->base_mode);
+  if (!tiled_ok (m)) {
+    hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
+                        m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
+                        m->d_emap, m->d_cls, m->d_btog, m->d_emask, s->d_Bm, G,
+                        m->nglobal, F, stride, nz, s->base_mode);
+    SX_HIP (hipGetLastError ());
+    return SX_OK;
+  }
+  if (m->ness)
+    hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
+                        s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
+                        m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
+  const size_t lds = tiled_lds (m, false);
+  SX_HIP (hipFuncSetAttribute ((const void*) k_condense_t,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_condense_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
+                      lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_emask,
+                      m->d_geo, s->d_Bm, F, stride, nz, s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// interior recovery + essential insertion of one field
+int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, const double* G,
+             int rnr, int roff, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  if (!tiled_ok (m)) {
+    hipLaunchKernelGGL (k_recover, dim3 (m->nel, n_solved_planes (nz, s->base_mode)), dim3 (64),
+                        0, st, m->np, m->nclass, m->d_emap, m->d_cls, m->d_btog, m->d_bmask,
+                        m->d_emask, s->d_Bm, s->d_Hinv, G, m->nglobal, F, U, stride,
+                        s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
+    SX_HIP (hipGetLastError ());
+    return SX_OK;
+  }
+  const size_t lds = tiled_lds (m, true);
+  SX_HIP (hipFuncSetAttribute ((const void*) k_recover_t,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_recover_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds,
+                      st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_btog,
+                      m->d_bmask, m->d_emask, s->d_Bm, s->d_Hinv, G, m->nglobal, F, U, stride,
+                      s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+}  // namespace
+
 extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
                                double* d_u, const double* d_lines, void* stream)
 {
@@ -594,33 +841,88 @@ extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
   }
   {
     PhaseScope ps (1, st);
-    hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
-                        m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
-                        m->d_emap, m->d_cls, m->d_btog, m->d_emask, s->d_Bm, s->d_G,
-                        m->nglobal, d_f, stride, nz, s->base_mode);
+    if ((rc = condense (s, nz, stride, d_f, s->d_G, st))) return rc;
   }
   {
     PhaseScope ps (2, st);
     hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                         st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
                         d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
-                        s->base_mode);
+                        s->base_mode, 2, 0);
   }
   SX_HIP (hipGetLastError ());
   {
     PhaseScope ps (3, st);
-    if ((rc = band_solve (s, st))) return rc;
+    if ((rc = band_solve (s, 2, st))) return rc;
   }
   {
     PhaseScope ps (4, st);
-    hipLaunchKernelGGL (k_recover, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
-                        m->d_emap, m->d_cls, m->d_btog, m->d_bmask, m->d_emask, s->d_Bm,
-                        s->d_Hinv, s->d_G, m->nglobal, d_f, d_u, stride, s->d_nsolve,
-                        s->d_rhs, s->nmax, nz, s->base_mode);
+    if ((rc = recover (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st))) return rc;
   }
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
+
+namespace sx {
+
+int msys_reserve_rhs (sx_msys* s, int nr)
+{
+  if (nr != 2 && nr != 4 && nr != 6) {
+    set_error ("msys_reserve_rhs: 2, 4 or 6 right-hand sides per mode");
+    return SX_ERR_ARG;
+  }
+  if (nr <= s->rhs_cap) return SX_OK;
+  if (!s->sw_panel || s->sw_D > 6) {
+    set_error ("msys_reserve_rhs: more than 2 right-hand sides per mode need the panel "
+               "sweeps with at most 6 off-diagonal blocks (band offsets kd <= 447)");
+    return SX_ERR_LIMIT;
+  }
+  double *r = nullptr, *g = nullptr;
+  SX_HIP (hipMalloc (&r, sizeof (double) * s->nmodes * nr * std::max (s->nmax, 1)));
+  SX_HIP (hipMalloc (&g, sizeof (double) * (nr / 2) * 2 * s->nmodes * s->m->nglobal));
+  SX_HIP (hipMemset (r, 0, sizeof (double) * s->nmodes * nr * std::max (s->nmax, 1)));
+  SX_HIP (hipMemset (g, 0, sizeof (double) * (nr / 2) * 2 * s->nmodes * s->m->nglobal));
+  (void) hipFree (s->d_rhs);
+  (void) hipFree (s->d_G);
+  s->d_rhs = r;
+  s->d_G = g;
+  s->rhs_cap = nr;
+  return SX_OK;
+}
+
+int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_f,
+                       double* const* d_u, const double* const* d_lines, hipStream_t st)
+{
+  if (nf < 1 || 2 * nf > s->rhs_cap) {
+    set_error ("field_solve_group: more fields than the reserved right-hand sides");
+    return SX_ERR_ARG;
+  }
+  for (int f = 0; f < nf; f++)
+    if (int rc = check_solve_args (s, nz, stride, d_f[f], d_u[f])) return rc;
+  const sx_mesh* m = s->m;
+  const int npl = n_solved_planes (nz, s->base_mode), nr = 2 * nf;
+  const int lstride = m->nbound * m->np;
+  const size_t gstride = (size_t) 2 * s->nmodes * m->nglobal;
+  for (int f = 0; f < nf; f++) {
+    const double* lines = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
+    double* G = s->d_G + f * gstride;
+    hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                        st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
+    if (int rc = condense (s, nz, stride, d_f[f], G, st)) return rc;
+    hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                        st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
+                        d_f[f], stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                        s->base_mode, nr, 2 * f);
+  }
+  SX_HIP (hipGetLastError ());
+  if (int rc = band_solve (s, nr, st)) return rc;
+  for (int f = 0; f < nf; f++)
+    if (int rc = recover (s, nz, stride, d_f[f], d_u[f], s->d_G + f * gstride, nr, 2 * f, st))
+      return rc;
+  return SX_OK;
+}
+
+}  // namespace sx
 
 extern "C" int sx_field_solve_host (sx_msys_t s, int nz, int stride, const double* f,
                                     double* u, const double* lines)

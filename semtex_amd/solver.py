@@ -17,7 +17,7 @@ import numpy as np
 from . import _lib
 from .mesh import Mesh2D
 
-SX_FP64, SX_MIXED = _lib.SX_FP64, _lib.SX_MIXED
+SX_FP64, SX_MIXED, SX_JACPCG = _lib.SX_FP64, _lib.SX_MIXED, _lib.SX_JACPCG
 
 
 def _stream():
@@ -102,15 +102,26 @@ class Field:
         self.lines = torch.zeros((nz, max(m.nbound * m.np, 1)), dtype=torch.float64,
                                  device=device)
 
-    def solve(self, f, M: ModalMatrixSys, mixed_tol=None, max_iter=20):
-        """Field::solve (src/field.cpp:598-709): f ([nz][plane_size] CUDA
-        float64 tensor) is destroyed; the solution overwrites self.data."""
+    def solve(self, f, M: ModalMatrixSys, mixed_tol=None, max_iter=20, tol_rel=1e-8,
+              step_max=500):
+        """Field::solve (src/field.cpp:598-805): f ([nz][plane_size] CUDA
+        float64 tensor) is destroyed; the solution overwrites self.data.
+        JACPCG systems start from self.data and stop at tol_rel (TOL_REL) or
+        step_max (STEP_MAX)."""
         import torch
         assert f.dtype == torch.float64 and f.is_cuda and f.is_contiguous()
         assert tuple(f.shape) == tuple(self.data.shape)
         L = _lib.load()
         lines = C.c_void_p(self.lines.data_ptr()) if self.gmesh.mesh.nbound else None
-        if M.precision == SX_MIXED:
+        if M.precision == SX_JACPCG:
+            it = C.c_int(0)
+            _lib.check(L.sx_field_solve_pcg(M.handle, self.nz, f.shape[1],
+                                            C.c_void_p(f.data_ptr()),
+                                            C.c_void_p(self.data.data_ptr()), lines,
+                                            float(tol_rel), int(step_max), C.byref(it),
+                                            _stream()), "sx_field_solve_pcg")
+            self.last_iters = it.value
+        elif M.precision == SX_MIXED:
             it = C.c_int(0)
             _lib.check(L.sx_field_solve_mixed(M.handle, self.nz, f.shape[1],
                                               C.c_void_p(f.data_ptr()),
@@ -134,6 +145,19 @@ def helmholtz_op(M: ModalMatrixSys, u, out):
                                            C.c_void_p(out.data_ptr()), _stream()),
                "sx_helmholtz_op")
     return out
+
+
+def fourier_transform(x, sign, out=None):
+    """AuxField::transform (src/auxfield.cpp:1157-1194): real z transform of
+    a [nz][stride] CUDA float64 tensor; sign +1 forward (packed, 1/nz), -1
+    inverse.  In place when `out` is None."""
+    import torch
+    assert x.dtype == torch.float64 and x.is_cuda and x.is_contiguous()
+    y = x if out is None else out
+    _lib.check(_lib.load().sx_fourier_transform(x.shape[0], x.shape[1], C.c_void_p(x.data_ptr()),
+                                                C.c_void_p(y.data_ptr()), int(sign), _stream()),
+               "sx_fourier_transform")
+    return y
 
 
 SX_SKEW, SX_ALTSKEW = _lib.SX_SKEW, _lib.SX_ALTSKEW
