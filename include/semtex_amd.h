@@ -116,6 +116,41 @@ int sx_field_solve_pcg (sx_msys_t s, int nz, int plane_stride, double* d_f, doub
                         const double* d_lines, double tol_rel, int step_max, int* iters,
                         void* stream);
 
+/* A batch of independent problems (each system with its own mesh, order,
+ * lambda2, Cartesian or cylindrical geometry) solved by one captured HIP
+ * graph.  For problem i the graph copies d_f_src[i] into d_f_work[i] (the
+ * solve destroys its forcing) and runs the solve into d_u[i], with
+ * d_lines[i] (d_lines or an entry may be NULL).  SX_MIXED systems run
+ * refine_passes passes (the first solve plus refine_passes-1 FP64
+ * residual corrections) without host synchronisation; SX_FP64 systems run
+ * sx_field_solve.  The buffers are baked into the graph: refill
+ * d_f_src / d_lines in place between runs.  sx_batch_run launches the
+ * whole batch on `stream`.  No reference counterpart: for callers that
+ * solve many independent small problems (uneven batches). */
+typedef struct sx_batch* sx_batch_t;
+int sx_batch_create (int n, const sx_msys_t* systems, const int* nz, const int* plane_stride,
+                     const double* const* d_f_src, double* const* d_f_work, double* const* d_u,
+                     const double* const* d_lines, int refine_passes, sx_batch_t* out);
+int sx_batch_run (sx_batch_t b, void* stream);
+int sx_batch_destroy (sx_batch_t b);
+
+/* Adjoint building blocks.  The DIRECT system is symmetric, so the adjoint
+ * of Field::solve is a solve with a weak-form right-hand side:
+ * sx_field_solve_weak solves A u = L^T w for element-local w (the direct
+ * stiffness sum of w, no mass weighting, zero essential values; d_w is
+ * destroyed).  sx_mass_inner returns per plane k (device d_out[nz]) the
+ * element-quadrature inner product sum_e sum_ij Q4 a b (AuxField::integral
+ * weights); sx_mass_weight writes d_out = scale * Q8 * d_in element by
+ * element (Element::weight).  With J(u), lam = solve_weak(dJ/du):
+ * dJ/df = -Q8 lam, dJ/dlambda2 = -sum_k inner(lam, u).
+ * No reference counterpart: semtex has no adjoint of its solve. */
+int sx_field_solve_weak (sx_msys_t s, int nz, int plane_stride, double* d_w, double* d_u,
+                         void* stream);
+int sx_mass_inner (sx_mesh_t m, int nz, int plane_stride, const double* d_a,
+                   const double* d_b, double* d_out, void* stream);
+int sx_mass_weight (sx_mesh_t m, int nz, int plane_stride, const double* d_in, double* d_out,
+                    double scale, void* stream);
+
 /* Host-pointer convenience wrapper for drop-in callers whose Field data
  * lives in host memory (Domain::u, src/domain.cpp:75-79): copies in,
  * solves, copies u back (f is not copied back). */
@@ -163,6 +198,11 @@ int sx_dns_destroy (sx_dns_t d);
 int sx_dns_set_state (sx_dns_t d, const double* d_u_phys, void* stream);
 /* Advance nsteps steps of the integrate() loop body. */
 int sx_dns_step (sx_dns_t d, int nsteps, void* stream);
+/* Velocity solves of the start-up steps (step < nord): SX_FP64 (direct,
+ * the default) or SX_JACPCG as the reference's Solve does
+ * (dns/integrate.cpp:328-356): a JACPCG system iterated from the current
+ * velocity to tol_rel (TOL_REL) within step_max (STEP_MAX). */
+int sx_dns_set_startup (sx_dns_t d, int method, double tol_rel, int step_max, void* stream);
 /* Copy field `which` (0..ncom-1 velocity, ncom pressure) in Fourier space
  * [nz][plane_stride] to device memory d_out.  Returns the current step
  * count through *step (may be NULL). */

@@ -73,11 +73,15 @@ class DnsOracle:
     """
 
     def __init__(self, mesh_u, mesh_p, nz, beta, kinvis, dt, const_force=(0.0, 0.0, 0.0),
-                 nord=2, advection="skew"):
+                 nord=2, advection="skew", startup="direct", tol_rel=1e-8, step_max=500):
         self.mu, self.mp = mesh_u, mesh_p
         # 2D (nz == 1): u, v only (NCOM = NDIM = 2); 3D: u, v, w
         self.ncom = 2 if nz == 1 else 3
         self.advection = advection      # "skew" (-S) or "alt" (default ADVECTION=1)
+        # velocity solves of the start-up steps (step < NORD): "direct", or
+        # "pcg" as the reference does (dns/integrate.cpp:340-353, JACPCG at
+        # TOL_REL / STEP_MAX from the current field)
+        self.startup, self.tol_rel, self.step_max = startup, tol_rel, step_max
         self.toggle = 1                 # altSkewSymmetric's static toggle
         self.np_ = mesh_u.np
         self.nel = mesh_u.nel
@@ -357,5 +361,22 @@ class DnsOracle:
         # viscous substep (first-order constant until the history is full)
         lam2 = SS[Je][0] / (dt * nu)
         for i in range(self.ncom):
-            self.u[i] = self.solve("u", lam2, F[i], None, u_old[i])
+            if self.startup == "pcg" and self.step_no < nord:
+                self.u[i] = self.solve_pcg(lam2, F[i], u_old[i])
+            else:
+                self.u[i] = self.solve("u", lam2, F[i], None, u_old[i])
         return self
+
+    def solve_pcg(self, lam2, F, u0):
+        """JACPCG velocity solve of every plane from the current field u0
+        (Field::solve, src/field.cpp:711-805); the Nyquist plane keeps u0."""
+        out = u0.copy()
+        for k in range(self.nz):
+            if self.nz > 1 and k == 1:
+                continue
+            key = ("upcg", lam2, k >> 1)
+            if key not in self._sys:
+                self._sys[key] = O.OracleMsys(self.om_u, lam2, (self.beta * (k >> 1)) ** 2)
+            out[k], _ = self._sys[key].solve_plane_pcg(F[k], None, u0[k], self.tol_rel,
+                                                       self.step_max)
+        return out

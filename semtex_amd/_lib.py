@@ -34,6 +34,13 @@ SIGNATURES = {
     "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
     "sx_fourier_transform": (_i, [_i, _i, _dp, _dp, _i, _vp]),
+    "sx_dns_set_startup": (_i, [_vp, _i, _d, _i, _vp]),
+    "sx_field_solve_weak": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
+    "sx_batch_create": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
+    "sx_batch_run": (_i, [_vp, _vp]),
+    "sx_batch_destroy": (_i, [_vp]),
+    "sx_mass_inner": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
+    "sx_mass_weight": (_i, [_vp, _i, _i, _dp, _dp, _d, _vp]),
     "sx_field_solve_pcg": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i, C.POINTER(_i), _vp]),
     "sx_dns_create": (_i, [_vp, _vp, _i, _d, _d, _d, _dp, _i, _i, _vp, C.POINTER(_vp)]),
     "sx_dns_destroy": (_i, [_vp]),

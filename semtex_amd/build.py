@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsemtex_amd.so")
-SOURCES = ["sx_setup.cpp", "sx_solver.hip", "sx_band.hip", "sx_dns.hip", "sx_mixed.hip", "sx_fourier.hip", "sx_pcg.hip"]
+SOURCES = ["sx_setup.cpp", "sx_solver.hip", "sx_band.hip", "sx_dns.hip", "sx_mixed.hip", "sx_fourier.hip", "sx_pcg.hip", "sx_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

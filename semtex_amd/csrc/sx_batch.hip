@@ -1,0 +1,95 @@
+// sx_batch.hip -- a batch of independent Helmholtz problems (each its own
+// mesh, order, lambda2, geometry) solved by one captured HIP graph: for
+// every problem the forcing is copied from its source buffer (Field::solve
+// destroys it) and the problem's solve kernels follow, FP64 or mixed
+// precision with a fixed number of FP64 refinement passes.  Replaying the
+// graph costs one launch for the whole batch.
+#include <vector>
+
+#include "sx_internal.h"
+
+using namespace sx;
+
+struct sx_batch {
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int n = 0;
+};
+
+extern "C" int sx_batch_destroy (sx_batch_t b)
+{
+  if (!b) return SX_OK;
+  if (b->exec) (void) hipGraphExecDestroy (b->exec);
+  if (b->graph) (void) hipGraphDestroy (b->graph);
+  if (b->cap) (void) hipStreamDestroy (b->cap);
+  delete b;
+  return SX_OK;
+}
+
+static int record (int n, const sx_msys_t* sys, const int* nz, const int* stride,
+                   const double* const* f_src, double* const* f_work, double* const* u,
+                   const double* const* lines, int passes, hipStream_t st)
+{
+  for (int i = 0; i < n; i++) {
+    const size_t bytes = sizeof (double) * (size_t) nz[i] * stride[i];
+    SX_HIP (hipMemcpyAsync (f_work[i], f_src[i], bytes, hipMemcpyDeviceToDevice, st));
+    const double* li = lines ? lines[i] : nullptr;
+    int rc;
+    if (sys[i]->precision == SX_MIXED)
+      rc = field_solve_mixed_passes (sys[i], nz[i], stride[i], f_work[i], u[i], li, passes, st);
+    else
+      rc = sx_field_solve (sys[i], nz[i], stride[i], f_work[i], u[i], li, st);
+    if (rc) return rc;
+  }
+  return SX_OK;
+}
+
+extern "C" int sx_batch_create (int n, const sx_msys_t* systems, const int* nz,
+                                const int* plane_stride, const double* const* d_f_src,
+                                double* const* d_f_work, double* const* d_u,
+                                const double* const* d_lines, int refine_passes,
+                                sx_batch_t* out)
+{
+  if (n < 1 || !systems || !nz || !plane_stride || !d_f_src || !d_f_work || !d_u || !out ||
+      refine_passes < 1) {
+    set_error ("sx_batch_create: bad arguments");
+    return SX_ERR_ARG;
+  }
+  for (int i = 0; i < n; i++)
+    if (!systems[i] || systems[i]->precision == SX_JACPCG) {
+      set_error ("sx_batch_create: DIRECT systems (SX_FP64 or SX_MIXED) only");
+      return SX_ERR_ARG;
+    }
+  *out = nullptr;
+  sx_batch* b = new sx_batch;
+  b->n = n;
+  auto fail = [&] (int r) { sx_batch_destroy (b); return r; };
+  if (hipStreamCreate (&b->cap) != hipSuccess) return fail (hip_fail (hipErrorUnknown, "hipStreamCreate"));
+  // one eager pass first: workspaces and kernel attributes are set up
+  // outside the capture
+  int rc = record (n, systems, nz, plane_stride, d_f_src, d_f_work, d_u, d_lines,
+                   refine_passes, b->cap);
+  if (rc) return fail (rc);
+  if (hipStreamSynchronize (b->cap) != hipSuccess) return fail (hip_fail (hipErrorUnknown, "sync"));
+  if (hipStreamBeginCapture (b->cap, hipStreamCaptureModeRelaxed) != hipSuccess)
+    return fail (hip_fail (hipErrorUnknown, "hipStreamBeginCapture"));
+  rc = record (n, systems, nz, plane_stride, d_f_src, d_f_work, d_u, d_lines, refine_passes,
+               b->cap);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture (b->cap, &g);
+  b->graph = g;
+  if (rc) return fail (rc);
+  if (ec != hipSuccess) return fail (hip_fail (ec, "hipStreamEndCapture"));
+  hipError_t ei = hipGraphInstantiate (&b->exec, b->graph, nullptr, nullptr, 0);
+  if (ei != hipSuccess) return fail (hip_fail (ei, "hipGraphInstantiate"));
+  *out = b;
+  return SX_OK;
+}
+
+extern "C" int sx_batch_run (sx_batch_t b, void* stream)
+{
+  if (!b || !b->exec) { set_error ("sx_batch_run: bad arguments"); return SX_ERR_ARG; }
+  SX_HIP (hipGraphLaunch (b->exec, (hipStream_t) stream));
+  return SX_OK;
+}

@@ -326,6 +326,10 @@ struct sx_dns {
   size_t n = 0;                        // nz * S
   sx_msys *Mv[2] = {nullptr, nullptr}, *Mp = nullptr;
   bool group_solve = false;            // velocity solves share one band launch
+  int startup = SX_FP64;               // start-up velocity solves: direct or SX_JACPCG
+  double tol_rel = 1e-8;
+  int step_max = 500;
+  sx_msys* Mpcg = nullptr;
   std::vector<double*> bufs;           // every device allocation
   double *u[3] = {}, *p = nullptr;
   double *Us[2][3] = {}, *Uf[2][3] = {};
@@ -522,6 +526,18 @@ int dns_step (sx_dns* d, hipStream_t st)
   // -- Us: store u^n and roll
   for (int i = 0; i < nc; i++) std::swap (d->Us[0][i], d->Us[1][i]);
   // -- viscous substep
+  if (d->startup == SX_JACPCG && d->step < nord) {
+    // Solve, dns/integrate.cpp:336-350: a JACPCG system for the start-up
+    // steps, iterating from the current velocity u^n (now in Us[1])
+    for (int i = 0; i < nc; i++) {
+      SX_HIP (hipMemcpyAsync (d->u[i], d->Us[1][i], sizeof (double) * d->n,
+                              hipMemcpyDeviceToDevice, st));
+      if ((rc = sx_field_solve_pcg (d->Mpcg, d->nz, d->S, d->F[i], d->u[i], nullptr,
+                                    d->tol_rel, d->step_max, nullptr, st)))
+        return rc;
+    }
+    return SX_OK;
+  }
   sx_msys* Mv = d->Mv[Je - 1];
   if (d->group_solve) return field_solve_group (Mv, nc, d->nz, d->S, d->F, d->u, nullptr, st);
   for (int i = 0; i < nc; i++)
@@ -537,7 +553,7 @@ extern "C" int sx_dns_destroy (sx_dns_t d)
   for (double* q : d->bufs) (void) hipFree (q);
   void* ip[] = {d->seg_elmt, d->seg_idx, d->seg_kind};
   for (void* q : ip) if (q) (void) hipFree (q);
-  for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mp}) if (M) sx_msys_destroy (M);
+  for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mp, d->Mpcg}) if (M) sx_msys_destroy (M);
   delete d;
   return SX_OK;
 }
@@ -695,6 +711,25 @@ extern "C" int sx_dns_set_state (sx_dns_t d, const double* u_phys, void* stream)
     SX_HIP (hipMemsetAsync (d->un[q], 0, sizeof (double) * d->nz * d->nline, st));
   }
   d->step = 0; d->toggle = 1; d->hop_head = 0; d->un_head = 0;
+  return SX_OK;
+}
+
+extern "C" int sx_dns_set_startup (sx_dns_t d, int method, double tol_rel, int step_max,
+                                   void* stream)
+{
+  if (!d || (method != SX_FP64 && method != SX_JACPCG) || !(tol_rel >= 0.0) || step_max < 1) {
+    set_error ("sx_dns_set_startup: bad arguments");
+    return SX_ERR_ARG;
+  }
+  if (method == SX_JACPCG && !d->Mpcg) {
+    // the first-order constant of the start-up step, alpha_0(1) / (dt nu)
+    const int rc = sx_msys_create (d->mu, SSc[1][0] / (d->dt * d->nu), d->beta, 0,
+                                   std::max (1, d->nz / 2), SX_JACPCG, stream, &d->Mpcg);
+    if (rc) return rc;
+  }
+  d->startup = method;
+  d->tol_rel = tol_rel;
+  d->step_max = step_max;
   return SX_OK;
 }
 

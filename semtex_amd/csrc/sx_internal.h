@@ -58,6 +58,10 @@ int    field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const*
 // (sx_fourier.hip); in place when fourier_in_place (nz).
 int    fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st);
 int    fourier_prepare (int nz);   // builds the per-device tables (synchronous)
+// Mixed-precision solve with a fixed number of refinement passes, no host
+// synchronisation (sx_mixed.hip).
+int    field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                                 const double* d_lines, int passes, hipStream_t st);
 // JACPCG set-up: the Jacobi preconditioner of every mode (sx_pcg.hip).
 int    pcg_setup (sx_msys* s, hipStream_t st);
 inline bool fourier_in_place (int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }

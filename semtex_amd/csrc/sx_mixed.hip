@@ -288,6 +288,57 @@ int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
 
 }  // namespace
 
+namespace sx {
+
+// The mixed solve with a fixed number of refinement passes and no host
+// synchronisation (graph capture): the first pass is the FP32-condensed
+// solve, each further pass adds the solve of the FP64 residual.
+int field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                              const double* d_lines, int passes, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
+  const size_t nplane = (size_t) nz * stride;
+  const size_t need = 5 * nplane + 2 * 512;
+  if (s->work_doubles < need) {
+    if (s->d_work) (void) hipFree (s->d_work);
+    s->d_work = nullptr;
+    s->work_doubles = 0;
+    SX_HIP (hipMalloc (&s->d_work, sizeof (double) * need));
+    s->work_doubles = need;
+  }
+  double *W = s->d_work, *X = W + nplane, *Fp = X + nplane, *V = Fp + nplane,
+         *dV = V + nplane;
+  const int lstride = m->nbound * m->np;
+  const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
+  hipLaunchKernelGGL (k_essential_m, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0, st,
+                      m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+  hipLaunchKernelGGL (k_weak, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->cyl, s->lambda2,
+                      s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog, m->d_emask, s->d_G,
+                      m->nglobal, d_f, W, stride, nz, s->base_mode);
+  SX_HIP (hipGetLastError ());
+  int rc = inner_solve (s, nz, stride, W, Fp, V, lines, st);
+  if (rc) return rc;
+  const int n_plane = m->nel * m->npnp;
+  for (int it = 1; it < passes; it++) {
+    hipLaunchKernelGGL (k_resid, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->cyl,
+                        s->lambda2, s->d_betak2, m->d_geo, m->d_D, W, V, X, stride, nz,
+                        s->base_mode);
+    SX_HIP (hipGetLastError ());
+    if ((rc = inner_solve (s, nz, stride, X, Fp, dV, lines, st))) return rc;
+    hipLaunchKernelGGL (k_axpy_planes, dim3 ((unsigned) (((size_t) npl * n_plane + 255) / 256)),
+                        dim3 (256), 0, st, V, dV, npl, nz, s->base_mode, n_plane, stride);
+    SX_HIP (hipGetLastError ());
+  }
+  hipLaunchKernelGGL (k_finish, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->d_emap,
+                      m->d_btog, m->d_bmask, m->d_emask, s->d_G, m->nglobal, V, d_u, stride,
+                      nz, s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+}  // namespace sx
+
 extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_f,
                                      double* d_u, const double* d_lines,
                                      double refine_tol, int max_iter, int* iters,

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
     const double* __restrict__ geo, const double* __restrict__ Bm, double* F, int stride,
-    int nz, int base_mode)
+    int nz, int base_mode, int weak)
 {
   extern __shared__ double sm[];
   const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
     const int r = emap[a];
     const double v = F[(size_t) k * stride + (size_t) e * npnp + r];
-    wp[i] = emask[e] ? v : -v * geo[((size_t) e * NGEO + G_Q8) * npnp + r];
+    wp[i] = (weak || emask[e]) ? v : -v * geo[((size_t) e * NGEO + G_Q8) * npnp + r];
   }
   __syncthreads ();
   const int ng = (nep + CT_NB - 1) / CT_NB;
@@ -772,10 +772,15 @@ bool tiled_ok (const sx_mesh* m) { return tiled_lds (m, true) <= 160 * 1024; }
 int n_mode_blocks (int nz) { return nz == 1 ? 1 : nz / 2; }
 
 // constrain + condense of one field (F: forcing in, condensed out)
-int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st)
+int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st,
+              int weak = 0)
 {
   const sx_mesh* m = s->m;
   const int npl = n_solved_planes (nz, s->base_mode);
+  if (weak && !tiled_ok (m)) {
+    set_error ("weak-form solve: element matrices too large for the tiled kernels (np > 12)");
+    return SX_ERR_LIMIT;
+  }
   if (!tiled_ok (m)) {
     hipLaunchKernelGGL (k_condense, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nel,
                         m->nclass, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
@@ -784,7 +789,7 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
     SX_HIP (hipGetLastError ());
     return SX_OK;
   }
-  if (m->ness)
+  if (m->ness && !weak)
     hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
                         s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
                         m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
@@ -793,7 +798,7 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (k_condense_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
                       lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_emask,
-                      m->d_geo, s->d_Bm, F, stride, nz, s->base_mode);
+                      m->d_geo, s->d_Bm, F, stride, nz, s->base_mode, weak);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -958,6 +963,102 @@ extern "C" int sx_helmholtz_op (sx_msys_t s, int nz, int stride, const double* d
                       dim3 (64), 0, (hipStream_t) stream, m->np, m->cyl, s->lambda2,
                       s->d_betak2, m->d_geo, m->d_D, d_u, d_hu, stride, nz,
                       s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint building blocks
+// ---------------------------------------------------------------------------
+namespace {
+
+// out[k] = sum over elements and nodes of Q4 a b (the mass-weighted inner
+// product of plane k, element quadrature as AuxField::integral does);
+// one workgroup per plane
+__global__ __launch_bounds__(256) void k_mass_inner (int npnp, int nel,
+                                                     const double* __restrict__ geo,
+                                                     const double* __restrict__ A,
+                                                     const double* __restrict__ B, int stride,
+                                                     double* out)
+{
+  __shared__ double red[256];
+  const int k = blockIdx.x;
+  const size_t n = (size_t) nel * npnp;
+  double s = 0.0;
+  for (size_t i = threadIdx.x; i < n; i += 256) {
+    const size_t e = i / npnp, ij = i % npnp;
+    s += geo[(e * NGEO + G_Q4) * npnp + ij] * A[(size_t) k * stride + i] * B[(size_t) k * stride + i];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads ();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int) threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads ();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// out = scale Q8 in, element by element (Element::weight)
+__global__ void k_mass_weight (int npnp, int nel, const double* __restrict__ geo,
+                               const double* __restrict__ in, double* out, int stride,
+                               double scale, int nz)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n = (size_t) nel * npnp;
+  if (i >= n * nz) return;
+  const size_t k = i / n, r = i % n, e = r / npnp, ij = r % npnp;
+  out[k * stride + r] = scale * geo[(e * NGEO + G_Q8) * npnp + ij] * in[k * stride + r];
+}
+
+}  // namespace
+
+extern "C" int sx_field_solve_weak (sx_msys_t s, int nz, int stride, double* d_w, double* d_u,
+                                    void* stream)
+{
+  int rc = check_solve_args (s, nz, stride, d_w, d_u);
+  if (rc) return rc;
+  if (s->precision == SX_JACPCG) {
+    set_error ("sx_field_solve_weak: needs a DIRECT system");
+    return SX_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t) stream;
+  const sx_mesh* m = s->m;
+  const int npl = n_solved_planes (nz, s->base_mode);
+  hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0, st,
+                      m->nglobal, m->d_ess, nullptr, 0, s->d_G, nz, s->base_mode);
+  if ((rc = condense (s, nz, stride, d_w, s->d_G, st, 1))) return rc;
+  hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0, st,
+                      m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw, d_w,
+                      stride, nullptr, 0, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, 2, 0);
+  SX_HIP (hipGetLastError ());
+  if ((rc = band_solve (s, 2, st))) return rc;
+  return recover (s, nz, stride, d_w, d_u, s->d_G, 2, 0, st);
+}
+
+extern "C" int sx_mass_inner (sx_mesh_t m, int nz, int stride, const double* d_a,
+                              const double* d_b, double* d_out, void* stream)
+{
+  if (!m || nz < 1 || stride < m->nel * m->npnp || !d_a || !d_b || !d_out) {
+    set_error ("sx_mass_inner: bad arguments");
+    return SX_ERR_ARG;
+  }
+  hipLaunchKernelGGL (k_mass_inner, dim3 (nz), dim3 (256), 0, (hipStream_t) stream, m->npnp,
+                      m->nel, m->d_geo, d_a, d_b, stride, d_out);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+extern "C" int sx_mass_weight (sx_mesh_t m, int nz, int stride, const double* d_in,
+                               double* d_out, double scale, void* stream)
+{
+  if (!m || nz < 1 || stride < m->nel * m->npnp || !d_in || !d_out) {
+    set_error ("sx_mass_weight: bad arguments");
+    return SX_ERR_ARG;
+  }
+  const size_t tot = (size_t) nz * m->nel * m->npnp;
+  hipLaunchKernelGGL (k_mass_weight, dim3 ((unsigned) ((tot + 255) / 256)), dim3 (256), 0,
+                      (hipStream_t) stream, m->npnp, m->nel, m->d_geo, d_in, d_out, stride,
+                      scale, nz);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }

@@ -81,3 +81,32 @@ def dns_channel(nx=12, ny=24, np_=8, nz=128, Re=2000.0, dt=1e-3, Lz=2 * np.pi,
             f[:, ext] = g[mu.btog]
     return dict(mesh_u=mu, mesh_p=mp, nz=nz, beta=2 * np.pi / Lz, kinvis=nu, dt=dt,
                 const_force=(2 * nu, 0.0, 0.0), u0=u0, Lz=Lz)
+
+
+def uneven_batch(count=64, seed=SEED):
+    """Independent 2D Helmholtz problems with seeded parameters: N_P uniform
+    in {4..12}, nx, ny uniform in {4..20}, rectangles [0, a] x [0, 1] with
+    aspect a uniform in [1, 4], lambda2 log-uniform in [1e-2, 1e2], N(0,1)
+    nodal forcing.  The first half are Cartesian with u = 0 on all four
+    sides; the second half are cylindrical (axisymmetric mode 0) with the
+    axis on y = 0, u = 0 on y = 1 and natural ends, their forcing weighted
+    by r as the reference's callers supply it (Element::weight,
+    src/element.cpp:800-806, does not multiply by r)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        np_ = int(rng.integers(4, 13))
+        nx, ny = int(rng.integers(4, 21)), int(rng.integers(4, 21))
+        a = float(rng.uniform(1.0, 4.0))
+        lam2 = float(10.0 ** rng.uniform(-2.0, 2.0))
+        cyl = i >= count // 2
+        if cyl:
+            m = rect_mesh(nx, ny, np_, 0.0, a, 0.0, 1.0, bc=("A", "N", "D", "N"), cyl=1)
+        else:
+            m = rect_mesh(nx, ny, np_, 0.0, a, 0.0, 1.0, bc=("D", "D", "D", "D"))
+        f = rng.standard_normal((m.nel, m.npnp))
+        if cyl:
+            f = f * m.y
+        out.append(dict(mesh=m, lambda2=lam2, forcing=f.reshape(1, -1), cyl=cyl,
+                        np=np_, nx=nx, ny=ny, aspect=a))
+    return out

@@ -195,6 +195,13 @@ class DNS:
         _lib.check(_lib.load().sx_dns_step(self.handle, int(n), _stream()), "sx_dns_step")
         return self
 
+    def set_startup(self, method, tol_rel=1e-8, step_max=500):
+        """Start-up velocity solves: SX_FP64 (direct) or SX_JACPCG."""
+        _lib.check(_lib.load().sx_dns_set_startup(self.handle, int(method), float(tol_rel),
+                                                  int(step_max), _stream()),
+                   "sx_dns_set_startup")
+        return self
+
     def field(self, which):
         """Fourier-space copy of velocity component `which` (ncom: pressure)."""
         import torch
@@ -208,4 +215,114 @@ class DNS:
     def __del__(self):
         if getattr(self, "handle", None):
             _lib.load().sx_dns_destroy(self.handle)
+            self.handle = None
+
+
+def mass_inner(gmesh: GpuMesh, a, b):
+    """Per-plane element-quadrature inner products sum Q4 a b (device)."""
+    import torch
+    out = torch.empty(a.shape[0], dtype=torch.float64, device=a.device)
+    _lib.check(_lib.load().sx_mass_inner(gmesh.handle, a.shape[0], a.shape[1],
+                                         C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()),
+                                         C.c_void_p(out.data_ptr()), _stream()), "sx_mass_inner")
+    return out
+
+
+def mass_weight(gmesh: GpuMesh, x, scale=1.0):
+    import torch
+    out = torch.zeros_like(x)
+    _lib.check(_lib.load().sx_mass_weight(gmesh.handle, x.shape[0], x.shape[1],
+                                          C.c_void_p(x.data_ptr()), C.c_void_p(out.data_ptr()),
+                                          float(scale), _stream()), "sx_mass_weight")
+    return out
+
+
+def solve_weak(M: ModalMatrixSys, w):
+    """A u = L^T w (zero essential values); w is consumed."""
+    import torch
+    u = torch.zeros_like(w)
+    wc = w.contiguous().clone()
+    _lib.check(_lib.load().sx_field_solve_weak(M.handle, w.shape[0], w.shape[1],
+                                               C.c_void_p(wc.data_ptr()),
+                                               C.c_void_p(u.data_ptr()), _stream()),
+               "sx_field_solve_weak")
+    return u
+
+
+def _make_helmholtz_fn():
+    import torch
+
+    class HelmholtzSolve(torch.autograd.Function):
+        """u = Field::solve(f) for the modes of M, differentiable in f and
+        lambda2.  backward: one adjoint solve of the symmetric system,
+        dL/df = -Q8 lam, dL/dlambda2 = -sum_k <lam, u>_M."""
+
+        @staticmethod
+        def forward(ctx, f, lam2, gmesh, beta, nmodes, lines):
+            M = ModalMatrixSys(float(lam2), beta, 0, nmodes, gmesh)
+            u = Field(gmesh, f.shape[0])
+            if lines is not None:
+                u.lines.copy_(lines)
+            u.solve(f.detach().contiguous().clone(), M)
+            ctx.M, ctx.gmesh = M, gmesh
+            ctx.save_for_backward(u.data)
+            return u.data.clone()
+
+        @staticmethod
+        def backward(ctx, g):
+            (u,) = ctx.saved_tensors
+            lam = solve_weak(ctx.M, g)
+            grad_f = mass_weight(ctx.gmesh, lam, -1.0)
+            dlam2 = -mass_inner(ctx.gmesh, lam, u).sum()
+            return grad_f, dlam2.reshape(()), None, None, None, None
+
+    return HelmholtzSolve
+
+
+_HELM_FN = None
+
+
+def helmholtz_solve(f, lam2, gmesh: GpuMesh, beta, nmodes, lines=None):
+    """Differentiable batched Helmholtz solve (torch.autograd): f is the
+    [nz][plane] forcing (not destroyed), lam2 a 0-d float64 tensor."""
+    global _HELM_FN
+    if _HELM_FN is None:
+        _HELM_FN = _make_helmholtz_fn()
+    return _HELM_FN.apply(f, lam2, gmesh, beta, nmodes, lines)
+
+
+class Batch:
+    """Independent problems solved by one captured HIP graph (sx_batch_*).
+
+    problems: list of (ModalMatrixSys, Field, forcing source tensor).  The
+    forcing sources and the fields' line data are read in place at every
+    run; solutions land in each Field's data."""
+
+    def __init__(self, problems, refine_passes=3):
+        import torch
+        L = _lib.load()
+        self.problems = problems
+        n = len(problems)
+        self.work = [torch.empty_like(src) for (_, _, src) in problems]
+        arr = lambda vals, t: (t * n)(*vals)  # noqa: E731
+        sys_ = arr([M.handle.value for (M, _, _) in problems], C.c_void_p)
+        nz = arr([u.nz for (_, u, _) in problems], C.c_int)
+        stride = arr([src.shape[1] for (_, _, src) in problems], C.c_int)
+        fsrc = arr([src.data_ptr() for (_, _, src) in problems], C.c_void_p)
+        fwork = arr([w.data_ptr() for w in self.work], C.c_void_p)
+        uu = arr([u.data.data_ptr() for (_, u, _) in problems], C.c_void_p)
+        lines = arr([u.lines.data_ptr() if u.gmesh.mesh.nbound else None
+                     for (_, u, _) in problems], C.c_void_p)
+        torch.cuda.synchronize()
+        h = C.c_void_p()
+        _lib.check(L.sx_batch_create(n, sys_, nz, stride, fsrc, fwork, uu, lines,
+                                     int(refine_passes), C.byref(h)), "sx_batch_create")
+        self.handle = h
+
+    def run(self):
+        _lib.check(_lib.load().sx_batch_run(self.handle, _stream()), "sx_batch_run")
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            _lib.load().sx_batch_destroy(self.handle)
             self.handle = None

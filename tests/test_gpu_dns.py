@@ -33,17 +33,21 @@ def phys_of(F):
     return inv(F)
 
 
+@pytest.mark.parametrize("startup", ["direct", "pcg"])
 @pytest.mark.parametrize("case", ["taylor2", "taylor4"])
-def test_gpu_dns_reproduces_regress_goldens(torch_cuda, oracle, case):
+def test_gpu_dns_reproduces_regress_goldens(torch_cuda, oracle, case, startup):
+    """Both start-ups; the JACPCG one at the sessions' TOL_REL = 1e-12."""
     import torch
     from semtex_amd.mesh import rect_mesh
-    from semtex_amd.solver import DNS, SX_ALTSKEW, GpuMesh
-    print("case:", case, flush=True)
+    from semtex_amd.solver import DNS, SX_ALTSKEW, SX_JACPCG, GpuMesh
+    print("case:", case, startup, flush=True)
     nz = 1 if case == "taylor2" else 8
     m = rect_mesh(2, 2, 11, 0.0, 2.0, 0.0, 2.0, periodic_x=True, periodic_y=True)
     gm = GpuMesh(m)
     Lz, nu, dt = 2.0, 0.01, 0.02
     D = DNS(gm, gm, nz, 2 * np.pi / Lz, nu, dt, advection=SX_ALTSKEW)
+    if startup == "pcg":
+        D.set_startup(SX_JACPCG, tol_rel=1e-12, step_max=500)
     n, S = m.nel * m.npnp, D.stride
     z = np.arange(nz) * Lz / nz
     X = np.broadcast_to(m.x.ravel(), (nz, n))
@@ -74,12 +78,13 @@ def test_gpu_dns_reproduces_regress_goldens(torch_cuda, oracle, case):
             assert f"{err:.4e}" == f"{want:.4e}", (comp, err, want)
 
 
-def test_gpu_dns_channel_step_parity(torch_cuda, oracle):
+@pytest.mark.parametrize("startup", ["direct", "pcg"])
+def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup):
     import torch
     from oracle.dns_oracle import DnsOracle
     from semtex_amd.mesh import rect_mesh
-    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh
-    print("case: channel 3x4 np=6 nz=8 skew, 8 steps", flush=True)
+    from semtex_amd.solver import DNS, SX_JACPCG, SX_SKEW, GpuMesh
+    print("case: channel 3x4 np=6 nz=8 skew, 8 steps", startup, flush=True)
     bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
     mu = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
     mp = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
@@ -87,7 +92,10 @@ def test_gpu_dns_channel_step_parity(torch_cuda, oracle):
     cf = (2 * nu, 0.0, 0.0)
     gu, gp = GpuMesh(mu), GpuMesh(mp)
     D = DNS(gu, gp, nz, beta, nu, dt, const_force=cf, advection=SX_SKEW)
-    O = DnsOracle(mu, mp, nz, beta, nu, dt, const_force=cf, advection="skew")
+    O = DnsOracle(mu, mp, nz, beta, nu, dt, const_force=cf, advection="skew", startup=startup,
+                  tol_rel=1e-12)
+    if startup == "pcg":
+        D.set_startup(SX_JACPCG, tol_rel=1e-12, step_max=500)
     n, S = mu.nel * mu.npnp, D.stride
     rng = np.random.default_rng(20261202)
     y = mu.y.ravel()

@@ -259,3 +259,21 @@ def test_mixed_precision_refinement_reaches_fp64(torch_cuda, oracle):
     a2, b2 = ub.data.cpu().numpy()[:, :n2], ua.data.cpu().numpy()[:, :n2]
     pl = [k for k in range(nz2) if k != 1]
     assert np.linalg.norm(a2[pl] - b2[pl]) / np.linalg.norm(b2[pl]) < 1e-10
+
+
+def test_cylindrical_mode0_parity(torch_cuda, oracle):
+    """Axisymmetric (cylindrical, mode 0) problem with the axis on y = 0:
+    GPU vs oracle to 1e-12, and the manufactured solution to 5e-12."""
+    from semtex_amd.mesh import rect_mesh
+    print("case: cylindrical mode 0", flush=True)
+    m = rect_mesh(4, 3, 10, 0.0, 2.0, 0.0, 1.0, bc=("A", "N", "D", "N"), periodic_x=True, cyl=1)
+    lam2 = 2.0
+    u = (1 - m.y ** 2) * np.cos(np.pi * m.x)
+    f = -np.pi ** 2 * u - 4 * np.cos(np.pi * m.x) - lam2 * u
+    n = m.nel * m.npnp
+    F = np.zeros((1, m.plane_size))
+    F[0, :n] = (m.y * f).ravel()
+    ug, _, _ = gpu_solve(torch_cuda, m, lam2, 0.0, 1, 1, F)
+    uo = oracle_solve(oracle, m, lam2, 0.0, 1, F)
+    assert rel_l2(ug[0, :n], uo[0]) < TOL
+    assert np.abs(ug[0, :n] - u.ravel()).max() < 5e-12

@@ -1,9 +1,11 @@
 """GPU JACPCG (sx_field_solve_pcg) against the oracle's JACPCG restatement
 (src/field.cpp:711-805) plane by plane, from the same initial guess:
 
-* at the reference's default tolerance (TOL_REL = 1e-8) every plane takes
-  the oracle's iteration count (+-1 where round-off moves the stopping
-  test) and agrees with it to 1e-9 relative;
+* at the reference's default tolerance (TOL_REL = 1e-8) the largest
+  iteration count matches the oracle's to +-2 and every plane agrees to
+  10 TOL_REL (two CG runs whose dot products are summed in different order
+  drift apart by about the tolerance times the conditioning before they
+  converge); at TOL_REL = 1e-12 they agree to 1e-10;
 * at a tight tolerance it equals the direct solve to 1e-10;
 * the DNS start-up constant lambda2 = 1/(dt nu) converges in a few
   iterations, as in the reference.
@@ -55,24 +57,26 @@ def _gpu(torch, m, nz, lam2, beta, F, lines, U0, tol, step_max):
     return u.data.cpu().numpy(), u.last_iters
 
 
+@pytest.mark.parametrize("tol", [1e-8, 1e-12])
 @pytest.mark.parametrize("lam2", [2.0, 1.0 / (1e-3 * 5e-4)])
-def test_pcg_matches_oracle_pcg(torch_cuda, oracle, lam2):
+def test_pcg_matches_oracle_pcg(torch_cuda, oracle, lam2, tol):
     from oracle.oracle import OracleMesh, OracleMsys
-    print("case: pcg lam2", lam2, flush=True)
+    print("case: pcg lam2", lam2, "tol", tol, flush=True)
     m, nz, F, lines, U0 = _case()
     beta, n = 1.0, m.nel * m.npnp
-    ug, itg = _gpu(torch_cuda, m, nz, lam2, beta, F, lines, U0, 1e-8, 500)
+    ug, itg = _gpu(torch_cuda, m, nz, lam2, beta, F, lines, U0, tol, 500)
     om = OracleMesh(m)
     its = []
+    bar = 10 * tol if tol > 1e-10 else 1e-10
     for k in range(nz):
         if k == 1:
             continue
         S = OracleMsys(om, lam2, (beta * (k >> 1)) ** 2)
-        uo, it = S.solve_plane_pcg(F[k, :n], lines[k], U0[k, :n], 1e-8, 500)
+        uo, it = S.solve_plane_pcg(F[k, :n], lines[k], U0[k, :n], tol, 500)
         its.append(it)
         rel = np.linalg.norm(ug[k, :n] - uo) / np.linalg.norm(uo)
-        assert rel < 1e-9, (k, rel)
-    assert abs(itg - max(its)) <= 1, (itg, its)
+        assert rel < bar, (k, rel, itg, it)
+    assert abs(itg - max(its)) <= 2, (itg, its)
     if lam2 > 1e5:
         assert max(its) < 20
 

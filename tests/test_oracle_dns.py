@@ -26,12 +26,14 @@ GOLD = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "regress
 NOISE = 1e-12
 
 
-def taylor_run(case, advection="alt", steps=20):
+def taylor_run(case, advection="alt", steps=20, startup="direct"):
     from oracle.dns_oracle import DnsOracle
     nz = 1 if case == "taylor2" else 8
     m = rect_mesh(2, 2, 11, 0.0, 2.0, 0.0, 2.0, periodic_x=True, periodic_y=True)
     Lz, nu, dt = 2.0, 0.01, 0.02
-    D = DnsOracle(m, m, nz, 2 * np.pi / Lz, nu, dt, advection=advection)
+    # the sessions set TOL_REL = 1e-12 (mesh/taylor2..taylor5)
+    D = DnsOracle(m, m, nz, 2 * np.pi / Lz, nu, dt, advection=advection, startup=startup,
+                  tol_rel=1e-12)
     z = np.arange(nz) * Lz / nz
     X = np.broadcast_to(m.x.ravel(), (nz, m.x.size))
     Y = np.broadcast_to(m.y.ravel(), (nz, m.x.size))
@@ -54,9 +56,13 @@ def taylor_run(case, advection="alt", steps=20):
             for i, c in enumerate("uvw"[:D.ncom])}
 
 
+@pytest.mark.parametrize("startup", ["direct", "pcg"])
 @pytest.mark.parametrize("case", ["taylor2", "taylor3", "taylor4", "taylor5"])
-def test_dns_oracle_reproduces_reference_regress_goldens(oracle, case):
-    errs = taylor_run(case)
+def test_dns_oracle_reproduces_reference_regress_goldens(oracle, case, startup):
+    """Both start-ups (the reference's JACPCG at the sessions' TOL_REL =
+    1e-12, and the direct first-order solve) reproduce the printed digits;
+    "noise-level" is utility/compare.cpp's NOISE = 1e-12."""
+    errs = taylor_run(case, startup=startup)
     for comp, want in GOLD[case].items():
         if want == "noise-level":
             assert errs[comp] < NOISE, (comp, errs[comp])

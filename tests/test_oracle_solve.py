@@ -163,3 +163,19 @@ def test_fourier_modes_3d_laplace(oracle):
     assert np.max(np.abs(out[4] + base.ravel() / (2 * np.pi ** 2 + 4))) < 1e-10
     assert np.max(np.abs(out[7] + 2 * base.ravel() / (2 * np.pi ** 2 + 9))) < 1e-10
     assert np.max(np.abs(out[[0, 2, 3, 5, 6]])) < 1e-14
+
+
+@pytest.mark.parametrize("np_,tol", [(8, 2e-9), (10, 5e-12)])
+def test_cylindrical_axisymmetric_manufactured(np_, tol):
+    """Cylindrical mode 0 (Q1..Q4 carry r, the forcing weight Q8 does not:
+    src/element.cpp:800-806, the caller supplies r f): u = (1 - r^2) cos(pi x)
+    on [0,2] x [0,1], axis on r = 0 (natural), u = 0 on r = 1, periodic x;
+    converges spectrally to the exact solution."""
+    from oracle.oracle import OracleMesh, OracleMsys
+    from semtex_amd.mesh import rect_mesh
+    m = rect_mesh(4, 3, np_, 0.0, 2.0, 0.0, 1.0, bc=("A", "N", "D", "N"), periodic_x=True, cyl=1)
+    lam2 = 2.0
+    u = (1 - m.y ** 2) * np.cos(np.pi * m.x)
+    f = -np.pi ** 2 * u - 4 * np.cos(np.pi * m.x) - lam2 * u
+    uo = OracleMsys(OracleMesh(m), lam2, 0.0).solve_plane((m.y * f).ravel())
+    assert np.abs(uo - u.ravel()).max() < tol
