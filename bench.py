@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--no-dns", action="store_true")
     p.add_argument("--dns-only", action="store_true",
                    help="run only the DNS leg (for kernel profiles)")
+    p.add_argument("--no-batch", action="store_true")
+    p.add_argument("--batch-runs", type=int, default=20)
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--mixed-tol", type=float, default=1e-11,
                    help="relative correction size that ends the mixed refinement")
@@ -193,6 +195,52 @@ def dns_leg(a, torch):
     ok = all(bool(torch.isfinite(D.field(i)).all()) for i in range(4))
     return {"P": P, "D": D, "u0": u0, "ms_per_step": ms / max(a.dns_steps, 1),
             "value": a.dns_steps / (ms / 1e3), "setup_s": setup_s, "finite": ok}
+
+
+def batch_leg(a, torch):
+    """Uneven batch: 64 independent 2D problems (N_P 4..12, 4x4..20x20
+    elements, aspect 1..4, lambda2 1e-2..1e2, half cylindrical), each solved
+    once per run by one captured graph; FP64 and mixed (3 passes)."""
+    from semtex_amd.problems import uneven_batch
+    from semtex_amd.solver import SX_FP64, SX_MIXED, Batch, Field, GpuMesh, ModalMatrixSys
+    probs = uneven_batch()
+    t0 = time.perf_counter()
+    res = {}
+    sols = {}
+    for name, prec in (("fp64", SX_FP64), ("mixed", SX_MIXED)):
+        items = []
+        for p in probs:
+            m = p["mesh"]
+            gm = GpuMesh(m)
+            M = ModalMatrixSys(p["lambda2"], 0.0, 0, 1, gm, precision=prec)
+            u = Field(gm, 1)
+            src = torch.zeros((1, m.plane_size), dtype=torch.float64, device="cuda")
+            src[0, :m.nel * m.npnp] = torch.from_numpy(p["forcing"][0])
+            items.append((M, u, src, gm))
+        B = Batch([(M, u, src) for (M, u, src, _) in items], refine_passes=3)
+        torch.cuda.synchronize()
+        B.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.batch_runs):
+            B.run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / a.batch_runs
+        sols[name] = [u.data[0, :p["mesh"].nel * p["mesh"].npnp].clone()
+                      for (p, (_, u, _, _)) in zip(probs, items)]
+        res[name] = {"ms_per_run": round(ms, 4), "problems_per_s": round(len(probs) / (ms / 1e3), 1)}
+        del B, items
+    err = max(float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
+              for x, y in zip(sols["mixed"], sols["fp64"]))
+    res["mixed"]["max_rel_err_vs_fp64"] = err
+    res["setup_s"] = round(time.perf_counter() - t0, 2)
+    res["problems"] = len(probs)
+    res["config"] = ("64 independent 2D Helmholtz problems, N_P 4..12, 4x4..20x20 elements, "
+                     "aspect 1..4, lambda2 log-uniform 1e-2..1e2, N(0,1) forcing, half Cartesian "
+                     "(Dirichlet), half cylindrical axisymmetric (axis y=0); one HIP graph per run")
+    return res
 
 
 def main():
@@ -345,6 +393,11 @@ def main():
         out["dns"] = dns
         print(f"# DNS: {dn['value']:.2f} steps/s ({dn['ms_per_step']:.3f} ms/step)",
               file=sys.stderr, flush=True)
+
+    if not a.no_batch:
+        out["batch"] = batch_leg(a, torch)
+        print(f"# batch: {out['batch']['fp64']['problems_per_s']} problems/s FP64, "
+              f"{out['batch']['mixed']['problems_per_s']} mixed", file=sys.stderr, flush=True)
 
     if not a.no_cpu_baseline:
         cb = cpu_baseline(P, a.cpu_seconds)

@@ -343,6 +343,15 @@ __device__ __forceinline__ void lds_sync ()
   asm volatile ("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
+// the value of lane l (wave-uniform l) in every lane
+__device__ __forceinline__ double lane_bcast (double v, int l)
+{
+  const long long x = __double_as_longlong (v);
+  const int lo = __builtin_amdgcn_readlane ((int) (x & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane ((int) (x >> 32), l);
+  return __longlong_as_double (((long long) hi << 32) | (unsigned) lo);
+}
+
 __device__ __forceinline__ size_t pidx (int t, int c)
 {
   return ((size_t) (t >> 1) * PB + c) * 2 + (t & 1);
@@ -412,22 +421,26 @@ struct PanelSweep {
 template <int NR, int D>
 __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
-    const int* __restrict__ nsolve, double* rhs, int nmax)
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G)
 {
   constexpr int NW = PanelSweep<NR, D>::NW, TRW = PanelSweep<NR, D>::TRW;
   constexpr int HB = PanelSweep<NR, D>::HB, H = PanelSweep<NR, D>::H;
   extern __shared__ double sm[];
-  double* ybuf = sm;                                // [64][NR] y of the last block
-  double* rbuf = ybuf + PB * NR;                    // [2][D][64][NR] pending r, by half
+  double* rbuf = sm;                                // [2][D][64][NR] pending r, by half
   double* part = rbuf + 2 * D * PB * NR;            // [NW][64][NR] triangle partials
-  const int mode = blockIdx.x, n = nsolve[mode];
+  // G workgroups share a mode, NR of its G*NR right-hand sides each; they
+  // get block indices congruent mod 8, i.e. the same XCD and its L2
+  const int b = blockIdx.x;
+  const int mode = (b / (8 * G)) * 8 + b % 8, grp = (b / 8) % G;
+  if (mode >= nmodes) return;
+  const int n = nsolve[mode];
   if (n == 0) return;
   const int nb = (n + PB - 1) / PB;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   const int slot = w / H, half = w % H;             // pending block J+slot, rows half*HB..
   const bool owner = slot < D;
-  double* bg = rhs + (size_t) mode * NR * nmax;
+  double* bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
   const size_t PS = (size_t) (D + 1) * PB * PB;     // doubles per panel
   double u[HB], tr[TRW], bp[NR];
 
@@ -472,7 +485,7 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         // reload behind the use (offset tied to the sums, as for the blocks)
         int off = (int) ((const char*) trow (Jn, k) - (const char*) P);
 #pragma unroll
-        for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(pp[r]));
+        for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(pp[r]));
         const double2 v = *(const double2*) ((const char*) P + off);
         tr[k] = v.x;
         tr[k + 1] = v.y;
@@ -480,17 +493,19 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
 #pragma unroll
       for (int r = 0; r < NR; r++) part[(w * PB + lane) * NR + r] = pp[r];
     };
+    // every wave sums the triangle partials itself: lane c holds y_J[c],
+    // which the update reads back by lane broadcast (v_readlane), so no
+    // barrier and no LDS round trip is needed; wave 0 writes y_J out
+    double yreg[NR];
     auto finish = [&] (int J) __attribute__ ((always_inline)) {
-      if (w == 0) {
-        const int j = J * PB + lane;
+      const int j = J * PB + lane;
 #pragma unroll
-        for (int r = 0; r < NR; r++) {
-          double y = 0.0;
+      for (int r = 0; r < NR; r++) {
+        double y = 0.0;
 #pragma unroll
-          for (int g = 0; g < NW; g++) y += part[(g * PB + lane) * NR + r];
-          ybuf[lane * NR + r] = y;
-          if (j < n) bg[(size_t) r * nmax + gidx (j)] = y;
-        }
+        for (int g = 0; g < NW; g++) y += part[(g * PB + lane) * NR + r];
+        yreg[r] = y;
+        if (w == 0 && j < n) bg[(size_t) r * nmax + gidx (j)] = y;
       }
     };
 
@@ -530,13 +545,12 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
       tri (J);
       lds_sync ();
       finish (J);
-      lds_sync ();
       // U_{J+1,J+2+slot} is loaded pair by pair right behind the use: the
       // load offset is tied to the running sums, so the loads cannot be
       // hoisted above the FMAs (which would need a second register buffer)
       {
         const int K = J + 1 + slot;
-        const double* yb = ybuf + half * HB * NR;
+        const int row0 = half * HB;                  // this wave's rows of y_J
         double acc0[NR], acc1[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) acc0[r] = acc1[r] = 0.0;
@@ -546,12 +560,12 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
           const double u0 = u[t], u1 = u[t + 1];
 #pragma unroll
           for (int r = 0; r < NR; r++) {
-            acc0[r] += u0 * yb[t * NR + r];
-            acc1[r] += u1 * yb[(t + 1) * NR + r];
+            acc0[r] += u0 * lane_bcast (yreg[r], row0 + t);
+            acc1[r] += u1 * lane_bcast (yreg[r], row0 + t + 1);
           }
           int off = t * PB * (int) sizeof (double);
 #pragma unroll
-          for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(acc0[r]), "v"(acc1[r]));
+          for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc0[r]), "v"(acc1[r]));
           const double2 v = *(const double2*) ((const char*) pn + off);
           u[t] = v.x;
           u[t + 1] = v.y;
@@ -561,7 +575,7 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         for (int r = 0; r < NR; r++) {
           acc[r] = acc0[r] + acc1[r];
           // bp is read only after the sums (its wait must not move up)
-          asm volatile ("" : "+v"(bp[r]) : "v"(acc[r]));
+          asm ("" : "+v"(bp[r]) : "v"(acc[r]));
         }
         if (owner) {
           double* rp = rbuf + ((size_t) (half * D + K % D) * PB + lane) * NR;
@@ -578,7 +592,7 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         {
           int off = 0;
 #pragma unroll
-          for (int r = 0; r < NR; r++) asm volatile ("" : "+v"(off) : "v"(acc[r]), "v"(bp[r]));
+          for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[r]), "v"(bp[r]));
           const int gi = gidx ((K + 1) * PB + lane) + off;
 #pragma unroll
           for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * nmax + gi];
@@ -598,32 +612,34 @@ int panel_D (int nband)
   return std::max (1, (kd + PB - 1) / PB);
 }
 
+// NR right-hand sides per workgroup, G workgroups per mode
 template <int NR, int D>
-int launch_panel_d (const sx_msys* s, hipStream_t st)
+int launch_panel_d (const sx_msys* s, int G, hipStream_t st)
 {
   constexpr int NW = PanelSweep<NR, D>::NW;
-  const size_t lds = sizeof (double) * ((size_t) PB * NR * (1 + 2 * D + NW));
+  const size_t lds = sizeof (double) * ((size_t) PB * NR * (2 * D + NW));
   auto kern = k_panel_sweeps<NR, D>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  hipLaunchKernelGGL (kern, dim3 (s->nmodes), dim3 (NW * 64), lds, st, s->d_swF, s->d_swR,
-                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax);
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G;
+  hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (NW * 64), lds, st, s->d_swF, s->d_swR,
+                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
 
 template <int NR>
-int launch_panel (const sx_msys* s, hipStream_t st)
+int launch_panel (const sx_msys* s, int G, hipStream_t st)
 {
   switch (s->sw_D) {
-  case 1: return launch_panel_d<NR, 1> (s, st);
-  case 2: return launch_panel_d<NR, 2> (s, st);
-  case 3: return launch_panel_d<NR, 3> (s, st);
-  case 4: return launch_panel_d<NR, 4> (s, st);
-  case 5: return launch_panel_d<NR, 5> (s, st);
-  case 6: return launch_panel_d<NR, 6> (s, st);
-  case 7: return launch_panel_d<NR, 7> (s, st);
-  case 8: return launch_panel_d<NR, 8> (s, st);
+  case 1: return launch_panel_d<NR, 1> (s, G, st);
+  case 2: return launch_panel_d<NR, 2> (s, G, st);
+  case 3: return launch_panel_d<NR, 3> (s, G, st);
+  case 4: return launch_panel_d<NR, 4> (s, G, st);
+  case 5: return launch_panel_d<NR, 5> (s, G, st);
+  case 6: return launch_panel_d<NR, 6> (s, G, st);
+  case 7: return launch_panel_d<NR, 7> (s, G, st);
+  case 8: return launch_panel_d<NR, 8> (s, G, st);
   default: break;
   }
   set_error ("panel sweep: unsupported block count");
@@ -750,10 +766,18 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
 int band_solve (const sx_msys* s, int nr, hipStream_t st)
 {
   if (s->sw_panel) {
-    switch (nr) {
-    case 2: return launch_panel<2> (s, st);
-    case 4: return launch_panel<4> (s, st);
-    case 6: return launch_panel<6> (s, st);
+    // right-hand sides per workgroup: SX_PANEL_NR in the environment, else
+    // 1 while the modes fill fewer than half of the 256 CUs, else 2
+    int per = 2;
+    if (const char* e = std::getenv ("SX_PANEL_NR")) per = std::atoi (e);
+    else if (s->nmodes * nr <= 128) per = 1;
+    if (per < 1 || nr % per) per = nr;
+    const int G = nr / per;
+    switch (per) {
+    case 1: return launch_panel<1> (s, G, st);
+    case 2: return launch_panel<2> (s, G, st);
+    case 4: return launch_panel<4> (s, G, st);
+    case 6: return launch_panel<6> (s, G, st);
     default: break;
     }
     set_error ("band solve: 2, 4 or 6 right-hand sides per mode");
