@@ -89,6 +89,11 @@ int sx_msys_create (sx_mesh_t m, double lambda2, double beta, int base_mode,
 int sx_msys_destroy (sx_msys_t s);
 /* Unknowns solved for mode index `mode` (0-based within the system). */
 int sx_msys_nsolve (sx_msys_t s, int mode);
+/* Layout of the factor's solve copies (diagnostic; no reference
+ * counterpart): info[0] 1 = 64x64 panel sweeps, 0 = offset-major blocks;
+ * info[1] off-diagonal panel blocks D; info[2] 1 = twisted two-piece solve;
+ * info[3] band width nband.  Fills min(n, 4) entries. */
+int sx_msys_layout (sx_msys_t s, int* info, int n);
 
 /* Field::solve (src/field.cpp:598-709, DIRECT branch) over nz planes.
  * d_f:     forcing planes [nz][plane_stride] (device; DESTROYED, as in the

@@ -28,6 +28,7 @@ SIGNATURES = {
     "sx_msys_create": (_i, [_vp, _d, _d, _i, _i, _i, _vp, C.POINTER(_vp)]),
     "sx_msys_destroy": (_i, [_vp]),
     "sx_msys_nsolve": (_i, [_vp, _i]),
+    "sx_msys_layout": (_i, [_vp, C.POINTER(C.c_int), _i]),
     "sx_field_solve": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
     "sx_field_solve_mixed": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i,
                                   C.POINTER(_i), _vp]),

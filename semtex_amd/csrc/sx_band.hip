@@ -37,17 +37,20 @@ namespace {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void k_band_factor (
     double* band, size_t band_stride, int ld, const int* __restrict__ nsolve,
-    int* status)
+    int* status, const int* __restrict__ nelim)
 {
   extern __shared__ double row[];
   __shared__ double s_dinv;
   __shared__ int s_bad;
+  // nelim (optional): eliminate only the first nelim[mode] columns, leaving
+  // the Schur complement of the rest in the trailing band
   const int mode = blockIdx.x, n = nsolve[mode], kd = ld - 1;
+  const int ne = nelim ? nelim[mode] : n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6,
             nw = blockDim.x >> 6;
   double* A = band + (size_t) mode * band_stride;
   if (tid == 0) s_bad = 0;
-  for (int j = 0; j < n; j++) {
+  for (int j = 0; j < ne; j++) {
     if (tid == 0) {
       const double d = A[(size_t) j * ld + kd];
       if (!(d > 0.0)) { s_bad = j + 1; s_dinv = 0.0; }
@@ -418,34 +421,84 @@ struct PanelSweep {
   static constexpr int HB = PB / H;                          // block rows per wave
 };
 
+// Twisted solve.  The rows are split into T = [0, m), S = [m, m+64D) and
+// B = [m+64D, n) (|S| >= kd, so T and B do not couple).  B is eliminated
+// from its far end (the bottom piece: rows [m, n) reversed and padded in
+// front by p identity rows to whole blocks, factorised through B only),
+// and S's Schur complement A_SS - U_BS^T U_BS closes the top piece, rows
+// [0, m+64D), which is factorised completely.  Then
+//   forward   y_B = U_BB^-T b_B  and  top U^T y = (b_T, b_S - U_BS^T y_B),
+//   backward  top U x = y        and  x_B = U_BB^-1 (y_B - U_BS x_S),
+// so each sweep runs the two pieces side by side and meets at S: a chain
+// of about nb + 2D block steps instead of 2 nb.  A launch runs a segment
+// of blocks of each pass; a segment that stops early writes its pending
+// window r_{J1..J1+D-1} back into the right-hand side, where the next
+// segment reads it as its b.  Per-mode parameters, TW ints per mode:
+constexpr int TW = 8;
+enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
+
 template <int NR, int D>
 __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
-    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G)
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb)
 {
   constexpr int NW = PanelSweep<NR, D>::NW, TRW = PanelSweep<NR, D>::TRW;
   constexpr int HB = PanelSweep<NR, D>::HB, H = PanelSweep<NR, D>::H;
   extern __shared__ double sm[];
   double* rbuf = sm;                                // [2][D][64][NR] pending r, by half
   double* part = rbuf + 2 * D * PB * NR;            // [NW][64][NR] triangle partials
-  // G workgroups share a mode, NR of its G*NR right-hand sides each; they
-  // get block indices congruent mod 8, i.e. the same XCD and its L2
-  const int b = blockIdx.x;
+  // G workgroups share a mode (and piece), NR of its G*NR right-hand sides
+  // each; they get block indices congruent mod 8, i.e. the same XCD and L2
+  const int per = (nmodes + 7) / 8 * 8 * G;
+  const int piece = blockIdx.x / per, b = blockIdx.x % per;
   const int mode = (b / (8 * G)) * 8 + b % 8, grp = (b / 8) % G;
   if (mode >= nmodes) return;
-  const int n = nsolve[mode];
+  // this piece's size, panels, right-hand sides and block ranges per pass
+  // ([f0, f1) forward, [b0, b1) backward; wf/wb: hand the window on)
+  int n, f0 = 0, f1 = 0, b0 = 0, b1 = 0;
+  bool wf = false, wb = false;
+  size_t poff = 0;
+  double* bg;
+  int rs;
+  if (stage == 0) {
+    n = nsolve[mode];
+    f1 = b1 = (n + PB - 1) / PB;
+    bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
+    rs = nmax;
+  } else {
+    const int* t = tw + TW * mode;
+    const int nbT = t[TW_NBT], nbB = t[TW_NBB];
+    if (piece == 0) {
+      n = t[TW_NTOP];
+      bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
+      rs = nmax;
+      if (stage == 1) { f1 = nbT; wf = true; }
+      if (stage == 2) { f0 = nbT; f1 = nbT + D; b1 = D; wb = true; }
+      if (stage == 3) { b0 = D; b1 = nbT + D; }
+    } else {
+      n = t[TW_NBOT];
+      poff = boff;
+      bg = rhsb + ((size_t) mode * G + grp) * NR * nmaxb;
+      rs = nmaxb;
+      if (stage == 1) { f1 = nbB; wf = true; }
+      if (stage == 3) b1 = nbB + D;
+    }
+  }
   if (n == 0) return;
   const int nb = (n + PB - 1) / PB;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   const int slot = w / H, half = w % H;             // pending block J+slot, rows half*HB..
   const bool owner = slot < D;
-  double* bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
   const size_t PS = (size_t) (D + 1) * PB * PB;     // doubles per panel
   double u[HB], tr[TRW], bp[NR];
 
   for (int pass = 0; pass < 2; pass++) {
-    const double* P = (pass == 0 ? PF : PR) + (size_t) mode * pstride;
+    const int J0 = pass ? b0 : f0, J1 = pass ? b1 : f1;
+    const bool wout = pass ? wb : wf;
+    if (J0 >= J1) continue;
+    const double* P = (pass == 0 ? PF : PR) + (size_t) mode * pstride + poff;
     const double* zero = P + (size_t) nb * PS + (size_t) PB * PB;    // a zero block
     auto gidx = [&] (int j) __attribute__ ((always_inline)) {
       j = min (j, n - 1);
@@ -463,7 +516,7 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     };
     auto load_b = [&] (int K) __attribute__ ((always_inline)) {
 #pragma unroll
-      for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * nmax + gidx (K * PB + lane)];
+      for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * rs + gidx (K * PB + lane)];
     };
     // triangle partials of block J, then prefetch of T_{J+1}
     auto tri = [&] (int J) __attribute__ ((always_inline)) {
@@ -505,28 +558,30 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
 #pragma unroll
         for (int g = 0; g < NW; g++) y += part[(g * PB + lane) * NR + r];
         yreg[r] = y;
-        if (w == 0 && j < n) bg[(size_t) r * nmax + gidx (j)] = y;
+        if (w == 0 && j < n) bg[(size_t) r * rs + gidx (j)] = y;
       }
     };
 
-    // r of blocks 0..D-1 from b (pass 1: the forward result, reversed);
-    // the second half's partial sums start at zero
+    // r of blocks J0..J0+D-1 from b (pass 1 of a whole solve: the forward
+    // result, reversed); the second half's partial sums start at zero
     if (owner) {
-      const bool in = slot * PB + lane < n;
+      const int K = J0 + slot;
+      const bool in = K * PB + lane < n;
 #pragma unroll
       for (int r = 0; r < NR; r++)
-        rbuf[((size_t) (half * D + slot) * PB + lane) * NR + r] =
-            (in && half == 0) ? bg[(size_t) r * nmax + gidx (slot * PB + lane)] : 0.0;
+        rbuf[((size_t) (half * D + K % D) * PB + lane) * NR + r] =
+            (in && half == 0) ? bg[(size_t) r * rs + gidx (K * PB + lane)] : 0.0;
     }
-    // prefetch in the steady-state order: T_0 rows, U_{0,1+slot}, b of block D
+    // prefetch in the steady-state order: T_J0 rows, U_{J0,J0+1+slot}, b of
+    // block J0+D
 #pragma unroll
     for (int k = 0; k < TRW; k += 2) {
-      const double2 v = *(const double2*) trow (0, k);
+      const double2 v = *(const double2*) trow (J0, k);
       tr[k] = v.x;
       tr[k + 1] = v.y;
     }
     {
-      const double* pu = ublk (0);
+      const double* pu = ublk (J0);
 #pragma unroll
       for (int t = 0; t < HB; t += 2) {
         const double2 v = *(const double2*) (pu + (size_t) t * PB);
@@ -534,14 +589,14 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         u[t + 1] = v.y;
       }
     }
-    load_b (D);
+    load_b (J0 + D);
     // drain once per pass: the loop's wait counts then follow its own,
     // fixed load order
     __builtin_amdgcn_s_waitcnt (0);
     __syncthreads ();
 
     // step J: y_J from the triangle, then r_{J+1+slot} -= U_{J,J+1+slot}^T y_J
-    for (int J = 0; J < nb; J++) {
+    for (int J = J0; J < J1; J++) {
       tri (J);
       lds_sync ();
       finish (J);
@@ -595,15 +650,97 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
           for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[r]), "v"(bp[r]));
           const int gi = gidx ((K + 1) * PB + lane) + off;
 #pragma unroll
-          for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * nmax + gi];
+          for (int r = 0; r < NR; r++) bp[r] = bg[(size_t) r * rs + gi];
         }
       }
       lds_sync ();
     }
-    // the backward pass reads what the forward pass wrote to global memory
+    // a segment that stops early hands its pending window on through b
+    if (wout && owner && half == 0) {
+      const int K = J1 + slot;
+      if (K * PB + lane < n) {
+        const double* r0 = rbuf + ((size_t) (K % D) * PB + lane) * NR;
+        const double* r1 = r0 + (size_t) D * PB * NR;
+#pragma unroll
+        for (int r = 0; r < NR; r++) bg[(size_t) r * rs + gidx (K * PB + lane)] = r0[r] + r1[r];
+      }
+    }
+    // the next pass reads what this one wrote to global memory
     __threadfence ();
     __syncthreads ();
   }
+}
+
+// Moves between the right-hand side (rows in global order) and the bottom
+// piece's vectors (rows [m, n) reversed, p zero rows in front), per
+// (row, column, mode):
+//   0  gather   rb[q] = b[n-1-(q-p)] for B rows, zero for pad and S rows
+//   1  combine  b[m+a] += rb[nbot-1-a]    (S: the two pieces' residuals)
+//   2  x_S      rb[nbot-1-a] = b[m+a]
+//   3  scatter  b[n-1-(q-p)] = rb[q]      for B rows
+__global__ void k_twist_move (double* rhs, int nmax, double* rhsb, int nmaxb, int nr,
+                              const int* __restrict__ tw, int S, int what)
+{
+  const int mode = blockIdx.z, col = blockIdx.y;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int* t = tw + TW * mode;
+  const int p = t[TW_PAD], m = t[TW_M], nbot = t[TW_NBOT], n = t[TW_N];
+  const int nbr = nbot - p - S;                     // B rows
+  double* b = rhs + ((size_t) mode * nr + col) * nmax;
+  double* rb = rhsb + ((size_t) mode * nr + col) * nmaxb;
+  switch (what) {
+  case 0:
+    if (q < nbot) rb[q] = (q >= p && q < p + nbr) ? b[n - 1 - (q - p)] : 0.0;
+    break;
+  case 1:
+    if (q < S) b[m + q] += rb[nbot - 1 - q];
+    break;
+  case 2:
+    if (q < S) rb[nbot - 1 - q] = b[m + q];
+    break;
+  default:
+    if (q >= p && q < p + nbr) b[n - 1 - (q - p)] = rb[q];
+    break;
+  }
+}
+
+// The bottom piece of a twisted solve: rows [m, n) of the band reversed,
+// p identity rows in front, R(q, q') = A(n-1-(q-p), n-1-(q'-p)).
+__global__ void k_twist_rbuild (const double* __restrict__ A, size_t astride, double* R,
+                                size_t rstride, int ld, const int* __restrict__ tw)
+{
+  const int mode = blockIdx.y, kd = ld - 1;
+  const int* t = tw + TW * mode;
+  const int p = t[TW_PAD], nbot = t[TW_NBOT], n = t[TW_N];
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) nbot * ld) return;
+  const int q = (int) (idx / ld), o = (int) (idx % ld);
+  double v = 0.0;
+  if (q < p) v = (o == kd) ? 1.0 : 0.0;
+  else {
+    const int r = n - 1 - (q - p) + kd - o;        // global row of R(q-kd+o, q)
+    if (r < n && q - kd + o >= p) v = A[(size_t) mode * astride + (size_t) r * ld + o];
+  }
+  R[(size_t) mode * rstride + idx] = v;
+}
+
+// S's Schur complement, left in the bottom piece's trailing block by its
+// partial factorisation, into the top piece's band rows [m, m+S); the
+// bottom piece's S block becomes the identity (its backward sweep takes
+// x_S as given).  Thread (be, o) owns entry (m+al, m+be), al = be-kd+o.
+__global__ void k_twist_sback (double* A, size_t astride, double* R, size_t rstride,
+                               int ld, const int* __restrict__ tw, int S)
+{
+  const int mode = blockIdx.y, kd = ld - 1;
+  const int* t = tw + TW * mode;
+  const int m = t[TW_M], nbot = t[TW_NBOT];
+  const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t) S * ld) return;
+  const int be = (int) (idx / ld), o = (int) (idx % ld), al = be - kd + o;
+  if (al < 0) return;
+  double* rv = R + (size_t) mode * rstride + (size_t) (nbot - 1 - al) * ld + o;
+  A[(size_t) mode * astride + (size_t) (m + be) * ld + o] = *rv;
+  *rv = (o == kd) ? 1.0 : 0.0;
 }
 
 int panel_D (int nband)
@@ -614,32 +751,34 @@ int panel_D (int nband)
 
 // NR right-hand sides per workgroup, G workgroups per mode
 template <int NR, int D>
-int launch_panel_d (const sx_msys* s, int G, hipStream_t st)
+int launch_panel_d (const sx_msys* s, int G, int stage, hipStream_t st)
 {
   constexpr int NW = PanelSweep<NR, D>::NW;
   const size_t lds = sizeof (double) * ((size_t) PB * NR * (2 * D + NW));
   auto kern = k_panel_sweeps<NR, D>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  const int nblocks = (s->nmodes + 7) / 8 * 8 * G;
+  const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (NW * 64), lds, st, s->d_swF, s->d_swR,
-                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G);
+                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G,
+                      s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
 
 template <int NR>
-int launch_panel (const sx_msys* s, int G, hipStream_t st)
+int launch_panel (const sx_msys* s, int G, int stage, hipStream_t st)
 {
   switch (s->sw_D) {
-  case 1: return launch_panel_d<NR, 1> (s, G, st);
-  case 2: return launch_panel_d<NR, 2> (s, G, st);
-  case 3: return launch_panel_d<NR, 3> (s, G, st);
-  case 4: return launch_panel_d<NR, 4> (s, G, st);
-  case 5: return launch_panel_d<NR, 5> (s, G, st);
-  case 6: return launch_panel_d<NR, 6> (s, G, st);
-  case 7: return launch_panel_d<NR, 7> (s, G, st);
-  case 8: return launch_panel_d<NR, 8> (s, G, st);
+  case 1: return launch_panel_d<NR, 1> (s, G, stage, st);
+  case 2: return launch_panel_d<NR, 2> (s, G, stage, st);
+  case 3: return launch_panel_d<NR, 3> (s, G, stage, st);
+  case 4: return launch_panel_d<NR, 4> (s, G, stage, st);
+  case 5: return launch_panel_d<NR, 5> (s, G, stage, st);
+  case 6: return launch_panel_d<NR, 6> (s, G, stage, st);
+  case 7: return launch_panel_d<NR, 7> (s, G, stage, st);
+  case 8: return launch_panel_d<NR, 8> (s, G, stage, st);
   default: break;
   }
   set_error ("panel sweep: unsupported block count");
@@ -698,14 +837,141 @@ size_t band_sweep_bytes_per_mode (int nmax, int nband)
   return (size_t) ((nmax + B - 1) / B) * nband * B;
 }
 
+// Split of every mode for the twisted solve (the TW ints per mode), or
+// false when some mode is too small for it to shorten the chain of block
+// steps by a quarter.  A sweep step is bound by how many loads one CU
+// keeps in flight, so the second piece pays off while the modes leave CUs
+// idle: systems of at most 32 modes.  SX_BAND_TWIST=0/1 in the environment
+// forces it off/on.
+static bool twist_split (const sx_msys* s, std::vector<int>& tw)
+{
+  if (const char* e = std::getenv ("SX_BAND_TWIST")) {
+    if (std::atoi (e) == 0) return false;
+  } else if (s->nmodes > 32) {
+    return false;
+  }
+  const int D = panel_D (s->m->nband), S = PB * D;
+  tw.assign ((size_t) TW * s->nmodes, 0);
+  for (int md = 0; md < s->nmodes; md++) {
+    const int n = s->nsolve[md], nbtot = (n + PB - 1) / PB;
+    int best = -1, bestT = 0;
+    for (int nbT = 1; nbT < nbtot; nbT++) {
+      const int nbr = n - nbT * PB - S;
+      if (nbr < 1) break;
+      const int nbB = (nbr + PB - 1) / PB;
+      // stage 1 (both forward), stage 2 (S both ways), stage 3 (both backward)
+      const int cost = std::max (nbT, nbB) + 2 * D + std::max (nbT, nbB + D);
+      if (best < 0 || cost < best) { best = cost; bestT = nbT; }
+    }
+    if (best < 0 || 4 * best > 3 * 2 * nbtot) return false;
+    const int m = bestT * PB, nbr = n - m - S, nbB = (nbr + PB - 1) / PB;
+    int* t = &tw[(size_t) TW * md];
+    t[TW_NBT] = bestT;
+    t[TW_NBB] = nbB;
+    t[TW_PAD] = nbB * PB - nbr;
+    t[TW_M] = m;
+    t[TW_NTOP] = m + S;
+    t[TW_NBOT] = (nbB + D) * PB;
+    t[TW_N] = n;
+  }
+  return true;
+}
+
+// Panels of one piece: T_J and the off-diagonal blocks of the factor U
+// (band `src`, per-mode sizes `d_n`) and of its reversal, into P + off.
+static int build_panels (sx_msys* s, const double* src, double* V, size_t stride,
+                         const int* d_n, int nbp, size_t off, hipStream_t st)
+{
+  const int ld = s->m->nband, D = s->sw_D;
+  hipLaunchKernelGGL (k_band_reverse, dim3 ((unsigned) ((stride + 255) / 256), s->nmodes),
+                      dim3 (256), 0, st, src, V, stride, ld, d_n);
+  const double* from[2] = {src, V};
+  double* to[2] = {s->d_swF + off, s->d_swR + off};
+  for (int k = 0; k < 2; k++) {
+    hipLaunchKernelGGL (k_panel_tinv, dim3 (nbp, s->nmodes), dim3 (64), 0, st, from[k],
+                        stride, ld, d_n, to[k], s->sw_stride, D);
+    const size_t tot = (size_t) nbp * D * PB * PB;
+    hipLaunchKernelGGL (k_panel_offdiag, dim3 ((unsigned) ((tot + 255) / 256), s->nmodes),
+                        dim3 (256), 0, st, from[k], stride, ld, d_n, to[k], s->sw_stride, D,
+                        nbp);
+  }
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// Twisted factorisation: the bottom piece built and factorised through B,
+// S's Schur complement moved into the top piece, the top piece factorised,
+// then the panels of both pieces (top at offset 0, bottom at tw_boff).
+static int band_factor_twisted (sx_msys* s, const std::vector<int>& tw, hipStream_t st,
+                                int* d_status)
+{
+  const int ld = s->m->nband, D = panel_D (ld), S = PB * D, nm = s->nmodes;
+  std::vector<int> aux (3 * nm);
+  int nbtm = 0, nbbm = 0;
+  for (int md = 0; md < nm; md++) {
+    const int* t = &tw[(size_t) TW * md];
+    aux[md] = t[TW_NTOP];
+    aux[nm + md] = t[TW_NBOT];
+    aux[2 * nm + md] = t[TW_NBOT] - S;
+    nbtm = std::max (nbtm, t[TW_NTOP] / PB);
+    nbbm = std::max (nbbm, t[TW_NBOT] / PB);
+  }
+  SX_HIP (hipMalloc (&s->d_tw, sizeof (int) * tw.size ()));
+  SX_HIP (hipMemcpy (s->d_tw, tw.data (), sizeof (int) * tw.size (), hipMemcpyHostToDevice));
+  int* d_aux = nullptr;
+  SX_HIP (hipMalloc (&d_aux, sizeof (int) * aux.size ()));
+  SX_HIP (hipMemcpy (d_aux, aux.data (), sizeof (int) * aux.size (), hipMemcpyHostToDevice));
+  const int *d_ntop = d_aux, *d_nbot = d_aux + nm, *d_nelim = d_aux + 2 * nm;
+  const size_t rstride = (size_t) nbbm * PB * ld;
+  double *R = nullptr, *V = nullptr;
+  const size_t vstride = std::max (rstride, s->band_stride);
+  SX_HIP (hipMalloc (&R, sizeof (double) * nm * rstride));
+  SX_HIP (hipMalloc (&V, sizeof (double) * nm * vstride));
+  SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * nm * vstride, st));
+  const size_t lds = sizeof (double) * std::max (ld, 1);
+  hipLaunchKernelGGL (k_twist_rbuild, dim3 ((unsigned) ((rstride + 255) / 256), nm),
+                      dim3 (256), 0, st, s->d_bandU, s->band_stride, R, rstride, ld, s->d_tw);
+  hipLaunchKernelGGL (k_band_factor, dim3 (nm), dim3 (1024), lds, st, R, rstride, ld, d_nbot,
+                      d_status, d_nelim);
+  hipLaunchKernelGGL (k_twist_sback, dim3 ((unsigned) (((size_t) S * ld + 255) / 256), nm),
+                      dim3 (256), 0, st, s->d_bandU, s->band_stride, R, rstride, ld, s->d_tw, S);
+  hipLaunchKernelGGL (k_band_factor, dim3 (nm), dim3 (1024), lds, st, s->d_bandU,
+                      s->band_stride, ld, d_ntop, d_status, nullptr);
+  SX_HIP (hipGetLastError ());
+  s->sw_panel = 1;
+  s->sw_D = D;
+  s->sw_block = PB;
+  s->tw = 1;
+  s->tw_S = S;
+  s->tw_boff = (size_t) (nbtm + 1) * (D + 1) * PB * PB;
+  s->sw_stride = s->tw_boff + (size_t) (nbbm + 1) * (D + 1) * PB * PB;
+  SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * nm * s->sw_stride));
+  SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * nm * s->sw_stride));
+  if (int rc = build_panels (s, s->d_bandU, V, s->band_stride, d_ntop, nbtm + 1, 0, st))
+    return rc;
+  SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * nm * vstride, st));
+  if (int rc = build_panels (s, R, V, rstride, d_nbot, nbbm + 1, s->tw_boff, st)) return rc;
+  s->nmaxb = nbbm * PB;
+  const size_t nb_rhs = (size_t) nm * s->rhs_cap * s->nmaxb;
+  SX_HIP (hipMalloc (&s->d_rhsb, sizeof (double) * nb_rhs));
+  SX_HIP (hipMemsetAsync (s->d_rhsb, 0, sizeof (double) * nb_rhs, st));
+  SX_HIP (hipStreamSynchronize (st));
+  (void) hipFree (R);
+  (void) hipFree (V);
+  (void) hipFree (d_aux);
+  return SX_OK;
+}
+
 int band_factor (sx_msys* s, hipStream_t st, int* d_status)
 {
   const int ld = s->m->nband;
+  const bool panels = use_panels (ld);
+  std::vector<int> tw;
+  if (panels && twist_split (s, tw)) return band_factor_twisted (s, tw, st, d_status);
   hipLaunchKernelGGL (k_band_factor, dim3 (s->nmodes), dim3 (1024),
                       sizeof (double) * std::max (ld, 1), st, s->d_bandU,
-                      s->band_stride, ld, s->d_nsolve, d_status);
+                      s->band_stride, ld, s->d_nsolve, d_status, nullptr);
   SX_HIP (hipGetLastError ());
-  const bool panels = use_panels (ld);
   const int B = panels ? PB : band_block (ld);
   const int nblk = (s->nmax + B - 1) / B;
   double* V = nullptr;
@@ -720,7 +986,7 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
     s->sw_D = D;
     s->sw_block = PB;
     s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
-    SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
+      SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
     SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * s->nmodes * s->sw_stride));
     const double* src[2] = {s->d_bandU, V};
     double* dst[2] = {s->d_swF, s->d_swR};
@@ -767,21 +1033,43 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
 {
   if (s->sw_panel) {
     // right-hand sides per workgroup: SX_PANEL_NR in the environment, else
-    // 1 while the modes fill fewer than half of the 256 CUs, else 2
-    int per = 2;
+    // the fewest that keep one workgroup per CU (256 CUs), counting both
+    // pieces of a twisted solve
+    const int pieces = s->tw ? 2 : 1, slots = (s->nmodes + 7) / 8 * 8 * pieces;
+    int per = nr;
     if (const char* e = std::getenv ("SX_PANEL_NR")) per = std::atoi (e);
-    else if (s->nmodes * nr <= 128) per = 1;
+    else
+      for (int c : {1, 2, 3, 6})
+        if (nr % c == 0 && slots * (nr / c) <= 256) { per = c; break; }
     if (per < 1 || nr % per) per = nr;
     const int G = nr / per;
-    switch (per) {
-    case 1: return launch_panel<1> (s, G, st);
-    case 2: return launch_panel<2> (s, G, st);
-    case 4: return launch_panel<4> (s, G, st);
-    case 6: return launch_panel<6> (s, G, st);
-    default: break;
-    }
-    set_error ("band solve: 2, 4 or 6 right-hand sides per mode");
-    return SX_ERR_ARG;
+    auto sweep = [&] (int stage) {
+      switch (per) {
+      case 1: return launch_panel<1> (s, G, stage, st);
+      case 2: return launch_panel<2> (s, G, stage, st);
+      case 3: return launch_panel<3> (s, G, stage, st);
+      case 4: return launch_panel<4> (s, G, stage, st);
+      case 6: return launch_panel<6> (s, G, stage, st);
+      default: break;
+      }
+      set_error ("band solve: 1, 2, 3, 4 or 6 right-hand sides per workgroup");
+      return (int) SX_ERR_ARG;
+    };
+    if (!s->tw) return sweep (0);
+    // twisted: gather B, both forward, S, both backward, scatter B
+    auto move = [&] (int what) {
+      const int rows = (what == 1 || what == 2) ? s->tw_S : s->nmaxb;
+      hipLaunchKernelGGL (k_twist_move, dim3 ((rows + 255) / 256, nr, s->nmodes), dim3 (256),
+                          0, st, s->d_rhs, s->nmax, s->d_rhsb, s->nmaxb, nr, s->d_tw,
+                          s->tw_S, what);
+      const hipError_t e = hipGetLastError ();
+      return e == hipSuccess ? (int) SX_OK : hip_fail (e, "k_twist_move");
+    };
+    int rc;
+    if ((rc = move (0)) || (rc = sweep (1)) || (rc = move (1)) || (rc = sweep (2)) ||
+        (rc = move (2)) || (rc = sweep (3)) || (rc = move (3)))
+      return rc;
+    return SX_OK;
   }
   if (nr != 2) {
     set_error ("band solve: the offset-major sweeps take 2 right-hand sides per mode");

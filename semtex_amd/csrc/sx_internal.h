@@ -125,6 +125,12 @@ struct sx_msys {
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks
+  int tw = 0;                  // 1: twisted two-piece panel solve (sx_band.hip)
+  int tw_S = 0;                //   separator rows, 64 sw_D
+  size_t tw_boff = 0;          //   doubles from a mode's top-piece panels to its bottom ones
+  int* d_tw = nullptr;         //   [nmodes][8] split of every mode
+  double* d_rhsb = nullptr;    //   [nmodes][nr][nmaxb] bottom-piece right-hand sides
+  int nmaxb = 0;
   double* d_rhs = nullptr;     // [nmodes][nr][nmax] solve workspace (nr <= rhs_cap)
   double* d_G = nullptr;       // [fields][2*nmodes][nglobal] essential values
   int rhs_cap = 2;             // right-hand sides per mode d_rhs / d_G can hold

@@ -626,7 +626,7 @@ extern "C" int sx_msys_destroy (sx_msys_t s)
   if (!s) return SX_OK;
   void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Bm32,
                   s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
-                  s->d_work, s->d_pc};
+                  s->d_work, s->d_pc, s->d_tw, s->d_rhsb};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete s;
   return SX_OK;
@@ -636,6 +636,17 @@ extern "C" int sx_msys_nsolve (sx_msys_t s, int mode)
 {
   if (!s || mode < 0 || mode >= s->nmodes) return -1;
   return s->nsolve[mode];
+}
+
+extern "C" int sx_msys_layout (sx_msys_t s, int* info, int n)
+{
+  if (!s || !info || n < 0) {
+    sx::set_error ("sx_msys_layout: null system or output");
+    return SX_ERR_ARG;
+  }
+  const int v[4] = {s->sw_panel, s->sw_D, s->tw, s->m->nband};
+  for (int i = 0; i < n && i < 4; i++) info[i] = v[i];
+  return SX_OK;
 }
 
 __global__ void k_to_f32 (const double* a, float* b, size_t n)
@@ -883,6 +894,14 @@ int msys_reserve_rhs (sx_msys* s, int nr)
     return SX_ERR_LIMIT;
   }
   double *r = nullptr, *g = nullptr;
+  if (s->tw) {
+    double* rb = nullptr;
+    const size_t nb = sizeof (double) * s->nmodes * nr * s->nmaxb;
+    SX_HIP (hipMalloc (&rb, nb));
+    SX_HIP (hipMemset (rb, 0, nb));
+    (void) hipFree (s->d_rhsb);
+    s->d_rhsb = rb;
+  }
   SX_HIP (hipMalloc (&r, sizeof (double) * s->nmodes * nr * std::max (s->nmax, 1)));
   SX_HIP (hipMalloc (&g, sizeof (double) * (nr / 2) * 2 * s->nmodes * s->m->nglobal));
   SX_HIP (hipMemset (r, 0, sizeof (double) * s->nmodes * nr * std::max (s->nmax, 1)));

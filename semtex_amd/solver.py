@@ -82,6 +82,12 @@ class ModalMatrixSys:
     def nsolve(self, mode):
         return _lib.load().sx_msys_nsolve(self.handle, mode)
 
+    def layout(self):
+        """{"panels", "D", "twisted", "nband"} of the factor's solve copies."""
+        info = (C.c_int * 4)()
+        _lib.check(_lib.load().sx_msys_layout(self.handle, info, 4), "sx_msys_layout")
+        return {"panels": info[0], "D": info[1], "twisted": info[2], "nband": info[3]}
+
     def __len__(self):
         return self.num_modes
 
