@@ -58,7 +58,35 @@ def parse():
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--mixed-tol", type=float, default=1e-11,
                    help="relative correction size that ends the mixed refinement")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only copy 256 MiB device to device 3 times (the reference "
+                        "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return p.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "band_sweep_hbm_traffic.json")
+
+
+def measured_traffic(workload):
+    """HBM bytes per band-solve launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or
+    None when there is none for it."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes_per_launch") if t.get("workload") == workload else None
+
+
+def copy_calibration(torch):
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
 
 
 def cpu_baseline(P, seconds):
@@ -250,6 +278,9 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.copy_calibration:
+        copy_calibration(torch)
+        return
     from semtex_amd import _lib
     from semtex_amd.problems import helmholtz_fourier
     from semtex_amd.solver import SX_FP64, SX_MIXED, GpuMesh
@@ -320,6 +351,9 @@ def main():
     band_ms = phases["band_solve"]
     achieved = band_bytes / (band_ms * 1e-3) / 1e9
 
+    workload = ("batched 2.5D Helmholtz, 16x16 elements on [0,2pi]x[-1,1], N_P=10, "
+                "N_Z=64 (32 modes, beta=1, lambda2=1+k^2), Dirichlet y=+-1, periodic x, "
+                "seeded Gaussian forcing ~(1+k)^-2; static condensation + banded Cholesky")
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -334,10 +368,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": "batched 2.5D Helmholtz, 16x16 elements on [0,2pi]x[-1,1], N_P=10, "
-                        "N_Z=64 (32 modes, beta=1, lambda2=1+k^2), Dirichlet y=+-1, periodic x, "
-                        "seeded Gaussian forcing ~(1+k)^-2; static condensation + "
-                        "banded Cholesky",
+            "workload": workload,
             "precision": chosen,
             "mode_solves_per_step": nmodes,
             "nglobal": m.nglobal, "nsolve": m.nsolve, "nband": m.nband,
@@ -351,7 +382,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(workload),
             "algorithmic_bytes": band_bytes,
             "avg_ms": round(band_ms, 5),
         },
