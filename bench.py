@@ -56,6 +56,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true")
     p.add_argument("--batch-runs", type=int, default=20)
     p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--no-adjoint", action="store_true")
     p.add_argument("--mixed-tol", type=float, default=1e-11,
                    help="relative correction size that ends the mixed refinement")
     p.add_argument("--copy-calibration", action="store_true",
@@ -271,6 +272,57 @@ def batch_leg(a, torch):
     return res
 
 
+def adjoint_leg(torch, P, gm, reps=3):
+    """Time of one gradient evaluation: (a) J = 1/2 ||u - u_t||_M^2 through
+    the batched Helmholtz solve w.r.t. the forcing and lambda2; (b) the same
+    J through 20 steps of the channel shrunk to 6x12 elements, N_P=6,
+    N_Z=32, w.r.t. the initial field, checkpointed every 5 steps.  Median
+    wall time of `reps` evaluations after one warm-up, synchronised."""
+    from semtex_amd.adjoint import DnsTape
+    from semtex_amd.problems import dns_channel
+    from semtex_amd.solver import helmholtz_solve, mass_weight
+
+    rng = np.random.default_rng(20261203)
+    F = torch.from_numpy(P["forcing"]).cuda()
+    Ut = torch.from_numpy(0.01 * rng.standard_normal(F.shape)).cuda()
+    W = mass_weight(gm, torch.ones_like(F), 1.0)       # Q8 = Q4 (Cartesian)
+
+    def grad_helmholtz():
+        f = F.clone().requires_grad_(True)
+        l2 = torch.tensor(P["lambda2"], dtype=torch.float64, requires_grad=True)
+        u = helmholtz_solve(f, l2, gm, P["beta"], P["nmodes"])
+        d = u - Ut
+        J = 0.5 * (d * d * W).sum()
+        J.backward()
+        torch.cuda.synchronize()
+
+    D = dns_channel(nx=6, ny=12, np_=6, nz=32)
+    T = DnsTape(D["mesh_u"], D["mesh_p"], D["nz"], D["beta"], D["kinvis"], D["dt"],
+                const_force=D["const_force"])
+    Udt = torch.from_numpy(0.1 * rng.standard_normal((3, D["nz"], T.n))).cuda()
+
+    def grad_dns():
+        x = torch.from_numpy(D["u0"]).cuda().requires_grad_(True)
+        st = T.run(x, 20, checkpoint=5)
+        J = 0.5 * T.mass_norm2(st[0] - Udt)
+        J.backward()
+        torch.cuda.synchronize()
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return round(1e3 * sorted(ts)[len(ts) // 2], 2)
+
+    return {"helmholtz_grad_ms": timed(grad_helmholtz),
+            "dns20_grad_ms": timed(grad_dns),
+            "config": "(a) 16x16 N_P=10 N_Z=64 batch, d/df and d/dlambda2; (b) channel 6x12 "
+                      "N_P=6 N_Z=32, 20 steps, d/du0, checkpoint every 5 steps"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -429,6 +481,10 @@ def main():
         out["batch"] = batch_leg(a, torch)
         print(f"# batch: {out['batch']['fp64']['problems_per_s']} problems/s FP64, "
               f"{out['batch']['mixed']['problems_per_s']} mixed", file=sys.stderr, flush=True)
+
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_leg(torch, P, gm)
+        print(f"# adjoint: {out['adjoint']}", file=sys.stderr, flush=True)
 
     if not a.no_cpu_baseline:
         cb = cpu_baseline(P, a.cpu_seconds)

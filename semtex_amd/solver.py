@@ -265,7 +265,15 @@ def _make_helmholtz_fn():
 
         @staticmethod
         def forward(ctx, f, lam2, gmesh, beta, nmodes, lines):
-            M = ModalMatrixSys(float(lam2), beta, 0, nmodes, gmesh)
+            # the last system built on this mesh is reused while lambda2,
+            # beta and the mode count repeat (gradient loops over f)
+            key = (float(lam2), float(beta), int(nmodes))
+            cached = getattr(gmesh, "_adjoint_msys", None)
+            if cached is not None and cached[0] == key:
+                M = cached[1]
+            else:
+                M = ModalMatrixSys(key[0], beta, 0, nmodes, gmesh)
+                gmesh._adjoint_msys = (key, M)
             u = Field(gmesh, f.shape[0])
             if lines is not None:
                 u.lines.copy_(lines)
