@@ -301,10 +301,52 @@ __global__ __launch_bounds__(64) void k_condense (
 // Batched condensation / recovery: one workgroup per (tile of same-class
 // elements, Fourier mode).  The mode's condensed matrices of that class
 // are staged in LDS once and applied to all element-planes of the tile
-// (both planes of the mode); each thread accumulates four element-planes
-// per matrix entry it reads.
+// (both planes of the mode) as one product: the element-planes are the
+// columns of an LDS operand [k][CG_EP], the matrix rows [k][a] are padded
+// to 4, and each thread accumulates a 4 (rows) x 2 (element-planes) tile
+// from 16-byte LDS reads -- 8 FMAs per 3 LDS instructions.
 // ---------------------------------------------------------------------------
-constexpr int CT_THREADS = 256, CT_NB = 4;
+constexpr int CT_THREADS = 256;
+constexpr int CG_EP = 34;          // element-plane stride in LDS (>= 2 ETILE, bank-shifted)
+
+__host__ __device__ constexpr int pad4 (int x) { return (x + 3) & ~3; }
+
+// dst(i) <- src(i) for i < n with SU loads in flight per thread (the loads
+// of a batch are issued before any of them is waited for)
+constexpr int SU = 8;
+template <typename Load, typename Store>
+__device__ __forceinline__ void stage (int n, Load load, Store store)
+{
+  for (int i0 = threadIdx.x; i0 < n; i0 += SU * CT_THREADS) {
+    double v[SU];
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int i = i0 + u * CT_THREADS;
+      v[u] = i < n ? load (i) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < SU; u++) {
+      const int i = i0 + u * CT_THREADS;
+      if (i < n) store (i, v[u]);
+    }
+  }
+}
+
+// acc[r][j] += sum_k A[k][a0 + r] X[k][e0 + j], k < K (A rows of stride NA)
+__device__ __forceinline__ void tile_4x2 (const double* __restrict__ A, int NA,
+                                          const double* __restrict__ X, int K,
+                                          double (&acc)[4][2])
+{
+  for (int k = 0; k < K; k++) {
+    const double2 a01 = *(const double2*) (A + (size_t) k * NA);
+    const double2 a23 = *(const double2*) (A + (size_t) k * NA + 2);
+    const double2 x = *(const double2*) (X + (size_t) k * CG_EP);
+    acc[0][0] += a01.x * x.x; acc[0][1] += a01.x * x.y;
+    acc[1][0] += a01.y * x.x; acc[1][1] += a01.y * x.y;
+    acc[2][0] += a23.x * x.x; acc[2][1] += a23.x * x.y;
+    acc[3][0] += a23.y * x.x; acc[3][1] += a23.y * x.y;
+  }
+}
 
 // the solved planes of mode m: k0 and their count (1 or 2)
 __device__ __forceinline__ int mode_planes (int m, int nz, int base_mode, int* k0)
@@ -348,57 +390,61 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int nz, int base_mode, int weak)
 {
   extern __shared__ double sm[];
-  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next, NE = pad4 (next);
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
   const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
             cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
-  double* Bs = sm;                                   // [next][nint]
-  double* wp = Bs + next * nint;                     // [nep][npnp], emap order
-  int* emap = (int*) (wp + (size_t) ETILE * 2 * npnp);
+  double* As = sm;                                   // [nint][NE]: As[q][a] = Bm[a][q]
+  double* Ws = As + (size_t) nint * NE;              // [npnp][CG_EP]: w in emap order
+  int* emap = (int*) (Ws + (size_t) npnp * CG_EP);
   const int tid = threadIdx.x;
   const double* B = Bm + ((size_t) mode * nclass + c) * next * nint;
-  for (int i = tid; i < next * nint; i += CT_THREADS) Bs[i] = B[i];
+  stage (next * nint, [&] (int i) { return B[i]; },
+         [&] (int i, double v) { As[(i % nint) * NE + i / nint] = v; });
+  for (int i = tid; i < nint * (NE - next); i += CT_THREADS)
+    As[(i / (NE - next)) * NE + next + i % (NE - next)] = 0.0;
   for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  for (int i = tid; i < npnp * (CG_EP - nep); i += CT_THREADS)
+    Ws[(i / (CG_EP - nep)) * CG_EP + nep + i % (CG_EP - nep)] = 0.0;
   __syncthreads ();
-  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
-    const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
-    const int r = emap[a];
-    const double v = F[(size_t) k * stride + (size_t) e * npnp + r];
-    wp[i] = (weak || emask[e]) ? v : -v * geo[((size_t) e * NGEO + G_Q8) * npnp + r];
-  }
+  // w = -Q8 f in emap order (f as given on essential elements / weak input)
+  stage (nep * npnp,
+         [&] (int i) {
+           const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+           const int r = emap[a];
+           const double v = F[(size_t) k * stride + (size_t) e * npnp + r];
+           return (weak || emask[e]) ? v : -v * geo[((size_t) e * NGEO + G_Q8) * npnp + r];
+         },
+         [&] (int i, double v) { Ws[(i % npnp) * CG_EP + i / npnp] = v; });
   __syncthreads ();
-  const int ng = (nep + CT_NB - 1) / CT_NB;
-  for (int task = tid; task < next * ng; task += CT_THREADS) {
-    const int a = task % next, g0 = (task / next) * CT_NB;
-    double acc[CT_NB];
+  const int nag = NE / 4, neg = (nep + 1) / 2;
+  for (int task = tid; task < nag * neg; task += CT_THREADS) {
+    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
+    double acc[4][2] = {};
+    tile_4x2 (As + a0, NE, Ws + (size_t) next * CG_EP + e0, nint, acc);
 #pragma unroll
-    for (int j = 0; j < CT_NB; j++) acc[j] = 0.0;
-    for (int q = 0; q < nint; q++) {
-      const double b = Bs[a * nint + q];
+    for (int j = 0; j < 2; j++) {
+      const int ep = e0 + j;
+      if (ep >= nep) continue;
+      const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+      double* out = F + (size_t) k * stride + (size_t) e * npnp;
 #pragma unroll
-      for (int j = 0; j < CT_NB; j++)
-        if (g0 + j < nep) acc[j] += b * wp[(g0 + j) * npnp + next + q];
-    }
-#pragma unroll
-    for (int j = 0; j < CT_NB; j++) {
-      const int ep = g0 + j;
-      if (ep < nep) {
-        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
-        F[(size_t) k * stride + (size_t) e * npnp + a] = wp[ep * npnp + a] - acc[j];
-      }
+      for (int r = 0; r < 4; r++)
+        if (a0 + r < next) out[a0 + r] = Ws[(a0 + r) * CG_EP + ep] - acc[r][j];
     }
   }
   for (int i = tid; i < nep * nint; i += CT_THREADS) {
     const int ep = i / nint, q = i % nint, e = elist[t0 + ep / npm], k = k0 + ep % npm;
-    F[(size_t) k * stride + (size_t) e * npnp + next + q] = wp[ep * npnp + next + q];
+    F[(size_t) k * stride + (size_t) e * npnp + next + q] = Ws[(next + q) * CG_EP + ep];
   }
 }
 
 // u_i = Hinv^T F_i - Bm^T u_b with u_b from the band solution, then the
 // essential values (Element::global2localSC src/element.cpp:368-407,
-// Field::setEssential src/field.cpp:1105-1124).
+// Field::setEssential src/field.cpp:1105-1124).  One operand
+// M[k][a] = (k < next ? -Bm[k][a] : Hinv[k-next][a]) against [u_b | F_i].
 __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -409,66 +455,85 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int nz, int base_mode, int rnr, int roff)
 {
   extern __shared__ double sm[];
-  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next, NI = pad4 (nint);
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
   const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
             cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
-  double* Hs = sm;                                   // [nint][nint]
-  double* Bs = Hs + nint * nint;                     // [next][nint]
-  double* xs = Bs + next * nint;                     // [nep][npnp]: u_b | F_i
-  int* emap = (int*) (xs + (size_t) ETILE * 2 * npnp);
+  double* Ms = sm;                                   // [npnp][NI]
+  double* Xs = Ms + (size_t) npnp * NI;              // [npnp][CG_EP]: u_b | F_i, then u
+  int* emap = (int*) (Xs + (size_t) npnp * CG_EP);
+  int* pm = emap + npnp;                             // row-major position -> emap index
   const int tid = threadIdx.x, ns = nsolve[mode];
   const size_t mc = (size_t) mode * nclass + c;
-  for (int i = tid; i < nint * nint; i += CT_THREADS) Hs[i] = Hinv[mc * nint * nint + i];
-  for (int i = tid; i < next * nint; i += CT_THREADS) Bs[i] = Bm[mc * next * nint + i];
-  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
-  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
-    const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
-    double v;
-    if (a < next) {
+  stage (next * nint, [&] (int i) { return -Bm[mc * next * nint + i]; },
+         [&] (int i, double v) { Ms[(i / nint) * NI + i % nint] = v; });
+  stage (nint * nint, [&] (int i) { return Hinv[mc * nint * nint + i]; },
+         [&] (int i, double v) { Ms[(next + i / nint) * NI + i % nint] = v; });
+  for (int i = tid; i < npnp * (NI - nint); i += CT_THREADS)
+    Ms[(i / (NI - nint)) * NI + nint + i % (NI - nint)] = 0.0;
+  for (int i = tid; i < npnp; i += CT_THREADS) {
+    const int r = emap_g[i];
+    emap[i] = r;
+    pm[r] = i;
+  }
+  for (int i = tid; i < npnp * (CG_EP - nep); i += CT_THREADS)
+    Xs[(i / (CG_EP - nep)) * CG_EP + nep + i % (CG_EP - nep)] = 0.0;
+  // u_b from the band solution (zero on essential nodes, whose values
+  // entered the right-hand side); the values the output takes there
+  // (essential: G) are kept in registers, BO per thread
+  constexpr int BO = 6;                              // >= 32 * 44 / CT_THREADS
+  double bout[BO];
+#pragma unroll
+  for (int u = 0; u < BO; u++) {
+    const int i = tid + u * CT_THREADS;
+    bout[u] = 0.0;
+    if (i < nep * next) {
+      const int ep = i / next, a = i % next, e = elist[t0 + ep / npm], k = k0 + ep % npm;
       const int gg = btog[(size_t) e * next + a];
-      v = gg < ns ? rhs[((size_t) mode * rnr + roff + (k & 1)) * nmax + gg] : 0.0;
-    } else {
-      v = F[(size_t) k * stride + (size_t) e * npnp + a];
+      const double v = gg < ns ? rhs[((size_t) mode * rnr + roff + (k & 1)) * nmax + gg] : 0.0;
+      bout[u] = (emask[e] && bmask[(size_t) e * next + a]) ? G[(size_t) k * nglobal + gg] : v;
+      Xs[a * CG_EP + ep] = v;
     }
-    xs[i] = v;
+  }
+  stage (nep * nint,
+         [&] (int i) {
+           const int ep = i / nint, q = i % nint, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+           return F[(size_t) k * stride + (size_t) e * npnp + next + q];
+         },
+         [&] (int i, double v) { Xs[(next + i % nint) * CG_EP + i / nint] = v; });
+  __syncthreads ();
+  const int nag = NI / 4, neg = (nep + 1) / 2, ntask = nag * neg;
+  // at most two tasks per thread (nint <= 100, nep <= 32): results are
+  // held in registers until every thread has finished reading Xs
+  double acc[2][4][2] = {};
+  for (int s = 0; s < 2; s++) {
+    const int task = tid + s * CT_THREADS;
+    if (task < ntask)
+      tile_4x2 (Ms + 4 * (task % nag), NI, Xs + 2 * (task / nag), npnp, acc[s]);
   }
   __syncthreads ();
-  const int ng = (nep + CT_NB - 1) / CT_NB;
-  for (int task = tid; task < nint * ng; task += CT_THREADS) {
-    const int a = task % nint, g0 = (task / nint) * CT_NB;
-    double acc[CT_NB];
+  for (int s = 0; s < 2; s++) {
+    const int task = tid + s * CT_THREADS;
+    if (task >= ntask) continue;
+    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
 #pragma unroll
-    for (int j = 0; j < CT_NB; j++) acc[j] = 0.0;
-    for (int b = 0; b < nint; b++) {
-      const double h = Hs[b * nint + a];
+    for (int r = 0; r < 4; r++)
 #pragma unroll
-      for (int j = 0; j < CT_NB; j++)
-        if (g0 + j < nep) acc[j] += h * xs[(g0 + j) * npnp + next + b];
-    }
-    for (int q = 0; q < next; q++) {
-      const double b = Bs[q * nint + a];
-#pragma unroll
-      for (int j = 0; j < CT_NB; j++)
-        if (g0 + j < nep) acc[j] -= b * xs[(g0 + j) * npnp + q];
-    }
-#pragma unroll
-    for (int j = 0; j < CT_NB; j++) {
-      const int ep = g0 + j;
-      if (ep < nep) {
-        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
-        U[(size_t) k * stride + (size_t) e * npnp + emap[next + a]] = acc[j];
-      }
-    }
+      for (int j = 0; j < 2; j++)
+        if (a0 + r < nint && e0 + j < nep) Xs[(next + a0 + r) * CG_EP + e0 + j] = acc[s][r][j];
   }
-  for (int i = tid; i < nep * next; i += CT_THREADS) {
-    const int ep = i / next, a = i % next, e = elist[t0 + ep / npm], k = k0 + ep % npm;
-    double v = xs[ep * npnp + a];
-    if (emask[e] && bmask[(size_t) e * next + a])
-      v = G[(size_t) k * nglobal + btog[(size_t) e * next + a]];
-    U[(size_t) k * stride + (size_t) e * npnp + emap[a]] = v;
+#pragma unroll
+  for (int u = 0; u < BO; u++) {
+    const int i = tid + u * CT_THREADS;
+    if (i < nep * next) Xs[(i % next) * CG_EP + i / next] = bout[u];
+  }
+  __syncthreads ();
+  // whole element-planes out, row-major
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, r = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    U[(size_t) k * stride + (size_t) e * npnp + r] = Xs[pm[r] * CG_EP + ep];
   }
 }
 
@@ -774,11 +839,17 @@ namespace {
 size_t tiled_lds (const sx_mesh* m, bool recover)
 {
   const size_t next = m->next, nint = m->nint, npnp = m->npnp;
-  return sizeof (double) * ((recover ? nint * nint : 0) + next * nint + 2 * ETILE * npnp) +
-         sizeof (int) * npnp;
+  if (recover)
+    return sizeof (double) * (npnp * pad4 ((int) nint) + npnp * CG_EP) + sizeof (int) * 2 * npnp;
+  return sizeof (double) * (nint * pad4 ((int) next) + npnp * CG_EP) + sizeof (int) * npnp;
 }
 
-bool tiled_ok (const sx_mesh* m) { return tiled_lds (m, true) <= 160 * 1024; }
+// the tiled kernels hold at most 2 product tasks and 6 boundary values per
+// thread: nint <= 100 and next <= 48 (np <= 12, which the LDS bound implies)
+bool tiled_ok (const sx_mesh* m)
+{
+  return tiled_lds (m, true) <= 160 * 1024 && m->nint <= 100 && 2 * ETILE * m->next <= 6 * 256;
+}
 
 int n_mode_blocks (int nz) { return nz == 1 ? 1 : nz / 2; }
 
