@@ -40,19 +40,29 @@ __global__ __launch_bounds__(64) void k_grad (int np, const double* __restrict__
   const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
   __shared__ double D[256], u[256];
   const size_t off = (size_t) k * stride + (size_t) e * npnp;
-  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; u[i] = in[off + i]; }
-  __syncthreads ();
   const double* ge = geo + (size_t) e * NGEO * npnp;
   const double* gr = ge + (dir == 0 ? G_RX : G_RY) * npnp;
   const double* gs = ge + (dir == 0 ? G_SX : G_SY) * npnp;
-  for (int ij = lane; ij < npnp; ij += 64) {
+  // every load (operator, field, metric factors) in one round trip;
+  // np <= 16, so a lane holds at most 4 nodes
+  double r_[4], s_[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int i = lane + 64 * t;
+    if (i < npnp) { D[i] = Dg[i]; u[i] = in[off + i]; r_[t] = gr[i]; s_[t] = gs[i]; }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij >= npnp) break;
     const int i = ij / np, j = ij % np;
     double ur = 0.0, us = 0.0;
     for (int q = 0; q < np; q++) {
       ur += u[i * np + q] * D[j * np + q];
       us += D[i * np + q] * u[q * np + j];
     }
-    out[off + ij] = ur * gr[ij] + us * gs[ij];
+    out[off + ij] = ur * r_[t] + us * s_[t];
   }
 }
 

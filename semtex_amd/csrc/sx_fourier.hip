@@ -51,9 +51,18 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft_z (const double* in, double*
   double* raw = (double*) B;
   const int s0 = blockIdx.x * TS, tid = threadIdx.x;
   for (int i = tid; i < N / 2; i += FT_THREADS) W[i] = tw[i];
-  for (int i = tid; i < N * TS; i += FT_THREADS) {
-    const int n = i / TS, c = i % TS, s = s0 + c;
-    raw[i] = s < S ? in[(size_t) n * S + s] : 0.0;
+  {
+    // the whole tile's loads in flight before any LDS write
+    constexpr int PER = N * TS / FT_THREADS;
+    static_assert (N * TS % FT_THREADS == 0, "tile must split evenly over the threads");
+    double v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int i = tid + u * FT_THREADS, n = i / TS, s = s0 + i % TS;
+      v[u] = s < S ? in[(size_t) n * S + s] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) raw[tid + u * FT_THREADS] = v[u];
   }
   __syncthreads ();
   if (sign > 0) {
