@@ -777,9 +777,12 @@ int launch_panel (const sx_msys* s, int G, int stage, hipStream_t st)
   case 4: return launch_panel_d<NR, 4> (s, G, stage, st);
   case 5: return launch_panel_d<NR, 5> (s, G, stage, st);
   case 6: return launch_panel_d<NR, 6> (s, G, stage, st);
-  case 7: return launch_panel_d<NR, 7> (s, G, stage, st);
-  case 8: return launch_panel_d<NR, 8> (s, G, stage, st);
   default: break;
+  }
+  // more than 2 right-hand sides per workgroup need D <= 6 (msys_reserve_rhs)
+  if constexpr (NR <= 2) {
+    if (s->sw_D == 7) return launch_panel_d<NR, 7> (s, G, stage, st);
+    if (s->sw_D == 8) return launch_panel_d<NR, 8> (s, G, stage, st);
   }
   set_error ("panel sweep: unsupported block count");
   return SX_ERR_LIMIT;
