@@ -130,6 +130,137 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft_z (const double* in, double*
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level FFT for nz = N = R1 R2 (32..256): with n = R2 n1 + n2 and
+// k = k1 + R1 k2,
+//   X[k1 + R1 k2] = sum_n2 W_R2^(n2 k2) W_N^(n2 k1) sum_n1 W_R1^(n1 k1) x[R2 n1 + n2].
+// Thread (q, n2) loads its R1 points straight from HBM and transforms them
+// in registers; one LDS exchange; thread (q, k1) transforms its R2 points
+// in registers.  The inverse writes its columns from there; the forward
+// separates the two real spectra through one more exchange.
+// ---------------------------------------------------------------------------
+template <int N, int R1>
+struct Fft2 {
+  static constexpr int R2 = N / R1;
+  static constexpr int NQ = FT_THREADS / R2;      // complex sequences per tile
+  static constexpr int TS = 2 * NQ;               // real columns per tile
+  static size_t lds () { return sizeof (double2) * ((size_t) N * NQ + N / 2); }
+};
+
+// in-register radix-2 DFT of R points (natural order in and out);
+// W_R^j = W[j wstep] with W the N/2 twiddles of the full length
+template <int R>
+__device__ __forceinline__ void dft_regs (double2 (&v)[R], const double2* W, int wstep, bool inv)
+{
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    int j = 0;
+#pragma unroll
+    for (int b = 1, r = R >> 1; b < R; b <<= 1, r >>= 1)
+      if (i & b) j |= r;
+    if (j > i) { const double2 t = v[i]; v[i] = v[j]; v[j] = t; }
+  }
+#pragma unroll
+  for (int len = 2; len <= R; len <<= 1) {
+#pragma unroll
+    for (int j = 0; j < len / 2; j++) {
+      double2 w = W[j * (R / len) * wstep];
+      if (inv) w.y = -w.y;
+#pragma unroll
+      for (int i = 0; i < R; i += len) {
+        const double2 u = v[i + j], t = cmul (v[i + j + len / 2], w);
+        v[i + j] = make_double2 (u.x + t.x, u.y + t.y);
+        v[i + j + len / 2] = make_double2 (u.x - t.x, u.y - t.y);
+      }
+    }
+  }
+}
+
+template <int N, int R1>
+__global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double* out, int S,
+                                                        int sign, const double2* __restrict__ tw)
+{
+  constexpr int R2 = Fft2<N, R1>::R2, NQ = Fft2<N, R1>::NQ, TS = Fft2<N, R1>::TS;
+  static_assert (NQ * R2 == FT_THREADS && NQ * R1 <= FT_THREADS, "tile shape");
+  extern __shared__ double2 lds[];
+  double2* T = lds;                   // [N][NQ] exchange, then the spectrum
+  double2* W = T + N * NQ;            // exp(-2 pi i j / N), j < N/2
+  const int s0 = blockIdx.x * TS, tid = threadIdx.x;
+  const bool inv = sign < 0;
+  for (int i = tid; i < N / 2; i += FT_THREADS) W[i] = tw[i];
+  // step A: thread (q, n2), columns c0 and c0 + 1 as one complex sequence
+  {
+    const int q = tid % NQ, n2 = tid / NQ, c0 = s0 + 2 * q;
+    const bool in0 = c0 < S, in1 = c0 + 1 < S;
+    double2 v[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int k = R2 * n1 + n2;
+      if (!inv) {
+        const double* row = in + (size_t) k * S;
+        v[n1] = make_double2 (in0 ? row[c0] : 0.0, in1 ? row[c0 + 1] : 0.0);
+      } else {
+        // Z_k = Xa_k + i Xb_k from the packed spectra (X_{N-k} = conj X_k)
+        const int kk = k <= N / 2 ? k : N - k;
+        const int pr = kk == 0 ? 0 : (kk == N / 2 ? 1 : 2 * kk);
+        const double* re = in + (size_t) pr * S;
+        const double* im = in + (size_t) (2 * kk + 1) * S;
+        const bool cplx = kk != 0 && kk != N / 2;
+        const double ar = in0 ? re[c0] : 0.0, br = in1 ? re[c0 + 1] : 0.0;
+        double ai = cplx && in0 ? im[c0] : 0.0, bi = cplx && in1 ? im[c0 + 1] : 0.0;
+        if (k > N / 2) { ai = -ai; bi = -bi; }
+        v[n1] = make_double2 (ar - bi, ai + br);
+      }
+    }
+    __syncthreads ();                 // twiddles in LDS
+    dft_regs<R1> (v, W, N / R1, inv);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+      const int m = n2 * k1;          // < N
+      double2 w = m < N / 2 ? W[m] : make_double2 (-W[m - N / 2].x, -W[m - N / 2].y);
+      if (inv) w.y = -w.y;
+      T[(k1 * R2 + n2) * NQ + q] = cmul (v[k1], w);
+    }
+  }
+  __syncthreads ();
+  // step B: thread (q, k1)
+  const int q = tid % NQ, k1 = tid / NQ, c0 = s0 + 2 * q;
+  const bool act = tid < NQ * R1;
+  double2 y[R2];
+  if (act) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) y[n2] = T[(k1 * R2 + n2) * NQ + q];
+    dft_regs<R2> (y, W, N / R2, inv);
+  }
+  if (inv) {
+    if (act)
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++) {
+        double* row = out + (size_t) (k1 + R1 * k2) * S;
+        if (c0 < S) row[c0] = y[k2].x;
+        if (c0 + 1 < S) row[c0 + 1] = y[k2].y;
+      }
+    return;
+  }
+  __syncthreads ();                   // every step-B read of T is done
+  if (act)
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) T[(k1 + R1 * k2) * NQ + q] = y[k2];
+  __syncthreads ();
+  const double scale = 1.0 / N;
+  for (int i = tid; i < N * TS; i += FT_THREADS) {
+    const int p = i / TS, c = i % TS, s = s0 + c;
+    if (s >= S) continue;
+    const int qq = c >> 1, k = p == 0 ? 0 : (p == 1 ? N / 2 : p >> 1);
+    const double2 zk = T[k * NQ + qq], zr = T[((N - k) & (N - 1)) * NQ + qq];
+    // Xa = (Z_k + conj Z_{N-k}) / 2,  Xb = (Z_k - conj Z_{N-k}) / 2i
+    double re, im;
+    if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
+    else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
+    out[(size_t) p * S + s] = ((p < 2 || (p & 1) == 0) ? re : im) * scale;
+  }
+}
+
 // Packed DFT matrix product out[k][s] = sum_n M[k][n] in[n][s] (any nz);
 // one thread per column, KG output planes per pass (M rows in LDS).
 constexpr int KG = 16;
@@ -210,6 +341,21 @@ int tables (int nz, Tables** out)
   return SX_OK;
 }
 
+template <int N, int R1>
+int launch_fft2 (int S, const double* in, double* out, int sign, const double2* tw,
+                 hipStream_t st)
+{
+  const size_t lds = Fft2<N, R1>::lds ();
+  auto kern = k_fft2_z<N, R1>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  const int TS = Fft2<N, R1>::TS;
+  hipLaunchKernelGGL (kern, dim3 ((S + TS - 1) / TS), dim3 (FT_THREADS), lds, st, in, out, S,
+                      sign, tw);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
 template <int N>
 int launch_fft (int S, const double* in, double* out, int sign, const double2* tw, hipStream_t st)
 {
@@ -248,10 +394,10 @@ int fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream
     switch (nz) {
     case 8: return launch_fft<8> (S, in, out, sign, t->tw, st);
     case 16: return launch_fft<16> (S, in, out, sign, t->tw, st);
-    case 32: return launch_fft<32> (S, in, out, sign, t->tw, st);
-    case 64: return launch_fft<64> (S, in, out, sign, t->tw, st);
-    case 128: return launch_fft<128> (S, in, out, sign, t->tw, st);
-    case 256: return launch_fft<256> (S, in, out, sign, t->tw, st);
+    case 32: return launch_fft2<32, 4> (S, in, out, sign, t->tw, st);
+    case 64: return launch_fft2<64, 8> (S, in, out, sign, t->tw, st);
+    case 128: return launch_fft2<128, 8> (S, in, out, sign, t->tw, st);
+    case 256: return launch_fft2<256, 16> (S, in, out, sign, t->tw, st);
     case 512: return launch_fft<512> (S, in, out, sign, t->tw, st);
     case 1024: return launch_fft<1024> (S, in, out, sign, t->tw, st);
     default: break;

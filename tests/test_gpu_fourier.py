@@ -18,8 +18,8 @@ def torch_cuda():
     return torch
 
 
-@pytest.mark.parametrize("nz,S", [(8, 484), (16, 1000), (128, 18432), (256, 333),
-                                  (1024, 70), (12, 500), (2, 64)])
+@pytest.mark.parametrize("nz,S", [(8, 484), (16, 1000), (32, 777), (64, 130), (128, 18432),
+                                  (128, 35), (256, 333), (1024, 70), (12, 500), (2, 64)])
 def test_transform_matches_oracle(torch_cuda, oracle, nz, S):
     import torch
     from oracle import oracle as O
