@@ -66,6 +66,64 @@ __global__ __launch_bounds__(64) void k_grad (int np, const double* __restrict__
   }
 }
 
+// The x and y terms of the nonlinear term of component u, per element-plane
+// in physical space (skewSymmetric / altSkewSymmetric, dns/nonlinear.cpp,
+// Cartesian branches :162-193, :342-380, with Element::grad for every
+// derivative):
+//   N = -[conv (ux d_x u + uy d_y u) + cons (d_x (u ux) + d_y (u uy))]
+// written, not accumulated: the z terms are subtracted afterwards.
+__global__ __launch_bounds__(64) void k_adv_xy (int np, const double* __restrict__ geo,
+                                                const double* __restrict__ Dg,
+                                                const double* __restrict__ ux,
+                                                const double* __restrict__ uy,
+                                                const double* __restrict__ ui, double* N,
+                                                int stride, int conv, int cons)
+{
+  const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  __shared__ double D[256], fu[256], fx[256], fy[256];     // u, u ux, u uy
+  const size_t off = (size_t) k * stride + (size_t) e * npnp;
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  double ax[4], ay[4], rx[4], ry[4], sx[4], sy[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {       // np <= 16: at most 4 nodes per lane
+    const int i = lane + 64 * t;
+    if (i < npnp) {
+      D[i] = Dg[i];
+      const double a = ui[off + i];
+      ax[t] = ux[off + i];
+      ay[t] = uy[off + i];
+      fu[i] = a;
+      fx[i] = a * ax[t];
+      fy[i] = a * ay[t];
+      rx[t] = ge[G_RX * npnp + i];
+      ry[t] = ge[G_RY * npnp + i];
+      sx[t] = ge[G_SX * npnp + i];
+      sy[t] = ge[G_SY * npnp + i];
+    }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij >= npnp) break;
+    const int i = ij / np, j = ij % np;
+    double ur = 0.0, us = 0.0, xr = 0.0, xs = 0.0, yr = 0.0, ys = 0.0;
+    for (int q = 0; q < np; q++) {
+      const double dj = D[j * np + q], di = D[i * np + q];
+      ur += fu[i * np + q] * dj;
+      us += di * fu[q * np + j];
+      xr += fx[i * np + q] * dj;
+      xs += di * fx[q * np + j];
+      yr += fy[i * np + q] * dj;
+      ys += di * fy[q * np + j];
+    }
+    double v = 0.0;
+    if (conv) v += ax[t] * (ur * rx[t] + us * sx[t]) + ay[t] * (ur * ry[t] + us * sy[t]);
+    if (cons) v += (xr * rx[t] + xs * sx[t]) + (yr * ry[t] + ys * sy[t]);
+    N[off + ij] = -v;
+  }
+}
+
 // AuxField::gradient(2) on Fourier planes: planes 0 and 1 zero,
 // Re_k <- -beta k Im_k, Im_k <- beta k Re_k.
 __global__ void k_gradz (const double* __restrict__ in, double* out, int nz,
@@ -426,26 +484,25 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
   }
   const bool conv = d->advection == SX_SKEW || d->toggle == 1;
   const bool cons = d->advection == SX_SKEW || d->toggle == 0;
+  const sx_mesh* m = d->mu;
   for (int i = 0; i < nc; i++) {
-    for (int j = 0; j < nc; j++) {
+    // x and y terms in one pass per element-plane, then the z terms
+    hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
+                        m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
+                        (int) cons);
+    if (nc == 3) {
+      // z terms through Fourier space: w d_z u and d_z (u w)
       if (conv) {
-        if (j == 2) {
-          if ((rc = grad (d, U[i], d->tmp, 2, st))) return rc;
-          if ((rc = transform (d, d->tmp, -1, st))) return rc;
-        } else if ((rc = grad (d, d->phys[i], d->tmp, j, st))) return rc;
-        EW (k_ew_submul, N[i], d->phys[j], d->tmp, d->n);
+        if ((rc = grad (d, U[i], d->tmp, 2, st))) return rc;
+        if ((rc = transform (d, d->tmp, -1, st))) return rc;
+        EW (k_ew_submul, N[i], d->phys[2], d->tmp, d->n);
       }
       if (cons) {
-        EW (k_ew_mul, d->tmp, d->phys[i], d->phys[j], d->n);
-        if (j == 2) {
-          if ((rc = transform (d, d->tmp, +1, st))) return rc;
-          if ((rc = grad (d, d->tmp, d->tmp2, 2, st))) return rc;
-          if ((rc = transform_to (d, d->tmp2, d->tmp, -1, st))) return rc;
-          EW (k_ew_sub, N[i], d->tmp, d->n);
-        } else {
-          if ((rc = grad (d, d->tmp, d->tmp2, j, st))) return rc;
-          EW (k_ew_sub, N[i], d->tmp2, d->n);
-        }
+        EW (k_ew_mul, d->tmp, d->phys[i], d->phys[2], d->n);
+        if ((rc = transform (d, d->tmp, +1, st))) return rc;
+        if ((rc = grad (d, d->tmp, d->tmp2, 2, st))) return rc;
+        if ((rc = transform_to (d, d->tmp2, d->tmp, -1, st))) return rc;
+        EW (k_ew_sub, N[i], d->tmp, d->n);
       }
     }
     if (d->advection == SX_SKEW) EW (k_ew_scale, N[i], 0.5, d->n);
