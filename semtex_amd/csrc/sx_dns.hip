@@ -490,6 +490,27 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
     hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
                         m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
                         (int) cons);
+    const bool fused = nc == 3 && fourier_fused_ok (d->nz);
+    if (fused) {
+      // z terms folded into the transforms: N -= w .* INV(d_z U) and
+      // N -= INV(d_z FWD(u .* w))
+      if (conv && (rc = fourier_z_fused (d->nz, d->S, U[i], nullptr, N[i], -1, d->beta, N[i],
+                                         d->phys[2], 1.0, 0.0, st)))
+        return rc;
+      if (cons &&
+          ((rc = fourier_z_fused (d->nz, d->S, d->phys[i], d->phys[2], d->tmp, +1, 0.0,
+                                  nullptr, nullptr, 1.0, 0.0, st)) ||
+           (rc = fourier_z_fused (d->nz, d->S, d->tmp, nullptr, N[i], -1, d->beta, N[i],
+                                  nullptr, 1.0, 0.0, st))))
+        return rc;
+      // back to Fourier space with the skew-symmetric 1/2 and ConstForce
+      const double c = std::fabs (d->cf[i]) > EPSDP ? d->cf[i] : 0.0;
+      if ((rc = fourier_z_fused (d->nz, d->S, N[i], nullptr, N[i], +1, 0.0, nullptr, nullptr,
+                                 d->advection == SX_SKEW ? 0.5 : 1.0, c, st)))
+        return rc;
+      if ((rc = smooth (d, N[i], st))) return rc;
+      continue;
+    }
     if (nc == 3) {
       // z terms through Fourier space: w d_z u and d_z (u w)
       if (conv) {

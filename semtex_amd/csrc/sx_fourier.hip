@@ -176,9 +176,21 @@ __device__ __forceinline__ void dft_regs (double2 (&v)[R], const double2* W, int
   }
 }
 
+// Fused forms (Fused): forward input in .* mul, forward output
+// scale * X (+ add0 on plane 0); inverse input multiplied by i beta k (the
+// z derivative, AuxField::gradient(2): Nyquist and k = 0 dropped), inverse
+// output acc - amul .* x.  Null pointers / zero values switch a part off.
+struct Fused {
+  const double* mul = nullptr;
+  double dz = 0.0, scale = 1.0, add0 = 0.0;
+  const double* acc = nullptr;
+  const double* amul = nullptr;
+};
+
 template <int N, int R1>
 __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double* out, int S,
-                                                        int sign, const double2* __restrict__ tw)
+                                                        int sign, const double2* __restrict__ tw,
+                                                        Fused fu)
 {
   constexpr int R2 = Fft2<N, R1>::R2, NQ = Fft2<N, R1>::NQ, TS = Fft2<N, R1>::TS;
   static_assert (NQ * R2 == FT_THREADS && NQ * R1 <= FT_THREADS, "tile shape");
@@ -199,6 +211,11 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double
       if (!inv) {
         const double* row = in + (size_t) k * S;
         v[n1] = make_double2 (in0 ? row[c0] : 0.0, in1 ? row[c0 + 1] : 0.0);
+        if (fu.mul) {
+          const double* mr = fu.mul + (size_t) k * S;
+          v[n1].x *= in0 ? mr[c0] : 0.0;
+          v[n1].y *= in1 ? mr[c0 + 1] : 0.0;
+        }
       } else {
         // Z_k = Xa_k + i Xb_k from the packed spectra (X_{N-k} = conj X_k)
         const int kk = k <= N / 2 ? k : N - k;
@@ -210,6 +227,10 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double
         double ai = cplx && in0 ? im[c0] : 0.0, bi = cplx && in1 ? im[c0 + 1] : 0.0;
         if (k > N / 2) { ai = -ai; bi = -bi; }
         v[n1] = make_double2 (ar - bi, ai + br);
+        if (fu.dz != 0.0) {           // times i beta k_s, k_s = k - N above N/2
+          const double a = fu.dz * (k < N / 2 ? k : (k == N / 2 ? 0 : k - N));
+          v[n1] = make_double2 (-a * v[n1].y, a * v[n1].x);
+        }
       }
     }
     __syncthreads ();                 // twiddles in LDS
@@ -236,9 +257,18 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double
     if (act)
 #pragma unroll
       for (int k2 = 0; k2 < R2; k2++) {
-        double* row = out + (size_t) (k1 + R1 * k2) * S;
-        if (c0 < S) row[c0] = y[k2].x;
-        if (c0 + 1 < S) row[c0 + 1] = y[k2].y;
+        const size_t ro = (size_t) (k1 + R1 * k2) * S;
+        double a = y[k2].x, b = y[k2].y;
+        if (fu.acc) {
+          if (fu.amul) {
+            a *= c0 < S ? fu.amul[ro + c0] : 0.0;
+            b *= c0 + 1 < S ? fu.amul[ro + c0 + 1] : 0.0;
+          }
+          a = (c0 < S ? fu.acc[ro + c0] : 0.0) - a;
+          b = (c0 + 1 < S ? fu.acc[ro + c0 + 1] : 0.0) - b;
+        }
+        if (c0 < S) out[ro + c0] = a;
+        if (c0 + 1 < S) out[ro + c0 + 1] = b;
       }
     return;
   }
@@ -247,7 +277,7 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double
 #pragma unroll
     for (int k2 = 0; k2 < R2; k2++) T[(k1 + R1 * k2) * NQ + q] = y[k2];
   __syncthreads ();
-  const double scale = 1.0 / N;
+  const double scale = fu.scale / N;
   for (int i = tid; i < N * TS; i += FT_THREADS) {
     const int p = i / TS, c = i % TS, s = s0 + c;
     if (s >= S) continue;
@@ -257,7 +287,8 @@ __global__ __launch_bounds__(FT_THREADS) void k_fft2_z (const double* in, double
     double re, im;
     if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
     else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
-    out[(size_t) p * S + s] = ((p < 2 || (p & 1) == 0) ? re : im) * scale;
+    out[(size_t) p * S + s] = ((p < 2 || (p & 1) == 0) ? re : im) * scale +
+                              (p == 0 ? fu.add0 : 0.0);
   }
 }
 
@@ -343,7 +374,7 @@ int tables (int nz, Tables** out)
 
 template <int N, int R1>
 int launch_fft2 (int S, const double* in, double* out, int sign, const double2* tw,
-                 hipStream_t st)
+                 hipStream_t st, Fused fu = Fused ())
 {
   const size_t lds = Fft2<N, R1>::lds ();
   auto kern = k_fft2_z<N, R1>;
@@ -351,7 +382,7 @@ int launch_fft2 (int S, const double* in, double* out, int sign, const double2* 
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   const int TS = Fft2<N, R1>::TS;
   hipLaunchKernelGGL (kern, dim3 ((S + TS - 1) / TS), dim3 (FT_THREADS), lds, st, in, out, S,
-                      sign, tw);
+                      sign, tw, fu);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -378,6 +409,33 @@ int fourier_prepare (int nz)
 {
   Tables* t = nullptr;
   return nz > 1 ? tables (nz, &t) : SX_OK;
+}
+
+bool fourier_fused_ok (int nz) { return nz == 32 || nz == 64 || nz == 128 || nz == 256; }
+
+int fourier_z_fused (int nz, int S, const double* in, const double* mul, double* out, int sign,
+                     double dz, const double* acc, const double* amul, double scale, double add0,
+                     hipStream_t st)
+{
+  if (!fourier_fused_ok (nz) || S < 1 || (sign != 1 && sign != -1)) {
+    set_error ("fourier_z_fused: nz must be 32, 64, 128 or 256");
+    return SX_ERR_ARG;
+  }
+  Tables* t = nullptr;
+  if (int rc = tables (nz, &t)) return rc;
+  Fused fu;
+  fu.mul = sign > 0 ? mul : nullptr;
+  fu.scale = sign > 0 ? scale : 1.0;
+  fu.add0 = sign > 0 ? add0 : 0.0;
+  fu.dz = sign < 0 ? dz : 0.0;
+  fu.acc = sign < 0 ? acc : nullptr;
+  fu.amul = sign < 0 ? amul : nullptr;
+  switch (nz) {
+  case 32: return launch_fft2<32, 4> (S, in, out, sign, t->tw, st, fu);
+  case 64: return launch_fft2<64, 8> (S, in, out, sign, t->tw, st, fu);
+  case 128: return launch_fft2<128, 8> (S, in, out, sign, t->tw, st, fu);
+  default: return launch_fft2<256, 16> (S, in, out, sign, t->tw, st, fu);
+  }
 }
 
 int fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st)

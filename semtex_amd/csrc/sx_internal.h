@@ -58,6 +58,14 @@ int    field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const*
 // (sx_fourier.hip); in place when fourier_in_place (nz).
 int    fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st);
 int    fourier_prepare (int nz);   // builds the per-device tables (synchronous)
+// Fused z transforms (nz 32, 64, 128, 256: fourier_fused_ok):
+//   sign +1: out = scale FWD(in .* mul) + add0 on plane 0   (mul may be null)
+//   sign -1: out = acc - amul .* INV(dz-derivative of in)   (dz = beta: multiply
+//            by i beta k first, 0: none; acc null: out = INV(..); amul null: 1)
+bool   fourier_fused_ok (int nz);
+int    fourier_z_fused (int nz, int S, const double* in, const double* mul, double* out,
+                        int sign, double dz, const double* acc, const double* amul,
+                        double scale, double add0, hipStream_t st);
 // Mixed-precision solve with a fixed number of refinement passes, no host
 // synchronisation (sx_mixed.hip).
 int    field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,

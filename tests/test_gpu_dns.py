@@ -78,17 +78,17 @@ def test_gpu_dns_reproduces_regress_goldens(torch_cuda, oracle, case, startup):
             assert f"{err:.4e}" == f"{want:.4e}", (comp, err, want)
 
 
-@pytest.mark.parametrize("startup", ["direct", "pcg"])
-def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup):
+@pytest.mark.parametrize("startup,nz", [("direct", 8), ("pcg", 8), ("direct", 32)])
+def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup, nz):
     import torch
     from oracle.dns_oracle import DnsOracle
     from semtex_amd.mesh import rect_mesh
     from semtex_amd.solver import DNS, SX_JACPCG, SX_SKEW, GpuMesh
-    print("case: channel 3x4 np=6 nz=8 skew, 8 steps", startup, flush=True)
+    print(f"case: channel 3x4 np=6 nz={nz} skew, 8 steps", startup, flush=True)
     bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
     mu = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
     mp = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
-    nz, beta, nu, dt = 8, 1.0, 1.0 / 50, 5e-3
+    beta, nu, dt = 1.0, 1.0 / 50, 5e-3
     cf = (2 * nu, 0.0, 0.0)
     gu, gp = GpuMesh(mu), GpuMesh(mp)
     D = DNS(gu, gp, nz, beta, nu, dt, const_force=cf, advection=SX_SKEW)
