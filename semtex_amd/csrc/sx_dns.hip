@@ -4,17 +4,19 @@
 // computed pressure boundary conditions (BCmgr::maintainFourier /
 // evaluateCNBCp, src/bcmgr.cpp:1037-1335, Edge::curlCurl src/edge.cpp:98),
 // Field::smooth (src/field.cpp:282-327) and ConstForce
-// (dns/fieldforce.cpp:268).  All fields stay on the GPU; the four
-// Helmholtz solves per step go through sx_field_solve.
+// (dns/fieldforce.cpp:268).  All fields stay on the GPU; the pressure and
+// the grouped velocity solves go through the library's Field::solve, the
+// z transforms through sx_fourier.hip (with the z terms of the advection
+// folded into them where nz allows).
 //
 // Kernels:
-//   k_dft        packed real DFT along z as an (nz x nz) matrix applied to
-//                [nz][stride] planes (femlib/fourier.c packing, 1/nz on
-//                the forward transform), FP64, LDS-staged matrix rows
+//   k_adv_xy     x and y terms of the nonlinear term of one component per
+//                element-plane (products, Element::grad contractions)
+//   k_div_h      setPForce: div H / dt per element-plane
+//   k_project3   project: F_i = H_i alpha + beta d_i p, all components
 //   k_grad       x / y derivative per element (Element::grad)
 //   k_gradz      Fourier z derivative (AuxField::gradient(2))
-//   k_ew_*       fused element-wise field algebra of the advection terms,
-//                waveProp, setPForce and project
+//   k_ew_*       element-wise field algebra (waveProp, the unfused z terms)
 //   k_smooth_*   mass-averaged direct stiffness sum on element boundaries
 //   k_hopbc      -d(u.n)/dt + n.N - nu n.curlcurl u on boundary segments
 //   k_plines     extrapolated dp/dn line data for the pressure solve
@@ -124,6 +126,128 @@ __global__ __launch_bounds__(64) void k_adv_xy (int np, const double* __restrict
   }
 }
 
+// the z derivative of a Fourier field at one node of plane k (k's partner
+// plane holds the other half of the mode): planes 0, 1 -> 0,
+// Re_k <- -beta m Im_k, Im_k <- beta m Re_k, m = k/2
+__device__ __forceinline__ double dz_at (const double* f, int k, int stride, size_t node,
+                                         double beta)
+{
+  const int m = k >> 1;
+  if (m < 1) return 0.0;
+  const double bk = beta * m;
+  return (k & 1) ? bk * f[(size_t) (k - 1) * stride + node]
+                 : -bk * f[(size_t) (k + 1) * stride + node];
+}
+
+// setPForce (dns/integrate.cpp:242-257): out = div H / dt, per
+// element-plane of the Fourier field H (x, y by Element::grad, z as
+// AuxField::gradient(2)); H2 null for two components
+__global__ __launch_bounds__(64) void k_div_h (int np, const double* __restrict__ geo,
+                                               const double* __restrict__ Dg,
+                                               const double* __restrict__ H0,
+                                               const double* __restrict__ H1,
+                                               const double* __restrict__ H2, double* out,
+                                               int stride, double beta, double dt)
+{
+  const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  __shared__ double D[256], a[256], b[256];
+  const size_t eo = (size_t) e * npnp, off = (size_t) k * stride + eo;
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  double rx[4], sx[4], ry[4], sy[4], gz[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {       // np <= 16: at most 4 nodes per lane
+    const int i = lane + 64 * t;
+    if (i < npnp) {
+      D[i] = Dg[i];
+      a[i] = H0[off + i];
+      b[i] = H1[off + i];
+      rx[t] = ge[G_RX * npnp + i];
+      sx[t] = ge[G_SX * npnp + i];
+      ry[t] = ge[G_RY * npnp + i];
+      sy[t] = ge[G_SY * npnp + i];
+      gz[t] = H2 ? dz_at (H2, k, stride, eo + i, beta) : 0.0;
+    }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij >= npnp) break;
+    const int i = ij / np, j = ij % np;
+    double ar = 0.0, as = 0.0, br = 0.0, bs = 0.0;
+    for (int q = 0; q < np; q++) {
+      const double dj = D[j * np + q], di = D[i * np + q];
+      ar += a[i * np + q] * dj;
+      as += di * a[q * np + j];
+      br += b[i * np + q] * dj;
+      bs += di * b[q * np + j];
+    }
+    double sum = (ar * rx[t] + as * sx[t]) + (br * ry[t] + bs * sy[t]);
+    if (H2) sum += gz[t];
+    out[off + ij] = sum / dt;
+  }
+}
+
+// project (dns/integrate.cpp:260-289) for every component at once:
+// F_i = H_i alpha + beta d_i p, per element-plane (x, y by Element::grad,
+// z as AuxField::gradient(2)); nc = 2 or 3
+__global__ __launch_bounds__(64) void k_project3 (int np, int nc,
+                                                  const double* __restrict__ geo,
+                                                  const double* __restrict__ Dg,
+                                                  const double* __restrict__ p,
+                                                  const double* __restrict__ H0,
+                                                  const double* __restrict__ H1,
+                                                  const double* __restrict__ H2, double* F0,
+                                                  double* F1, double* F2, int stride,
+                                                  double beta, double alpha, double betap)
+{
+  const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
+  __shared__ double D[256], P[256];
+  const size_t eo = (size_t) e * npnp, off = (size_t) k * stride + eo;
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  double rx[4], sx[4], ry[4], sy[4], gz[4], h0[4], h1[4], h2[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int i = lane + 64 * t;
+    if (i < npnp) {
+      D[i] = Dg[i];
+      P[i] = p[off + i];
+      rx[t] = ge[G_RX * npnp + i];
+      sx[t] = ge[G_SX * npnp + i];
+      ry[t] = ge[G_RY * npnp + i];
+      sy[t] = ge[G_SY * npnp + i];
+      h0[t] = H0[off + i];
+      h1[t] = H1[off + i];
+      h2[t] = nc == 3 ? H2[off + i] : 0.0;
+      gz[t] = nc == 3 ? dz_at (p, k, stride, eo + i, beta) : 0.0;
+    }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij >= npnp) break;
+    const int i = ij / np, j = ij % np;
+    double pr = 0.0, ps = 0.0;
+    for (int q = 0; q < np; q++) {
+      pr += P[i * np + q] * D[j * np + q];
+      ps += D[i * np + q] * P[q * np + j];
+    }
+    const double gx = pr * rx[t] + ps * sx[t], gy = pr * ry[t] + ps * sy[t];
+    double f = h0[t] * alpha;
+    f += betap * gx;
+    F0[off + ij] = f;
+    f = h1[t] * alpha;
+    f += betap * gy;
+    F1[off + ij] = f;
+    if (nc == 3) {
+      f = h2[t] * alpha;
+      f += betap * gz[t];
+      F2[off + ij] = f;
+    }
+  }
+}
+
 // AuxField::gradient(2) on Fourier planes: planes 0 and 1 zero,
 // Re_k <- -beta k Im_k, Im_k <- beta k Re_k.
 __global__ void k_gradz (const double* __restrict__ in, double* out, int nz,
@@ -182,26 +306,6 @@ __global__ void k_ew_wave (double* H, const double* us0, const double* us1,
   H[i] = h;
 }
 // setPForce (dns/integrate.cpp:242-257): f = (g0 + g1 [+ g2]) / dt
-__global__ void k_ew_div (double* f, const double* g0, const double* g1,
-                          const double* g2, double dt, size_t n)
-{
-  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = g0[i] + g1[i];
-  if (g2) s += g2[i];
-  f[i] = s / dt;
-}
-// project (dns/integrate.cpp:260-289): F = H * alpha + beta * grad p
-__global__ void k_ew_project (double* F, const double* H, const double* gp,
-                              double alpha, double beta, size_t n)
-{
-  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double f = H[i] * alpha;
-  f += beta * gp[i];
-  F[i] = f;
-}
-
 // Field::smooth: ds[k][g] = imass[g] sum Q4 f over the element-boundary
 // slots of global node g, then written back to every slot.
 __global__ void k_smooth_sum (int npnp, int nglobal, const int* __restrict__ gptr,
@@ -598,19 +702,20 @@ int dns_step (sx_dns* d, hipStream_t st)
                               hipMemcpyDeviceToDevice, st));
   // rollm (Uf)
   for (int i = 0; i < nc; i++) std::swap (d->Uf[0][i], d->Uf[1][i]);
-  // -- setPForce: div u^ / dt into F[0]
-  if ((rc = grad (d, d->H[0], d->F[0], 0, st))) return rc;
-  if ((rc = grad (d, d->H[1], d->F[1], 1, st))) return rc;
-  if (nc == 3 && (rc = grad (d, d->H[2], d->F[2], 2, st))) return rc;
-  EW (k_ew_div, d->tmp, d->F[0], d->F[1], nc == 3 ? d->F[2] : nullptr, d->dt, d->n);
+  // -- setPForce: div u^ / dt
+  const sx_mesh* m = d->mu;
+  hipLaunchKernelGGL (k_div_h, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo, m->d_D,
+                      d->H[0], d->H[1], nc == 3 ? d->H[2] : nullptr, d->tmp, d->S, d->beta,
+                      d->dt);
+  SX_HIP (hipGetLastError ());
   // -- pressure
   if ((rc = sx_field_solve (d->Mp, d->nz, d->S, d->tmp, d->p, plines, st))) return rc;
   // -- project: F_i = -u^_i/(dt nu) + grad_i p / nu
   const double alpha = -1.0 / (d->dt * d->nu), betap = 1.0 / d->nu;
-  for (int i = 0; i < nc; i++) {
-    if ((rc = grad (d, d->p, d->tmp, i, st))) return rc;
-    EW (k_ew_project, d->F[i], d->H[i], d->tmp, alpha, betap, d->n);
-  }
+  hipLaunchKernelGGL (k_project3, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, nc, m->d_geo,
+                      m->d_D, d->p, d->H[0], d->H[1], nc == 3 ? d->H[2] : nullptr, d->F[0],
+                      d->F[1], nc == 3 ? d->F[2] : nullptr, d->S, d->beta, alpha, betap);
+  SX_HIP (hipGetLastError ());
   // -- Us: store u^n and roll
   for (int i = 0; i < nc; i++) std::swap (d->Us[0][i], d->Us[1][i]);
   // -- viscous substep
