@@ -123,6 +123,8 @@ struct sx_msys {
   int* d_nsolve = nullptr;     // [nmodes]
   double* d_Bm = nullptr;      // [nmodes][nclass][next][nint]  hbi*inv(hii)
   double* d_Hinv = nullptr;    // [nmodes][nclass][nint][nint]  inv(hii)
+  double* d_Mrec = nullptr;    // [nmodes][nclass][npnp][pad4 nint] [-Bm; Hinv] (recovery)
+  double* d_Mcon = nullptr;    // [nmodes][nclass][nint][pad4 next] Bm^T (condensation)
   float* d_Bm32 = nullptr;     // FP32 copies for SX_MIXED
   float* d_Hinv32 = nullptr;
   double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband

@@ -332,6 +332,55 @@ __device__ __forceinline__ void stage (int n, Load load, Store store)
   }
 }
 
+// dst[i] = src[i] for i < n (n even, both 16-byte aligned): 16-byte loads,
+// sixteen in flight per thread
+__device__ __forceinline__ void copy16 (double* dst, const double* __restrict__ src, int n)
+{
+  constexpr int CU = 16;
+  const double2* s2 = (const double2*) src;
+  double2* d2 = (double2*) dst;
+  const int n2 = n / 2;
+  for (int i0 = threadIdx.x; i0 < n2; i0 += CU * CT_THREADS) {
+    double2 v[CU];
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int i = i0 + u * CT_THREADS;
+      v[u] = i < n2 ? s2[i] : make_double2 (0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < CU; u++) {
+      const int i = i0 + u * CT_THREADS;
+      if (i < n2) d2[i] = v[u];
+    }
+  }
+}
+
+// The tiled kernels' operands per (mode, class), in their LDS layout:
+// Mrec[k][a] = -Bm[k][a] (k < next), Hinv[k-next][a] (k >= next), rows
+// padded to pad4 (nint); Mcon[q][a] = Bm[a][q], rows padded to pad4 (next)
+__global__ void k_tiled_ops (int np, int nmc, const double* __restrict__ Bm,
+                             const double* __restrict__ Hinv, double* Mrec, double* Mcon)
+{
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const int NI = pad4 (nint), NE = pad4 (next);
+  const size_t nr = (size_t) npnp * NI, nc = (size_t) nint * NE;
+  for (size_t mc = blockIdx.y; mc < (size_t) nmc; mc += gridDim.y)
+  for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < nr + nc;
+       i += (size_t) gridDim.x * blockDim.x) {
+    if (i < nr) {
+      const int k = (int) (i / NI), a = (int) (i % NI);
+      double v = 0.0;
+      if (a < nint)
+        v = k < next ? -Bm[(mc * next + k) * nint + a] : Hinv[(mc * nint + k - next) * nint + a];
+      Mrec[mc * nr + i] = v;
+    } else {
+      const size_t j = i - nr;
+      const int q = (int) (j / NE), a = (int) (j % NE);
+      Mcon[mc * nc + j] = a < next ? Bm[(mc * next + a) * nint + q] : 0.0;
+    }
+  }
+}
+
 // acc[r][j] += sum_k A[k][a0 + r] X[k][e0 + j], k < K (A rows of stride NA)
 __device__ __forceinline__ void tile_4x2 (const double* __restrict__ A, int NA,
                                           const double* __restrict__ X, int K,
@@ -386,7 +435,7 @@ __global__ __launch_bounds__(64) void k_weak_ess (
 __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
-    const double* __restrict__ geo, const double* __restrict__ Bm, double* F, int stride,
+    const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
     int nz, int base_mode, int weak)
 {
   extern __shared__ double sm[];
@@ -400,11 +449,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
   double* Ws = As + (size_t) nint * NE;              // [npnp][CG_EP]: w in emap order
   int* emap = (int*) (Ws + (size_t) npnp * CG_EP);
   const int tid = threadIdx.x;
-  const double* B = Bm + ((size_t) mode * nclass + c) * next * nint;
-  stage (next * nint, [&] (int i) { return B[i]; },
-         [&] (int i, double v) { As[(i % nint) * NE + i / nint] = v; });
-  for (int i = tid; i < nint * (NE - next); i += CT_THREADS)
-    As[(i / (NE - next)) * NE + next + i % (NE - next)] = 0.0;
+  copy16 (As, Mcon + ((size_t) mode * nclass + c) * nint * NE, nint * NE);
   for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
   for (int i = tid; i < npnp * (CG_EP - nep); i += CT_THREADS)
     Ws[(i / (CG_EP - nep)) * CG_EP + nep + i % (CG_EP - nep)] = 0.0;
@@ -449,7 +494,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
     const int* __restrict__ bmask, const int* __restrict__ emask,
-    const double* __restrict__ Bm, const double* __restrict__ Hinv,
+    const double* __restrict__ Mrec,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
     int nz, int base_mode, int rnr, int roff)
@@ -467,12 +512,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
   int* pm = emap + npnp;                             // row-major position -> emap index
   const int tid = threadIdx.x, ns = nsolve[mode];
   const size_t mc = (size_t) mode * nclass + c;
-  stage (next * nint, [&] (int i) { return -Bm[mc * next * nint + i]; },
-         [&] (int i, double v) { Ms[(i / nint) * NI + i % nint] = v; });
-  stage (nint * nint, [&] (int i) { return Hinv[mc * nint * nint + i]; },
-         [&] (int i, double v) { Ms[(next + i / nint) * NI + i % nint] = v; });
-  for (int i = tid; i < npnp * (NI - nint); i += CT_THREADS)
-    Ms[(i / (NI - nint)) * NI + nint + i % (NI - nint)] = 0.0;
+  copy16 (Ms, Mrec + mc * npnp * NI, npnp * NI);
   for (int i = tid; i < npnp; i += CT_THREADS) {
     const int r = emap_g[i];
     emap[i] = r;
@@ -689,7 +729,7 @@ extern "C" int sx_timing_read (double* ms, int* count, int nphase)
 extern "C" int sx_msys_destroy (sx_msys_t s)
 {
   if (!s) return SX_OK;
-  void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Bm32,
+  void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Mrec, s->d_Mcon, s->d_Bm32,
                   s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
                   s->d_work, s->d_pc, s->d_tw, s->d_rhsb};
   for (void* p : ptrs) if (p) (void) hipFree (p);
@@ -719,6 +759,8 @@ __global__ void k_to_f32 (const double* a, float* b, size_t n)
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = (float) a[i];
 }
+
+namespace { bool tiled_ok (const sx_mesh* m); }
 
 extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                                int base_mode, int num_modes, int precision,
@@ -780,6 +822,16 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                       m->nclass, m->cyl, lambda2, s->d_betak2, m->d_cgeo, m->d_D,
                       m->d_emap, d_ws, d_S, s->d_Bm, s->d_Hinv, d_status + num_modes);
   TRY (hipGetLastError ());
+  if (tiled_ok (m)) {
+    const size_t per = (size_t) m->npnp * pad4 (nint) + (size_t) nint * pad4 (next);
+    TRY (hipMalloc (&s->d_Mrec, sizeof (double) * nmc * m->npnp * pad4 (nint)));
+    TRY (hipMalloc (&s->d_Mcon, sizeof (double) * nmc * nint * pad4 (next)));
+    hipLaunchKernelGGL (k_tiled_ops, dim3 ((unsigned) ((per + 255) / 256),
+                                          (unsigned) std::min<size_t> (nmc, 65535)),
+                        dim3 (256), 0, st, m->np, (int) nmc, s->d_Bm, s->d_Hinv, s->d_Mrec,
+                        s->d_Mcon);
+    TRY (hipGetLastError ());
+  }
   if (m->npairs)
     hipLaunchKernelGGL (k_band_assemble, dim3 ((m->npairs + 255) / 256, num_modes),
                         dim3 (256), 0, st, m->npairs, m->d_ppos, m->d_pmn, m->d_pptr,
@@ -880,7 +932,7 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (k_condense_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
                       lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_emask,
-                      m->d_geo, s->d_Bm, F, stride, nz, s->base_mode, weak);
+                      m->d_geo, s->d_Mcon, F, stride, nz, s->base_mode, weak);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -903,7 +955,7 @@ int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, c
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (k_recover_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds,
                       st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_btog,
-                      m->d_bmask, m->d_emask, s->d_Bm, s->d_Hinv, G, m->nglobal, F, U, stride,
+                      m->d_bmask, m->d_emask, s->d_Mrec, G, m->nglobal, F, U, stride,
                       s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
   SX_HIP (hipGetLastError ());
   return SX_OK;
