@@ -125,6 +125,7 @@ struct sx_msys {
   double* d_Hinv = nullptr;    // [nmodes][nclass][nint][nint]  inv(hii)
   double* d_Mrec = nullptr;    // [nmodes][nclass][npnp][pad4 nint] [-Bm; Hinv] (recovery)
   double* d_Mcon = nullptr;    // [nmodes][nclass][nint][pad4 next] Bm^T (condensation)
+  int tile_mf = 0;             // 1: tiled kernels on FP64 MFMA (operands padded to 16)
   float* d_Bm32 = nullptr;     // FP32 copies for SX_MIXED
   float* d_Hinv32 = nullptr;
   double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband

@@ -310,6 +310,43 @@ constexpr int CT_THREADS = 256;
 constexpr int CG_EP = 34;          // element-plane stride in LDS (>= 2 ETILE, bank-shifted)
 
 __host__ __device__ constexpr int pad4 (int x) { return (x + 3) & ~3; }
+__host__ __device__ constexpr int pad16 (int x) { return (x + 15) & ~15; }
+
+// Operand layout of the tiled kernels.  MF (FP64 MFMA 16x16x4 tiles): rows
+// of the matrix operand padded to 16 columns and its k range to a multiple
+// of 4 (zeros); vector path: columns padded to 4.
+//   recovery      M[k][a], k < KR(npnp), a < NA(nint)
+//   condensation  M[q][a], q < KR(nint), a < NA(next)
+struct TileShape {
+  int NA, KR;
+};
+__host__ __device__ inline TileShape rec_shape (int np, bool mf)
+{
+  const int npnp = np * np, nint = npnp - 4 * (np - 1);
+  return mf ? TileShape {pad16 (nint), pad4 (npnp)} : TileShape {pad4 (nint), npnp};
+}
+__host__ __device__ inline TileShape con_shape (int np, bool mf)
+{
+  const int next = 4 * (np - 1), nint = np * np - next;
+  return mf ? TileShape {pad16 (next), pad4 (nint)} : TileShape {pad4 (next), nint};
+}
+typedef double v4d __attribute__ ((ext_vector_type (4)));
+
+// D += A B over k < K for one 16x16 output tile (v_mfma_f64_16x16x4f64):
+// lane l supplies A[l%16][k + l/16] = M[k + l/16][a0 + l%16] and
+// B[k + l/16][l%16] = X[k + l/16][e0 + l%16]; D[4r + l/16][l%16] is
+// component r of lane l
+__device__ __forceinline__ v4d mfma_tile (const double* __restrict__ M, int NA,
+                                          const double* __restrict__ X, int K, int lane)
+{
+  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  const double* A = M + (lane >> 4) * NA + (lane & 15);
+  const double* B = X + (lane >> 4) * CG_EP + (lane & 15);
+  for (int k = 0; k < K; k += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc,
+                                                0, 0, 0);
+  return acc;
+}
 
 // dst(i) <- src(i) for i < n with SU loads in flight per thread (the loads
 // of a batch are issued before any of them is waited for)
@@ -355,28 +392,28 @@ __device__ __forceinline__ void copy16 (double* dst, const double* __restrict__ 
   }
 }
 
-// The tiled kernels' operands per (mode, class), in their LDS layout:
-// Mrec[k][a] = -Bm[k][a] (k < next), Hinv[k-next][a] (k >= next), rows
-// padded to pad4 (nint); Mcon[q][a] = Bm[a][q], rows padded to pad4 (next)
-__global__ void k_tiled_ops (int np, int nmc, const double* __restrict__ Bm,
+// The tiled kernels' operands per (mode, class), in their LDS layout
+// (TileShape; padding zero): Mrec[k][a] = -Bm[k][a] (k < next),
+// Hinv[k-next][a] (next <= k < npnp); Mcon[q][a] = Bm[a][q]
+__global__ void k_tiled_ops (int np, int nmc, int mf, const double* __restrict__ Bm,
                              const double* __restrict__ Hinv, double* Mrec, double* Mcon)
 {
   const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
-  const int NI = pad4 (nint), NE = pad4 (next);
-  const size_t nr = (size_t) npnp * NI, nc = (size_t) nint * NE;
+  const TileShape R = rec_shape (np, mf), Cs = con_shape (np, mf);
+  const size_t nr = (size_t) R.KR * R.NA, nc = (size_t) Cs.KR * Cs.NA;
   for (size_t mc = blockIdx.y; mc < (size_t) nmc; mc += gridDim.y)
   for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < nr + nc;
        i += (size_t) gridDim.x * blockDim.x) {
     if (i < nr) {
-      const int k = (int) (i / NI), a = (int) (i % NI);
+      const int k = (int) (i / R.NA), a = (int) (i % R.NA);
       double v = 0.0;
-      if (a < nint)
+      if (a < nint && k < npnp)
         v = k < next ? -Bm[(mc * next + k) * nint + a] : Hinv[(mc * nint + k - next) * nint + a];
       Mrec[mc * nr + i] = v;
     } else {
       const size_t j = i - nr;
-      const int q = (int) (j / NE), a = (int) (j % NE);
-      Mcon[mc * nc + j] = a < next ? Bm[(mc * next + a) * nint + q] : 0.0;
+      const int q = (int) (j / Cs.NA), a = (int) (j % Cs.NA);
+      Mcon[mc * nc + j] = (a < next && q < nint) ? Bm[(mc * next + a) * nint + q] : 0.0;
     }
   }
 }
@@ -432,6 +469,7 @@ __global__ __launch_bounds__(64) void k_weak_ess (
 
 // f (row-major; already weak on essential elements, else -M f is formed
 // here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i.
+template <bool MF>
 __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
@@ -439,20 +477,23 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int nz, int base_mode, int weak)
 {
   extern __shared__ double sm[];
-  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next, NE = pad4 (next);
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const TileShape Cs = con_shape (np, MF);
+  const int NE = Cs.NA, WR = next + Cs.KR;           // Ws rows (the k range padded)
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
   const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
             cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
-  double* As = sm;                                   // [nint][NE]: As[q][a] = Bm[a][q]
-  double* Ws = As + (size_t) nint * NE;              // [npnp][CG_EP]: w in emap order
-  int* emap = (int*) (Ws + (size_t) npnp * CG_EP);
+  double* As = sm;                                   // [KR][NE]: As[q][a] = Bm[a][q]
+  double* Ws = As + (size_t) Cs.KR * NE;             // [WR][CG_EP]: w in emap order
+  int* emap = (int*) (Ws + (size_t) WR * CG_EP);
   const int tid = threadIdx.x;
-  copy16 (As, Mcon + ((size_t) mode * nclass + c) * nint * NE, nint * NE);
+  copy16 (As, Mcon + ((size_t) mode * nclass + c) * Cs.KR * NE, Cs.KR * NE);
   for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
   for (int i = tid; i < npnp * (CG_EP - nep); i += CT_THREADS)
     Ws[(i / (CG_EP - nep)) * CG_EP + nep + i % (CG_EP - nep)] = 0.0;
+  for (int i = npnp * CG_EP + tid; i < WR * CG_EP; i += CT_THREADS) Ws[i] = 0.0;
   __syncthreads ();
   // w = -Q8 f in emap order (f as given on essential elements / weak input)
   stage (nep * npnp,
@@ -464,20 +505,38 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
          },
          [&] (int i, double v) { Ws[(i % npnp) * CG_EP + i / npnp] = v; });
   __syncthreads ();
-  const int nag = NE / 4, neg = (nep + 1) / 2;
-  for (int task = tid; task < nag * neg; task += CT_THREADS) {
-    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
-    double acc[4][2] = {};
-    tile_4x2 (As + a0, NE, Ws + (size_t) next * CG_EP + e0, nint, acc);
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int ep = e0 + j;
+  if constexpr (MF) {
+    // 16x16 tiles of F_b (a x element-plane), a wavefront each in turn
+    const int wave = tid >> 6, lane = tid & 63, nat = NE / 16, ntile = nat * ((nep + 15) / 16);
+    for (int t = wave; t < ntile; t += CT_THREADS / 64) {
+      const int a0 = 16 * (t % nat), e0 = 16 * (t / nat);
+      const v4d acc = mfma_tile (As + a0, NE, Ws + (size_t) next * CG_EP + e0, Cs.KR, lane);
+      const int ep = e0 + (lane & 15);
       if (ep >= nep) continue;
       const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
       double* out = F + (size_t) k * stride + (size_t) e * npnp;
 #pragma unroll
-      for (int r = 0; r < 4; r++)
-        if (a0 + r < next) out[a0 + r] = Ws[(a0 + r) * CG_EP + ep] - acc[r][j];
+      for (int r = 0; r < 4; r++) {
+        const int a = a0 + 4 * r + (lane >> 4);
+        if (a < next) out[a] = Ws[a * CG_EP + ep] - acc[r];
+      }
+    }
+  } else {
+    const int nag = NE / 4, neg = (nep + 1) / 2;
+    for (int task = tid; task < nag * neg; task += CT_THREADS) {
+      const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
+      double acc[4][2] = {};
+      tile_4x2 (As + a0, NE, Ws + (size_t) next * CG_EP + e0, nint, acc);
+#pragma unroll
+      for (int j = 0; j < 2; j++) {
+        const int ep = e0 + j;
+        if (ep >= nep) continue;
+        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+        double* out = F + (size_t) k * stride + (size_t) e * npnp;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+          if (a0 + r < next) out[a0 + r] = Ws[(a0 + r) * CG_EP + ep] - acc[r][j];
+      }
     }
   }
   for (int i = tid; i < nep * nint; i += CT_THREADS) {
@@ -490,6 +549,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
 // essential values (Element::global2localSC src/element.cpp:368-407,
 // Field::setEssential src/field.cpp:1105-1124).  One operand
 // M[k][a] = (k < next ? -Bm[k][a] : Hinv[k-next][a]) against [u_b | F_i].
+template <bool MF>
 __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -500,19 +560,22 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int nz, int base_mode, int rnr, int roff)
 {
   extern __shared__ double sm[];
-  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next, NI = pad4 (nint);
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const TileShape R = rec_shape (np, MF);
+  const int NI = R.NA;
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
   const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
             cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
-  double* Ms = sm;                                   // [npnp][NI]
-  double* Xs = Ms + (size_t) npnp * NI;              // [npnp][CG_EP]: u_b | F_i, then u
-  int* emap = (int*) (Xs + (size_t) npnp * CG_EP);
+  double* Ms = sm;                                   // [KR][NI]
+  double* Xs = Ms + (size_t) R.KR * NI;              // [KR][CG_EP]: u_b | F_i, then u
+  int* emap = (int*) (Xs + (size_t) R.KR * CG_EP);
   int* pm = emap + npnp;                             // row-major position -> emap index
   const int tid = threadIdx.x, ns = nsolve[mode];
   const size_t mc = (size_t) mode * nclass + c;
-  copy16 (Ms, Mrec + mc * npnp * NI, npnp * NI);
+  copy16 (Ms, Mrec + mc * R.KR * NI, R.KR * NI);
+  for (int i = npnp * CG_EP + tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0;
   for (int i = tid; i < npnp; i += CT_THREADS) {
     const int r = emap_g[i];
     emap[i] = r;
@@ -544,25 +607,51 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
          },
          [&] (int i, double v) { Xs[(next + i % nint) * CG_EP + i / nint] = v; });
   __syncthreads ();
-  const int nag = NI / 4, neg = (nep + 1) / 2, ntask = nag * neg;
-  // at most two tasks per thread (nint <= 100, nep <= 32): results are
-  // held in registers until every thread has finished reading Xs
-  double acc[2][4][2] = {};
-  for (int s = 0; s < 2; s++) {
-    const int task = tid + s * CT_THREADS;
-    if (task < ntask)
-      tile_4x2 (Ms + 4 * (task % nag), NI, Xs + 2 * (task / nag), npnp, acc[s]);
-  }
-  __syncthreads ();
-  for (int s = 0; s < 2; s++) {
-    const int task = tid + s * CT_THREADS;
-    if (task >= ntask) continue;
-    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
+  if constexpr (MF) {
+    // 16x16 tiles of u_i (a x element-plane), at most 4 per wavefront
+    // (NI <= 112, nep <= 32), held until every wavefront has read Xs
+    const int wave = tid >> 6, lane = tid & 63, nat = NI / 16, ntile = nat * ((nep + 15) / 16);
+    v4d acc[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++)
+    for (int s = 0; s < 4; s++) {
+      const int t = wave + 4 * s;
+      acc[s] = v4d {0.0, 0.0, 0.0, 0.0};
+      if (t < ntile)
+        acc[s] = mfma_tile (Ms + 16 * (t % nat), NI, Xs + 16 * (t / nat), R.KR, lane);
+    }
+    __syncthreads ();
 #pragma unroll
-      for (int j = 0; j < 2; j++)
-        if (a0 + r < nint && e0 + j < nep) Xs[(next + a0 + r) * CG_EP + e0 + j] = acc[s][r][j];
+    for (int s = 0; s < 4; s++) {
+      const int t = wave + 4 * s;
+      if (t >= ntile) continue;
+      const int a0 = 16 * (t % nat), ep = 16 * (t / nat) + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int a = a0 + 4 * r + (lane >> 4);
+        if (a < nint && ep < nep) Xs[(next + a) * CG_EP + ep] = acc[s][r];
+      }
+    }
+  } else {
+    const int nag = NI / 4, neg = (nep + 1) / 2, ntask = nag * neg;
+    // at most two tasks per thread (nint <= 100, nep <= 32): results are
+    // held in registers until every thread has finished reading Xs
+    double acc[2][4][2] = {};
+    for (int s = 0; s < 2; s++) {
+      const int task = tid + s * CT_THREADS;
+      if (task < ntask)
+        tile_4x2 (Ms + 4 * (task % nag), NI, Xs + 2 * (task / nag), npnp, acc[s]);
+    }
+    __syncthreads ();
+    for (int s = 0; s < 2; s++) {
+      const int task = tid + s * CT_THREADS;
+      if (task >= ntask) continue;
+      const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          if (a0 + r < nint && e0 + j < nep) Xs[(next + a0 + r) * CG_EP + e0 + j] = acc[s][r][j];
+    }
   }
 #pragma unroll
   for (int u = 0; u < BO; u++) {
@@ -760,7 +849,7 @@ __global__ void k_to_f32 (const double* a, float* b, size_t n)
   if (i < n) b[i] = (float) a[i];
 }
 
-namespace { bool tiled_ok (const sx_mesh* m); }
+namespace { bool tiled_ok (const sx_mesh* m); bool tiled_mfma (const sx_mesh* m); }
 
 extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                                int base_mode, int num_modes, int precision,
@@ -823,13 +912,16 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                       m->d_emap, d_ws, d_S, s->d_Bm, s->d_Hinv, d_status + num_modes);
   TRY (hipGetLastError ());
   if (tiled_ok (m)) {
-    const size_t per = (size_t) m->npnp * pad4 (nint) + (size_t) nint * pad4 (next);
-    TRY (hipMalloc (&s->d_Mrec, sizeof (double) * nmc * m->npnp * pad4 (nint)));
-    TRY (hipMalloc (&s->d_Mcon, sizeof (double) * nmc * nint * pad4 (next)));
-    hipLaunchKernelGGL (k_tiled_ops, dim3 ((unsigned) ((per + 255) / 256),
+    s->tile_mf = tiled_mfma (m) ? 1 : 0;
+    if (const char* e = std::getenv ("SX_TILE_MFMA")) s->tile_mf = s->tile_mf && std::atoi (e);
+    const TileShape R = rec_shape (m->np, s->tile_mf), Cs = con_shape (m->np, s->tile_mf);
+    const size_t nr = (size_t) R.KR * R.NA, nc = (size_t) Cs.KR * Cs.NA;
+    TRY (hipMalloc (&s->d_Mrec, sizeof (double) * nmc * nr));
+    TRY (hipMalloc (&s->d_Mcon, sizeof (double) * nmc * nc));
+    hipLaunchKernelGGL (k_tiled_ops, dim3 ((unsigned) ((nr + nc + 255) / 256),
                                           (unsigned) std::min<size_t> (nmc, 65535)),
-                        dim3 (256), 0, st, m->np, (int) nmc, s->d_Bm, s->d_Hinv, s->d_Mrec,
-                        s->d_Mcon);
+                        dim3 (256), 0, st, m->np, (int) nmc, s->tile_mf, s->d_Bm, s->d_Hinv,
+                        s->d_Mrec, s->d_Mcon);
     TRY (hipGetLastError ());
   }
   if (m->npairs)
@@ -888,19 +980,32 @@ static int check_solve_args (sx_msys_t s, int nz, int stride, const void* f,
 
 namespace {
 
-size_t tiled_lds (const sx_mesh* m, bool recover)
+size_t tiled_lds (const sx_mesh* m, bool recover, bool mf)
 {
-  const size_t next = m->next, nint = m->nint, npnp = m->npnp;
-  if (recover)
-    return sizeof (double) * (npnp * pad4 ((int) nint) + npnp * CG_EP) + sizeof (int) * 2 * npnp;
-  return sizeof (double) * (nint * pad4 ((int) next) + npnp * CG_EP) + sizeof (int) * npnp;
+  const size_t next = m->next, npnp = m->npnp;
+  if (recover) {
+    const TileShape R = rec_shape (m->np, mf);
+    return sizeof (double) * ((size_t) R.KR * R.NA + (size_t) R.KR * CG_EP) +
+           sizeof (int) * 2 * npnp;
+  }
+  const TileShape Cs = con_shape (m->np, mf);
+  return sizeof (double) * ((size_t) Cs.KR * Cs.NA + (next + Cs.KR) * CG_EP) +
+         sizeof (int) * npnp;
 }
 
 // the tiled kernels hold at most 2 product tasks and 6 boundary values per
 // thread: nint <= 100 and next <= 48 (np <= 12, which the LDS bound implies)
 bool tiled_ok (const sx_mesh* m)
 {
-  return tiled_lds (m, true) <= 160 * 1024 && m->nint <= 100 && 2 * ETILE * m->next <= 6 * 256;
+  return tiled_lds (m, true, false) <= 160 * 1024 && m->nint <= 100 &&
+         2 * ETILE * m->next <= 6 * 256;
+}
+
+// the MFMA form where its (16-padded) operands fit the LDS as well
+bool tiled_mfma (const sx_mesh* m)
+{
+  return tiled_ok (m) && tiled_lds (m, true, true) <= 160 * 1024 &&
+         tiled_lds (m, false, true) <= 160 * 1024;
 }
 
 int n_mode_blocks (int nz) { return nz == 1 ? 1 : nz / 2; }
@@ -927,10 +1032,12 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
     hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
                         s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
                         m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
-  const size_t lds = tiled_lds (m, false);
-  SX_HIP (hipFuncSetAttribute ((const void*) k_condense_t,
+  const bool mf = s->tile_mf;
+  const size_t lds = tiled_lds (m, false, mf);
+  auto kern = mf ? k_condense_t<true> : k_condense_t<false>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  hipLaunchKernelGGL (k_condense_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
                       lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_emask,
                       m->d_geo, s->d_Mcon, F, stride, nz, s->base_mode, weak);
   SX_HIP (hipGetLastError ());
@@ -950,10 +1057,12 @@ int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, c
     SX_HIP (hipGetLastError ());
     return SX_OK;
   }
-  const size_t lds = tiled_lds (m, true);
-  SX_HIP (hipFuncSetAttribute ((const void*) k_recover_t,
+  const bool mf = s->tile_mf;
+  const size_t lds = tiled_lds (m, true, mf);
+  auto kern = mf ? k_recover_t<true> : k_recover_t<false>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  hipLaunchKernelGGL (k_recover_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds,
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds,
                       st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_btog,
                       m->d_bmask, m->d_emask, s->d_Mrec, G, m->nglobal, F, U, stride,
                       s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);

@@ -101,3 +101,27 @@ def test_base_mode_two_solves_every_plane(torch_cuda, oracle):
                                                  else rng.standard_normal(m.np))
     ug, _ = _solve(torch_cuda, m, 0.5, 1.0, 4, F, base_mode=2, lines=lines)
     _check(oracle, m, 0.5, 1.0, F, ug, [0, 1, 2, 3], base_mode=2, lines=lines)
+
+
+@pytest.mark.parametrize("mfma", ["1", "0"])
+def test_tiled_condensation_mfma_and_vector(torch_cuda, oracle, monkeypatch, mfma):
+    """The class-tiled condensation/recovery on FP64 MFMA tiles (default where
+    the 16-padded operands fit the LDS) and on the vector FMA tiles
+    (SX_TILE_MFMA=0), N_P = 7 and 11, with Dirichlet and Neumann data."""
+    from semtex_amd.mesh import rect_mesh
+    from semtex_amd.problems import fourier_forcing
+    monkeypatch.setenv("SX_TILE_MFMA", mfma)
+    for np_ in (7, 11):
+        print("case: tiled kernels mfma", mfma, "np", np_, flush=True)
+        m = rect_mesh(5, 3, np_, 0.0, 2 * np.pi, -1.0, 1.0, bc=("D", "N", "N", "N"),
+                      periodic_x=True)
+        F = fourier_forcing(m, 4, seed=40 + np_)
+        rng = np.random.default_rng(np_)
+        lines = np.zeros((4, m.nbound * m.np))
+        for k in range(4):
+            for b in range(m.nbound):
+                xs, _ = m.side_xy(b)
+                lines[k, b * m.np:(b + 1) * m.np] = (np.cos((k + 1) * xs) if m.bnd_kind[b] == 0
+                                                     else rng.standard_normal(m.np))
+        ug, _ = _solve(torch_cuda, m, 1.5, 1.0, 4, F, lines=lines)
+        _check(oracle, m, 1.5, 1.0, F, ug, [0, 2, 3], lines=lines)
