@@ -57,8 +57,11 @@ def parse():
     p.add_argument("--batch-runs", type=int, default=20)
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-adjoint", action="store_true")
-    p.add_argument("--mixed-tol", type=float, default=1e-11,
-                   help="relative correction size that ends the mixed refinement")
+    p.add_argument("--mixed-tol", type=float, default=1e-6,
+                   help="relative correction size that ends the mixed refinement (each "
+                        "pass contracts the error by ~1e-7, so the pass that applies a "
+                        "correction below 1e-6 leaves ~1e-13; the error against the FP64 "
+                        "path is measured and reported)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only copy 256 MiB device to device 3 times (the reference "
                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")

@@ -70,6 +70,13 @@ int    fourier_z_fused (int nz, int S, const double* in, const double* mul, doub
 // synchronisation (sx_mixed.hip).
 int    field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
                                  const double* d_lines, int passes, hipStream_t st);
+// The mixed solve's FP32 element passes on MFMA tiles (systems with
+// d_Mrec32): X (weak, row-major) -> Fp [condensed F_b | F_i] in emap order;
+// Fp + band solution (d_rhs, 2 per mode) -> V row-major.
+int    condense32_tiled (const sx_msys* s, int nz, int stride, const double* X, double* Fp,
+                         hipStream_t st);
+int    recover32_tiled (const sx_msys* s, int nz, int stride, const double* Fp, double* V,
+                        hipStream_t st);
 // JACPCG set-up: the Jacobi preconditioner of every mode (sx_pcg.hip).
 int    pcg_setup (sx_msys* s, hipStream_t st);
 inline bool fourier_in_place (int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }
@@ -126,6 +133,8 @@ struct sx_msys {
   double* d_Mrec = nullptr;    // [nmodes][nclass][npnp][pad4 nint] [-Bm; Hinv] (recovery)
   double* d_Mcon = nullptr;    // [nmodes][nclass][nint][pad4 next] Bm^T (condensation)
   int tile_mf = 0;             // 1: tiled kernels on FP64 MFMA (operands padded to 16)
+  float* d_Mrec32 = nullptr;   // SX_MIXED with tile_mf: FP32 copies of d_Mrec, d_Mcon
+  float* d_Mcon32 = nullptr;
   float* d_Bm32 = nullptr;     // FP32 copies for SX_MIXED
   float* d_Hinv32 = nullptr;
   double* d_bandU = nullptr;   // [nmodes][band_stride] 'U' band, ld = nband

@@ -270,8 +270,12 @@ int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
   const sx_mesh* m = s->m;
   const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
   const int lstride = m->nbound * m->np;
-  hipLaunchKernelGGL (k_condense32, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
-                      m->d_emap, m->d_cls, s->d_Bm32, X, Fp, stride, nz, s->base_mode);
+  if (s->d_Mcon32) {
+    if (int rc = condense32_tiled (s, nz, stride, X, Fp, st)) return rc;
+  } else {
+    hipLaunchKernelGGL (k_condense32, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
+                        m->d_emap, m->d_cls, s->d_Bm32, X, Fp, stride, nz, s->base_mode);
+  }
   hipLaunchKernelGGL (k_assemble_m, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0, st,
                       m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw, Fp,
                       stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
@@ -279,9 +283,13 @@ int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
   SX_HIP (hipGetLastError ());
   int rc = band_solve (s, 2, st);
   if (rc) return rc;
-  hipLaunchKernelGGL (k_recover32, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
-                      m->d_emap, m->d_cls, m->d_btog, s->d_Bm32, s->d_Hinv32, Fp, V, stride,
-                      s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode);
+  if (s->d_Mrec32) {
+    if ((rc = recover32_tiled (s, nz, stride, Fp, V, st))) return rc;
+  } else {
+    hipLaunchKernelGGL (k_recover32, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->nclass,
+                        m->d_emap, m->d_cls, m->d_btog, s->d_Bm32, s->d_Hinv32, Fp, V, stride,
+                        s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode);
+  }
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }

@@ -392,6 +392,28 @@ __device__ __forceinline__ void copy16 (double* dst, const double* __restrict__ 
   }
 }
 
+// The same product in FP32 (v_mfma_f32_16x16x4f32, FP32 accumulation):
+// operands as for mfma_tile, but D[4 (l/16) + r][l%16] is component r of
+// lane l
+typedef float v4f __attribute__ ((ext_vector_type (4)));
+__device__ __forceinline__ v4f mfma_tile32 (const float* __restrict__ M, int NA,
+                                            const float* __restrict__ X, int K, int lane)
+{
+  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  const float* A = M + (lane >> 4) * NA + (lane & 15);
+  const float* B = X + (lane >> 4) * CG_EP + (lane & 15);
+  for (int k = 0; k < K; k += 4)
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc,
+                                                0, 0, 0);
+  return acc;
+}
+
+// dst[i] = src[i] for i < n floats (n a multiple of 4, 16-byte aligned)
+__device__ __forceinline__ void copy16f (float* dst, const float* __restrict__ src, int n)
+{
+  copy16 ((double*) dst, (const double*) src, n / 2);
+}
+
 // The tiled kernels' operands per (mode, class), in their LDS layout
 // (TileShape; padding zero): Mrec[k][a] = -Bm[k][a] (k < next),
 // Hinv[k-next][a] (next <= k < npnp); Mcon[q][a] = Bm[a][q]
@@ -666,6 +688,145 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
   }
 }
 
+// FP32 forms of the tiled condensation and recovery for the mixed solve's
+// inner passes (FP32 operands and MFMA tiles, FP32 accumulation; the
+// boundary values stay FP64).
+// Condensation: X (row-major, already weak) -> Fp = [w_b - Bm w_i | w_i] in
+// emap order.
+__global__ __launch_bounds__(CT_THREADS) void k_condense32_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const float* __restrict__ Mcon,
+    const double* __restrict__ X, double* Fp, int stride, int nz, int base_mode)
+{
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const TileShape Cs = con_shape (np, true);
+  const int NE = Cs.NA;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Wb = sm;                                   // [next][CG_EP] w_b (FP64)
+  float* As = (float*) (Wb + (size_t) next * CG_EP); // [KR][NE]
+  float* Wi = As + (size_t) Cs.KR * NE;              // [KR][CG_EP] w_i (FP32)
+  int* emap = (int*) (Wi + (size_t) Cs.KR * CG_EP);
+  const int tid = threadIdx.x;
+  copy16f (As, Mcon + ((size_t) mode * nclass + c) * Cs.KR * NE, Cs.KR * NE);
+  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  for (int i = tid; i < Cs.KR * CG_EP; i += CT_THREADS) Wi[i] = 0.0f;
+  __syncthreads ();
+  stage (nep * npnp,
+         [&] (int i) {
+           const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+           return X[(size_t) k * stride + (size_t) e * npnp + emap[a]];
+         },
+         [&] (int i, double v) {
+           const int ep = i / npnp, a = i % npnp;
+           if (a < next) {
+             Wb[a * CG_EP + ep] = v;
+           } else {
+             Wi[(a - next) * CG_EP + ep] = (float) v;
+             const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+             Fp[(size_t) k * stride + (size_t) e * npnp + a] = v;
+           }
+         });
+  __syncthreads ();
+  const int wave = tid >> 6, lane = tid & 63, nat = NE / 16, ntile = nat * ((nep + 15) / 16);
+  for (int t = wave; t < ntile; t += CT_THREADS / 64) {
+    const int a0 = 16 * (t % nat), ep = 16 * (t / nat) + (lane & 15);
+    const v4f acc = mfma_tile32 (As + a0, NE, Wi + 16 * (t / nat), Cs.KR, lane);
+    if (ep >= nep) continue;
+    const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    double* out = Fp + (size_t) k * stride + (size_t) e * npnp;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int a = a0 + 4 * (lane >> 4) + r;
+      if (a < next) out[a] = Wb[a * CG_EP + ep] - (double) acc[r];
+    }
+  }
+}
+
+// Recovery: V (row-major) = [u_b | Hinv^T F_i - Bm^T u_b] with u_b from the
+// band solution (no essential insertion: the correction's essential values
+// are zero).
+__global__ __launch_bounds__(CT_THREADS) void k_recover32_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const float* __restrict__ Mrec, const double* __restrict__ Fp, double* V, int stride,
+    const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax, int nz,
+    int base_mode)
+{
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const TileShape R = rec_shape (np, true);
+  const int NI = R.NA;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Ub = sm;                                   // [next][CG_EP] u_b (FP64)
+  float* Ms = (float*) (Ub + (size_t) next * CG_EP); // [KR][NI]
+  float* Xs = Ms + (size_t) R.KR * NI;               // [KR][CG_EP] u_b | F_i, then u_i
+  int* pm = (int*) (Xs + (size_t) R.KR * CG_EP);     // row-major position -> emap index
+  const int tid = threadIdx.x, ns = nsolve[mode];
+  copy16f (Ms, Mrec + ((size_t) mode * nclass + c) * R.KR * NI, R.KR * NI);
+  for (int i = tid; i < npnp; i += CT_THREADS) pm[emap_g[i]] = i;
+  for (int i = tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0f;
+  __syncthreads ();
+  stage (nep * npnp,
+         [&] (int i) {
+           const int ep = i / npnp, a = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+           if (a < next) {
+             const int gg = btog[(size_t) e * next + a];
+             return gg < ns ? rhs[((size_t) mode * 2 + (k & 1)) * nmax + gg] : 0.0;
+           }
+           return Fp[(size_t) k * stride + (size_t) e * npnp + a];
+         },
+         [&] (int i, double v) {
+           const int ep = i / npnp, a = i % npnp;
+           Xs[a * CG_EP + ep] = (float) v;
+           if (a < next) Ub[a * CG_EP + ep] = v;
+         });
+  __syncthreads ();
+  const int wave = tid >> 6, lane = tid & 63, nat = NI / 16, ntile = nat * ((nep + 15) / 16);
+  v4f acc[4];
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int t = wave + 4 * s;
+    acc[s] = v4f {0.0f, 0.0f, 0.0f, 0.0f};
+    if (t < ntile) acc[s] = mfma_tile32 (Ms + 16 * (t % nat), NI, Xs + 16 * (t / nat), R.KR, lane);
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int s = 0; s < 4; s++) {
+    const int t = wave + 4 * s;
+    if (t >= ntile) continue;
+    const int a0 = 16 * (t % nat), ep = 16 * (t / nat) + (lane & 15);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int a = a0 + 4 * (lane >> 4) + r;
+      if (a < nint && ep < nep) Xs[(next + a) * CG_EP + ep] = acc[s][r];
+    }
+  }
+  __syncthreads ();
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, r = i % npnp, e = elist[t0 + ep / npm], k = k0 + ep % npm;
+    const int a = pm[r];
+    V[(size_t) k * stride + (size_t) e * npnp + r] =
+        a < next ? Ub[a * CG_EP + ep] : (double) Xs[a * CG_EP + ep];
+  }
+}
+
+// FP32 copies of the MFMA-layout operands (mixed systems)
+__global__ void k_ops32 (const double* __restrict__ a, float* b, size_t n)
+{
+  for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t) gridDim.x * blockDim.x)
+    b[i] = (float) a[i];
+}
+
 __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
                             const int* __restrict__ gsrc,
                             const int* __restrict__ nptr,
@@ -818,7 +979,8 @@ extern "C" int sx_timing_read (double* ms, int* count, int nphase)
 extern "C" int sx_msys_destroy (sx_msys_t s)
 {
   if (!s) return SX_OK;
-  void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Mrec, s->d_Mcon, s->d_Bm32,
+  void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Mrec, s->d_Mcon,
+                  s->d_Mrec32, s->d_Mcon32, s->d_Bm32,
                   s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
                   s->d_work, s->d_pc, s->d_tw, s->d_rhsb};
   for (void* p : ptrs) if (p) (void) hipFree (p);
@@ -949,6 +1111,17 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                  std::to_string (status[md]) + ")");
       return fail (SX_ERR_FACTOR);
     }
+  if (precision == SX_MIXED && s->tile_mf) {
+    const TileShape R = rec_shape (m->np, true), Cs = con_shape (m->np, true);
+    const size_t nr = nmc * R.KR * R.NA, nc = nmc * Cs.KR * Cs.NA;
+    TRY (hipMalloc (&s->d_Mrec32, sizeof (float) * nr));
+    TRY (hipMalloc (&s->d_Mcon32, sizeof (float) * nc));
+    hipLaunchKernelGGL (k_ops32, dim3 ((unsigned) std::min<size_t> ((nr + 255) / 256, 65535)),
+                        dim3 (256), 0, st, s->d_Mrec, s->d_Mrec32, nr);
+    hipLaunchKernelGGL (k_ops32, dim3 ((unsigned) std::min<size_t> ((nc + 255) / 256, 65535)),
+                        dim3 (256), 0, st, s->d_Mcon, s->d_Mcon32, nc);
+    TRY (hipGetLastError ());
+  }
   if (precision == SX_MIXED) {
     TRY (hipMalloc (&s->d_Bm32, sizeof (float) * std::max<size_t> (nmc * next * nint, 1)));
     TRY (hipMalloc (&s->d_Hinv32, sizeof (float) * std::max<size_t> (nmc * nint * nint, 1)));
@@ -1143,6 +1316,41 @@ int msys_reserve_rhs (sx_msys* s, int nr)
   s->d_rhs = r;
   s->d_G = g;
   s->rhs_cap = nr;
+  return SX_OK;
+}
+
+int condense32_tiled (const sx_msys* s, int nz, int stride, const double* X, double* Fp,
+                      hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const TileShape Cs = con_shape (m->np, true);
+  const size_t lds = sizeof (double) * m->next * CG_EP +
+                     sizeof (float) * ((size_t) Cs.KR * Cs.NA + (size_t) Cs.KR * CG_EP) +
+                     sizeof (int) * m->npnp;
+  SX_HIP (hipFuncSetAttribute ((const void*) k_condense32_t,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_condense32_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
+                      lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, s->d_Mcon32,
+                      X, Fp, stride, nz, s->base_mode);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+int recover32_tiled (const sx_msys* s, int nz, int stride, const double* Fp, double* V,
+                     hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const TileShape R = rec_shape (m->np, true);
+  const size_t lds = sizeof (double) * m->next * CG_EP +
+                     sizeof (float) * ((size_t) R.KR * R.NA + (size_t) R.KR * CG_EP) +
+                     sizeof (int) * m->npnp;
+  SX_HIP (hipFuncSetAttribute ((const void*) k_recover32_t,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_recover32_t, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
+                      lds, st, m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_btog,
+                      s->d_Mrec32, Fp, V, stride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                      s->base_mode);
+  SX_HIP (hipGetLastError ());
   return SX_OK;
 }
 

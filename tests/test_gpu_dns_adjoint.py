@@ -146,7 +146,7 @@ def test_dns_gradient_matches_finite_differences(torch_cuda, setup):
         return _J_np(T, U, Ut)
 
     j0 = J_gpu(P["u0"])
-    assert abs(j0 - float(J)) <= 1e-10 * abs(j0), (j0, float(J))
+    assert abs(j0 - float(J.detach())) <= 1e-10 * abs(j0), (j0, float(J.detach()))
     fd = (J_gpu(P["u0"] + H * delta) - J_gpu(P["u0"] - H * delta)) / (2 * H)
     print("adjoint", adj, "GPU fd", fd, flush=True)
     assert abs(adj - fd) <= 1e-6 * abs(fd), (adj, fd)
