@@ -441,7 +441,7 @@ template <int NR, int D>
 __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
-    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb)
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
 {
   constexpr int NW = PanelSweep<NR, D>::NW, TRW = PanelSweep<NR, D>::TRW;
   constexpr int HB = PanelSweep<NR, D>::HB, H = PanelSweep<NR, D>::H;
@@ -499,20 +499,40 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     const bool wout = pass ? wb : wf;
     if (J0 >= J1) continue;
     const double* P = (pass == 0 ? PF : PR) + (size_t) mode * pstride + poff;
-    const double* zero = P + (size_t) nb * PS + (size_t) PB * PB;    // a zero block
     auto gidx = [&] (int j) __attribute__ ((always_inline)) {
       j = min (j, n - 1);
       return pass ? n - 1 - j : j;
     };
-    // this wave's rows of block U_{J, J+1+slot} (or zeros)
-    auto ublk = [&] (int J) __attribute__ ((always_inline)) {
-      return (owner ? P + (size_t) J * PS + (size_t) (slot + 1) * PB * PB : zero) +
-             (size_t) half * HB * PB + 2 * lane;
+    // Panel loads go through a buffer descriptor over the piece's nb+1
+    // panels: a lane whose row pair has no entry inside the band (the lower
+    // half of T_J, the tail of the last off-diagonal block, waves without a
+    // block) is given an offset past the range and reads 0 with no memory
+    // access -- the stored values there are 0 as well.  Every wave still
+    // issues the same loads in the same order.
+    const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc (
+        (void*) P, (short) 0, (int) (sizeof (double) * (size_t) (nb + 1) * PS), 0x00020000);
+    constexpr int OOB = 0x7ffffff0;
+    auto pld = [&] (int boff) __attribute__ ((always_inline)) {
+      typedef int v4i __attribute__ ((ext_vector_type (4)));
+      const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (prs, boff, 0, 0);
+      return __builtin_bit_cast (double2, t);
     };
-    // triangle rows [w*TRW, w*TRW+TRW) of T_J (rows past 63 re-read row 63)
-    auto trow = [&] (int J, int k) __attribute__ ((always_inline)) {
+    // byte offset of this wave's rows of block U_{J, J+1+slot}, lane's column;
+    // row pair t of it reaches into the band for this lane iff lane <= ulim + t
+    auto uoff = [&] (int J) __attribute__ ((always_inline)) {
+      return (int) (sizeof (double) * ((size_t) J * PS + (size_t) (slot + 1) * PB * PB +
+                                       (size_t) half * HB * PB + 2 * lane));
+    };
+    const int ulim = owner ? kd - PB * (slot + 1) + half * HB + 1 : -2 * PB;
+    // triangle rows [w*TRW, w*TRW+TRW) of T_J (upper triangular: row pair t
+    // is needed by lanes >= t; rows past 63 by none)
+    auto toff = [&] (int J, int k) __attribute__ ((always_inline)) {
       const int t = min (w * TRW + k, PB - 2);
-      return P + (size_t) J * PS + pidx (t, lane);
+      return (int) (sizeof (double) * ((size_t) J * PS + pidx (t, lane)));
+    };
+    auto tneed = [&] (int k) __attribute__ ((always_inline)) {
+      const int t = w * TRW + k;
+      return t < PB && lane >= t;
     };
     auto load_b = [&] (int K) __attribute__ ((always_inline)) {
 #pragma unroll
@@ -536,10 +556,10 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
           pp[r] += wt * (t0 * (r0[ta * NR + r] + r1[ta * NR + r]) +
                          t1 * (r0[(ta + 1) * NR + r] + r1[(ta + 1) * NR + r]));
         // reload behind the use (offset tied to the sums, as for the blocks)
-        int off = (int) ((const char*) trow (Jn, k) - (const char*) P);
+        int off = toff (Jn, k);
 #pragma unroll
         for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(pp[r]));
-        const double2 v = *(const double2*) ((const char*) P + off);
+        const double2 v = pld (tneed (k) ? off : OOB);
         tr[k] = v.x;
         tr[k + 1] = v.y;
       }
@@ -576,15 +596,15 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     // block J0+D
 #pragma unroll
     for (int k = 0; k < TRW; k += 2) {
-      const double2 v = *(const double2*) trow (J0, k);
+      const double2 v = pld (tneed (k) ? toff (J0, k) : OOB);
       tr[k] = v.x;
       tr[k + 1] = v.y;
     }
     {
-      const double* pu = ublk (J0);
+      const int ub = uoff (J0);
 #pragma unroll
       for (int t = 0; t < HB; t += 2) {
-        const double2 v = *(const double2*) (pu + (size_t) t * PB);
+        const double2 v = pld (lane <= ulim + t ? ub + t * PB * (int) sizeof (double) : OOB);
         u[t] = v.x;
         u[t + 1] = v.y;
       }
@@ -609,7 +629,7 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         double acc0[NR], acc1[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) acc0[r] = acc1[r] = 0.0;
-        const double* pn = ublk (J + 1);
+        const int un = uoff (J + 1);
 #pragma unroll
         for (int t = 0; t < HB; t += 2) {
           const double u0 = u[t], u1 = u[t + 1];
@@ -618,10 +638,10 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
             acc0[r] += u0 * lane_bcast (yreg[r], row0 + t);
             acc1[r] += u1 * lane_bcast (yreg[r], row0 + t + 1);
           }
-          int off = t * PB * (int) sizeof (double);
+          int off = un + t * PB * (int) sizeof (double);
 #pragma unroll
           for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc0[r]), "v"(acc1[r]));
-          const double2 v = *(const double2*) ((const char*) pn + off);
+          const double2 v = pld (lane <= ulim + t ? off : OOB);
           u[t] = v.x;
           u[t + 1] = v.y;
         }
@@ -762,7 +782,7 @@ int launch_panel_d (const sx_msys* s, int G, int stage, hipStream_t st)
   const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (NW * 64), lds, st, s->d_swF, s->d_swR,
                       s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G,
-                      s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb);
+                      s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb, s->m->nband - 1);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
