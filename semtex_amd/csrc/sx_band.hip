@@ -437,6 +437,57 @@ struct PanelSweep {
 constexpr int TW = 8;
 enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
 
+// One sweep launch's work for this workgroup: its mode and right-hand sides,
+// the piece's size, panels and block ranges per pass ([f0, f1) forward,
+// [b0, b1) backward; wf/wb: hand the window on).  G workgroups share a mode
+// (and piece), NR of its G*NR right-hand sides each; they get block indices
+// congruent mod 8, i.e. the same XCD and L2.  False: nothing to do.
+struct SweepSeg {
+  int mode, n, f0, f1, b0, b1, rs;
+  bool wf, wb;
+  size_t poff;
+  double* bg;
+};
+
+__device__ __forceinline__ bool sweep_seg (SweepSeg& g, int NR, int D,
+                                           const int* __restrict__ nsolve, double* rhs,
+                                           int nmax, int nmodes, int G,
+                                           const int* __restrict__ tw, int stage, size_t boff,
+                                           double* rhsb, int nmaxb)
+{
+  const int per = (nmodes + 7) / 8 * 8 * G;
+  const int piece = blockIdx.x / per, b = blockIdx.x % per;
+  const int mode = (b / (8 * G)) * 8 + b % 8, grp = (b / 8) % G;
+  if (mode >= nmodes) return false;
+  g.mode = mode;
+  g.f0 = g.f1 = g.b0 = g.b1 = 0;
+  g.wf = g.wb = false;
+  g.poff = 0;
+  g.bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
+  g.rs = nmax;
+  if (stage == 0) {
+    g.n = nsolve[mode];
+    g.f1 = g.b1 = (g.n + PB - 1) / PB;
+  } else {
+    const int* t = tw + TW * mode;
+    const int nbT = t[TW_NBT], nbB = t[TW_NBB];
+    if (piece == 0) {
+      g.n = t[TW_NTOP];
+      if (stage == 1) { g.f1 = nbT; g.wf = true; }
+      if (stage == 2) { g.f0 = nbT; g.f1 = nbT + D; g.b1 = D; g.wb = true; }
+      if (stage == 3) { g.b0 = D; g.b1 = nbT + D; }
+    } else {
+      g.n = t[TW_NBOT];
+      g.poff = boff;
+      g.bg = rhsb + ((size_t) mode * G + grp) * NR * nmaxb;
+      g.rs = nmaxb;
+      if (stage == 1) { g.f1 = nbB; g.wf = true; }
+      if (stage == 3) g.b1 = nbB + D;
+    }
+  }
+  return g.n > 0;
+}
+
 template <int NR, int D>
 __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
@@ -448,44 +499,12 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
   extern __shared__ double sm[];
   double* rbuf = sm;                                // [2][D][64][NR] pending r, by half
   double* part = rbuf + 2 * D * PB * NR;            // [NW][64][NR] triangle partials
-  // G workgroups share a mode (and piece), NR of its G*NR right-hand sides
-  // each; they get block indices congruent mod 8, i.e. the same XCD and L2
-  const int per = (nmodes + 7) / 8 * 8 * G;
-  const int piece = blockIdx.x / per, b = blockIdx.x % per;
-  const int mode = (b / (8 * G)) * 8 + b % 8, grp = (b / 8) % G;
-  if (mode >= nmodes) return;
-  // this piece's size, panels, right-hand sides and block ranges per pass
-  // ([f0, f1) forward, [b0, b1) backward; wf/wb: hand the window on)
-  int n, f0 = 0, f1 = 0, b0 = 0, b1 = 0;
-  bool wf = false, wb = false;
-  size_t poff = 0;
-  double* bg;
-  int rs;
-  if (stage == 0) {
-    n = nsolve[mode];
-    f1 = b1 = (n + PB - 1) / PB;
-    bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
-    rs = nmax;
-  } else {
-    const int* t = tw + TW * mode;
-    const int nbT = t[TW_NBT], nbB = t[TW_NBB];
-    if (piece == 0) {
-      n = t[TW_NTOP];
-      bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
-      rs = nmax;
-      if (stage == 1) { f1 = nbT; wf = true; }
-      if (stage == 2) { f0 = nbT; f1 = nbT + D; b1 = D; wb = true; }
-      if (stage == 3) { b0 = D; b1 = nbT + D; }
-    } else {
-      n = t[TW_NBOT];
-      poff = boff;
-      bg = rhsb + ((size_t) mode * G + grp) * NR * nmaxb;
-      rs = nmaxb;
-      if (stage == 1) { f1 = nbB; wf = true; }
-      if (stage == 3) b1 = nbB + D;
-    }
-  }
-  if (n == 0) return;
+  SweepSeg g;
+  if (!sweep_seg (g, NR, D, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb)) return;
+  const int mode = g.mode, n = g.n, f0 = g.f0, f1 = g.f1, b0 = g.b0, b1 = g.b1, rs = g.rs;
+  const bool wf = g.wf, wb = g.wb;
+  const size_t poff = g.poff;
+  double* bg = g.bg;
   const int nb = (n + PB - 1) / PB;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
@@ -808,6 +827,289 @@ int launch_panel (const sx_msys* s, int G, int stage, hipStream_t st)
   return SX_ERR_LIMIT;
 }
 
+// ---------------------------------------------------------------------------
+// Unit-diagonal panel sweeps (D = 3..5): two steps of panel loads in flight
+// ---------------------------------------------------------------------------
+// With U = diag(U_JJ) V, the off-diagonal blocks of V are W_{J,K} = T_J U_{J,K}
+// and a forward sweep U^T y = b becomes
+//   z_J = r_J,   r_K -= W_{J,K}^T z_J  (K = J+1..J+D),   y_J = T_J^T z_J,
+// so the chain of dependent block steps carries no triangle: a step is the
+// update alone and one barrier, and y_J is formed beside it.  The backward
+// sweep is the same on the reversed factor (its own blocking, its own T_J).
+// W_{J,J+D} fills its upper triangle (columns c <= kd - 64D + 63 in every
+// row), the other blocks are full.
+//
+// Eight wavefronts (two per SIMD, up to 256 VGPRs each) share a panel by
+// rows: wave w takes row pairs 4w..4w+3 (rows 8w..8w+7) of each of its D+1
+// blocks, so per step it broadcasts only its eight entries of z_J into
+// SGPRs and every pair costs two FMAs.  Each wave holds its pairs of panels
+// J+1 and J+2 in registers and reloads a pair right behind its use, two
+// steps ahead.  Pending right-hand sides live in LDS in a ring of D+1 entries
+// (block K at K mod (D+1), so the block entering the window never shares an
+// entry with the one being read), one slot per wave; the triangle partials
+// are double-buffered and summed into y one step later.
+constexpr int UW = 8;                               // wavefronts per workgroup
+constexpr int UPW = 4;                              // row pairs per wave and block
+
+template <int NR, int D>
+__global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
+{
+  constexpr int NP = UPW * (D + 1);                 // pairs per wave and panel
+  static_assert (UW * UPW * 2 == PB, "rows per wave");
+  extern __shared__ double sm[];
+  double* ring = sm;                                // [D+1][UW][64][NR] pending r
+  double* part = ring + (D + 1) * UW * PB * NR;     // [2][UW][64][NR] triangle partials
+  SweepSeg g;
+  if (!sweep_seg (g, NR, D, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb)) return;
+  const int n = g.n, rs = g.rs;
+  double* bg = g.bg;
+  const int nb = (n + PB - 1) / PB;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
+  const size_t PS = (size_t) (D + 1) * PB * PB;     // doubles per panel
+  double2 ua[NP], ub[NP];
+  double ba[NR], bb[NR];
+
+  for (int pass = 0; pass < 2; pass++) {
+    const int J0 = pass ? g.b0 : g.f0, J1 = pass ? g.b1 : g.f1;
+    const bool wout = pass ? g.wb : g.wf;
+    if (J0 >= J1) continue;
+    const double* P = (pass == 0 ? PF : PR) + (size_t) g.mode * pstride + g.poff;
+    auto gidx = [&] (int j) __attribute__ ((always_inline)) {
+      j = min (j, n - 1);
+      return pass ? n - 1 - j : j;
+    };
+    // Panel loads through a buffer descriptor over the piece's nb+1 panels.
+    // Pair (d, k) of this wave sits at byte 32768 d + 1024 (4w + k) of a
+    // panel, lane c at 16 c in it: the lane's offset (one register per
+    // step) plus a constant.  A lane whose pair holds no entry in its column
+    // (T_J: c < 2p; W_{J,J+D}: c > kd - 64D + 63), and every lane past panel
+    // nb, gets a base past the range and reads 0 with no memory access.
+    const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc (
+        (void*) P, (short) 0, (int) (sizeof (double) * (size_t) (nb + 1) * PS), 0x00020000);
+    constexpr int OOB = 0x7f000000;
+    auto pld = [&] (int voff, int soff) __attribute__ ((always_inline)) {
+      typedef int v4i __attribute__ ((ext_vector_type (4)));
+      const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (prs, voff, soff, 0);
+      return __builtin_bit_cast (double2, t);
+    };
+    auto vbase = [&] (int Jn) __attribute__ ((always_inline)) {
+      return Jn <= nb ? (int) (sizeof (double) * (size_t) Jn * PS) + 1024 * UPW * w + 16 * lane
+                      : OOB;
+    };
+    const bool dneed = lane <= kd - PB * D + PB - 1;
+    auto ring_at = [&] (int K, int s) __attribute__ ((always_inline)) {
+      return ring + ((size_t) ((K % (D + 1)) * UW + s) * PB + lane) * NR;
+    };
+    auto part_at = [&] (int J, int s) __attribute__ ((always_inline)) {
+      return part + ((size_t) ((J & 1) * UW + s) * PB + lane) * NR;
+    };
+    // the voffset of pair (d, k) for a panel whose base is vb
+    auto pvoff = [&] (int vb, int d, int k) __attribute__ ((always_inline)) {
+      if (d == 0) return lane >= 8 * w + 2 * k ? vb : OOB;
+      if (d == D) return dneed ? vb : OOB;
+      return vb;
+    };
+    // y_J = the sum of the triangle partials, out by wave 0
+    auto put_y = [&] (int J) __attribute__ ((always_inline)) {
+      const int j = J * PB + lane;
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        double y = part_at (J, 0)[r];
+#pragma unroll
+        for (int s = 1; s < UW; s++) y += part_at (J, s)[r];
+        if (w == 0 && j < n) bg[(size_t) r * rs + gidx (j)] = y;
+      }
+    };
+
+    // window: blocks J0..J0+D-1 from b, one wave per block
+    if (w < D) {
+      const int K = J0 + w;
+      const bool in = K * PB + lane < n;
+#pragma unroll
+      for (int s = 0; s < UW; s++)
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+          ring_at (K, s)[r] = (in && s == 0) ? bg[(size_t) r * rs + gidx (K * PB + lane)] : 0.0;
+    }
+    // panels J0 and J0+1, b of the blocks entering at steps J0 and J0+1
+    {
+      const int v0 = vbase (J0), v1 = vbase (J0 + 1);
+#pragma unroll
+      for (int k = 0; k < UPW; k++)
+#pragma unroll
+        for (int d = 0; d <= D; d++) ua[k * (D + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+#pragma unroll
+      for (int r = 0; r < NR; r++) ba[r] = bg[(size_t) r * rs + gidx ((J0 + D) * PB + lane)];
+#pragma unroll
+      for (int k = 0; k < UPW; k++)
+#pragma unroll
+        for (int d = 0; d <= D; d++) ub[k * (D + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
+#pragma unroll
+      for (int r = 0; r < NR; r++) bb[r] = bg[(size_t) r * rs + gidx ((J0 + D + 1) * PB + lane)];
+    }
+    __builtin_amdgcn_s_waitcnt (0);
+    __syncthreads ();
+
+    auto step = [&] (int J, double2 (&u)[NP], double (&bq)[NR]) __attribute__ ((always_inline)) {
+      // z_J = r_J (the sum of its slots); this wave's eight rows of it
+      double zs[NR][2 * UPW];
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        double z = ring_at (J, 0)[r];
+#pragma unroll
+        for (int s = 1; s < UW; s++) z += ring_at (J, s)[r];
+#pragma unroll
+        for (int j = 0; j < 2 * UPW; j++) zs[r][j] = lane_bcast (z, 8 * w + j);
+      }
+      if (J > J0) put_y (J - 1);
+      // products with this wave's pairs of panel J (all D+1 targets); each
+      // pair reloaded from panel J+2 right behind its use, its offset tied
+      // to the sums so the loads stay behind the FMAs
+      double acc[D + 1][NR];
+#pragma unroll
+      for (int d = 0; d <= D; d++)
+#pragma unroll
+        for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
+      const int vb = vbase (J + 2);
+#pragma unroll
+      for (int k = 0; k < UPW; k++) {
+#pragma unroll
+        for (int d = 0; d <= D; d++) {
+          const double2 v = u[k * (D + 1) + d];
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            acc[d][r] = fma (v.y, zs[r][2 * k + 1], fma (v.x, zs[r][2 * k], acc[d][r]));
+          int so = 32768 * d + 1024 * k;
+#pragma unroll
+          for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(acc[d][r]));
+          u[k * (D + 1) + d] = pld (pvoff (vb, d, k), so);
+        }
+      }
+      // into the targets: the triangle partial, pending blocks J+1..J+D
+      // (block J+D enters the window: every wave sets its slot, wave 0 adds b)
+#pragma unroll
+      for (int r = 0; r < NR; r++) part_at (J, w)[r] = acc[0][r];
+#pragma unroll
+      for (int d = 1; d < D; d++)
+#pragma unroll
+        for (int r = 0; r < NR; r++) ring_at (J + d, w)[r] -= acc[d][r];
+      {
+        const bool in = w == 0 && (J + D) * PB + lane < n;
+#pragma unroll
+        for (int r = 0; r < NR; r++) ring_at (J + D, w)[r] = (in ? bq[r] : 0.0) - acc[D][r];
+      }
+      // b of the block entering two steps on, loaded after this one's use
+      {
+        int off = 0;
+#pragma unroll
+        for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[D][r]), "v"(bq[r]));
+        const int gi = gidx ((J + D + 2) * PB + lane) + off;
+#pragma unroll
+        for (int r = 0; r < NR; r++) bq[r] = bg[(size_t) r * rs + gi];
+      }
+      lds_sync ();
+    };
+
+    // both steps on every path to the back edge, so that the compiler's wait
+    // counts see buffer a's loads two steps old and b's one step old
+    int J = J0;
+    for (; J + 1 < J1; J += 2) {
+      step (J, ua, ba);
+      step (J + 1, ub, bb);
+    }
+    if (J < J1) step (J, ua, ba);
+    put_y (J1 - 1);
+    // a segment that stops early hands its pending window on through b
+    if (wout && w < D) {
+      const int K = J1 + w;
+      if (K * PB + lane < n) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          double v = ring_at (K, 0)[r];
+#pragma unroll
+          for (int s = 1; s < UW; s++) v += ring_at (K, s)[r];
+          bg[(size_t) r * rs + gidx (K * PB + lane)] = v;
+        }
+      }
+    }
+    // the next pass reads what this one wrote to global memory
+    __threadfence ();
+    __syncthreads ();
+  }
+}
+
+// W_{J,J+d} = T_J U_{J,J+d} in place of U_{J,J+d}, d = 1..D: one workgroup
+// per (J, d, mode), T_J and the block staged in LDS, sums in a fixed order
+__global__ __launch_bounds__(256) void k_panel_unit (double* P, size_t pstride, int D, int nbp)
+{
+  extern __shared__ double sm[];
+  double* T = sm;                                   // [64][65]
+  double* B = sm + PB * (PB + 1);                   // [64][65]
+  const int mode = blockIdx.y, J = blockIdx.x / D, d = 1 + blockIdx.x % D;
+  if (J >= nbp) return;
+  const double* base = P + (size_t) mode * pstride + (size_t) J * (D + 1) * PB * PB;
+  double* blk = P + (size_t) mode * pstride + ((size_t) J * (D + 1) + d) * PB * PB;
+  for (int k = threadIdx.x; k < PB * PB; k += blockDim.x) {
+    const int t = k / PB, c = k % PB;
+    T[t * (PB + 1) + c] = base[pidx (t, c)];
+    B[t * (PB + 1) + c] = blk[pidx (t, c)];
+  }
+  __syncthreads ();
+  for (int k = threadIdx.x; k < PB * PB; k += blockDim.x) {
+    const int t = k / PB, c = k % PB;
+    double acc = 0.0;
+    for (int s = t; s < PB; s++) acc = fma (T[t * (PB + 1) + s], B[s * (PB + 1) + c], acc);
+    blk[pidx (t, c)] = acc;
+  }
+}
+
+bool unit_ok (int D) { return D >= 3 && D <= 5; }
+
+template <int NR, int D>
+int launch_unit_d (const sx_msys* s, int G, int stage, hipStream_t st)
+{
+  const size_t lds = sizeof (double) * ((size_t) (D + 3) * UW * PB * NR);
+  auto kern = k_unit_sweeps<NR, D>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
+  hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
+                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G,
+                      s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb, s->m->nband - 1);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+template <int NR>
+int launch_unit (const sx_msys* s, int G, int stage, hipStream_t st)
+{
+  switch (s->sw_D) {
+  case 3: return launch_unit_d<NR, 3> (s, G, stage, st);
+  case 4: return launch_unit_d<NR, 4> (s, G, stage, st);
+  case 5: return launch_unit_d<NR, 5> (s, G, stage, st);
+  default: break;
+  }
+  set_error ("unit panel sweep: unsupported block count");
+  return SX_ERR_LIMIT;
+}
+
+// scale the off-diagonal blocks of nbp panels per mode to W = T_J U
+int unit_panels (const sx_msys* s, double* P, int nbp, hipStream_t st)
+{
+  const size_t lds = sizeof (double) * 2 * PB * (PB + 1);
+  SX_HIP (hipFuncSetAttribute ((const void*) k_panel_unit,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (k_panel_unit, dim3 ((unsigned) (nbp * s->sw_D), s->nmodes), dim3 (256),
+                      lds, st, P, s->sw_stride, s->sw_D, nbp);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
 // rows per wavefront, rounded up to a multiple of 4
 #define SX_QS_CASES(B) \
   switch (qs) { \
@@ -843,6 +1145,15 @@ static int rows_per_wave (int nband)
   qs = (qs + 3) / 4 * 4;
   if (qs > 64) qs = (qs + 7) / 8 * 8;
   return qs;
+}
+
+// Unit-diagonal panels and their sweeps for D = 3..5 (SX_BAND_UNIT=0 in the
+// environment keeps the triangle in the chain, k_panel_sweeps)
+static bool use_unit (int D)
+{
+  if (const char* e = std::getenv ("SX_BAND_UNIT"))
+    if (std::atoi (e) == 0) return false;
+  return unit_ok (D);
 }
 
 static bool use_panels (int nband)
@@ -917,6 +1228,8 @@ static int build_panels (sx_msys* s, const double* src, double* V, size_t stride
     hipLaunchKernelGGL (k_panel_offdiag, dim3 ((unsigned) ((tot + 255) / 256), s->nmodes),
                         dim3 (256), 0, st, from[k], stride, ld, d_n, to[k], s->sw_stride, D,
                         nbp);
+    if (s->sw_unit)
+      if (int rc = unit_panels (s, to[k], nbp, st)) return rc;
   }
   SX_HIP (hipGetLastError ());
   return SX_OK;
@@ -963,6 +1276,7 @@ static int band_factor_twisted (sx_msys* s, const std::vector<int>& tw, hipStrea
   SX_HIP (hipGetLastError ());
   s->sw_panel = 1;
   s->sw_D = D;
+  s->sw_unit = use_unit (D);
   s->sw_block = PB;
   s->tw = 1;
   s->tw_S = S;
@@ -1007,6 +1321,7 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
     const int D = panel_D (ld);
     s->sw_panel = 1;
     s->sw_D = D;
+    s->sw_unit = use_unit (D);
     s->sw_block = PB;
     s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
       SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
@@ -1022,6 +1337,8 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
       hipLaunchKernelGGL (k_panel_offdiag, dim3 ((unsigned) ((tot + 255) / 256), s->nmodes),
                           dim3 (256), 0, st, src[k], s->band_stride, ld, s->d_nsolve,
                           dst[k], s->sw_stride, D, nblk + 1);
+      if (s->sw_unit)
+        if (int rc = unit_panels (s, dst[k], nblk + 1, st)) return rc;
     }
     SX_HIP (hipGetLastError ());
     SX_HIP (hipStreamSynchronize (st));
@@ -1064,9 +1381,16 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
     else
       for (int c : {1, 2, 3, 6})
         if (nr % c == 0 && slots * (nr / c) <= 256) { per = c; break; }
+    if (s->sw_unit && per > 2) per = nr % 2 == 0 ? 2 : 1;
     if (per < 1 || nr % per) per = nr;
     const int G = nr / per;
     auto sweep = [&] (int stage) {
+      if (s->sw_unit) {
+        if (per == 1) return launch_unit<1> (s, G, stage, st);
+        if (per == 2) return launch_unit<2> (s, G, stage, st);
+        set_error ("band solve: unit panel sweeps take 1 or 2 right-hand sides per workgroup");
+        return (int) SX_ERR_ARG;
+      }
       switch (per) {
       case 1: return launch_panel<1> (s, G, stage, st);
       case 2: return launch_panel<2> (s, G, stage, st);

@@ -142,6 +142,7 @@ struct sx_msys {
   int sw_block = 0;            // sweep block size B
   int sw_panel = 0;            // 1: right-looking panel sweeps (sw_D off-diagonal blocks)
   int sw_D = 0;
+  int sw_unit = 0;             // 1: off-diagonal panels scaled to W = T_J U (unit diagonal blocks)
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks
