@@ -443,22 +443,27 @@ enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
 // (and piece), NR of its G*NR right-hand sides each; they get block indices
 // congruent mod 8, i.e. the same XCD and L2.  False: nothing to do.
 struct SweepSeg {
-  int mode, n, f0, f1, b0, b1, rs;
+  int mode, n, f0, f1, b0, b1, rs, role, chain;
   bool wf, wb;
   size_t poff;
   double* bg;
 };
 
+// roles: workgroups per chain (a chain: one piece of one mode, NR right-hand
+// sides); they also get block indices congruent mod 8
 __device__ __forceinline__ bool sweep_seg (SweepSeg& g, int NR, int D,
                                            const int* __restrict__ nsolve, double* rhs,
                                            int nmax, int nmodes, int G,
                                            const int* __restrict__ tw, int stage, size_t boff,
-                                           double* rhsb, int nmaxb)
+                                           double* rhsb, int nmaxb, int roles = 1)
 {
-  const int per = (nmodes + 7) / 8 * 8 * G;
+  const int per = (nmodes + 7) / 8 * 8 * G * roles;
   const int piece = blockIdx.x / per, b = blockIdx.x % per;
-  const int mode = (b / (8 * G)) * 8 + b % 8, grp = (b / 8) % G;
+  const int mode = (b / (8 * G * roles)) * 8 + b % 8, gr = (b / 8) % (G * roles);
+  const int grp = gr / roles;
   if (mode >= nmodes) return false;
+  g.role = gr % roles;
+  g.chain = (piece * nmodes + mode) * G + grp;
   g.mode = mode;
   g.f0 = g.f1 = g.b0 = g.b1 = 0;
   g.wf = g.wb = false;
@@ -1042,6 +1047,382 @@ __global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
   }
 }
 
+// ---------------------------------------------------------------------------
+// Split unit sweeps: one chain of block steps on two CUs
+// ---------------------------------------------------------------------------
+// A step moves the whole panel (up to 192 KB) through one CU's vector
+// memory path, which bounds it.  Here the panel of each step is shared by
+// two workgroups on one XCD: the near one (role 0) takes T_J and
+// W_{J,J+1..J+H} (H = D/2), the far one (role 1) W_{J,J+H+1..J+D}.
+//   near, step J:  z_J = (own pending r_J) + p_J;  publish z_J;  y_J; near updates
+//   far,  step J:  wait z_J;  far updates;  block J+H complete -> publish p_{J+H}
+// p_K (the far contributions to block K) is published H steps before the
+// near side needs it, so neither side waits in the steady state.  Messages
+// go through L2 as 16-byte records {value, epoch, seq} in rings of XR
+// entries per direction: a record is accepted when its epoch and sequence
+// number match, so no flag has to be ordered against the data.  The epoch
+// of a chain lives in device memory and advances once per launch (graph
+// replays included).  Both workgroups of a chain must be resident at once:
+// the host launches this kernel only when every workgroup of the grid fits
+// on the device at once.  A wait that exceeds its bound marks the side dead:
+// it skips further waits and its values become NaN, so a failure shows in
+// the solution rather than as a hang.
+constexpr int XR = 8;                               // ring entries per direction
+constexpr int XSPIN = 1 << 19;                      // polls before a wait gives up
+
+__device__ __forceinline__ void msg_put (__amdgpu_buffer_rsrc_t rs, int off, double v,
+                                         unsigned ep, unsigned seq)
+{
+  typedef int v4i __attribute__ ((ext_vector_type (4)));
+  const long long x = __double_as_longlong (v);
+  v4i t;
+  t.x = (int) (x & 0xffffffffll);
+  t.y = (int) (x >> 32);
+  t.z = (int) ep;
+  t.w = (int) seq;
+  __builtin_amdgcn_raw_buffer_store_b128 (t, rs, off, 0, 17);   // sc0 sc1: to L2
+}
+
+// every lane's record of (ep, seq), or NaN once the wait gives up
+template <int NR>
+__device__ __forceinline__ void msg_wait (__amdgpu_buffer_rsrc_t rs, int off0, int rstride,
+                                          unsigned ep, unsigned seq, double (&v)[NR],
+                                          bool& dead)
+{
+  typedef int v4i __attribute__ ((ext_vector_type (4)));
+  if (dead) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) v[r] = __builtin_nan ("");
+    return;
+  }
+  for (int it = 0;; it++) {
+    bool ok = true;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      asm volatile ("" ::: "memory");
+      const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (rs, off0 + r * rstride, 0, 17);
+      v[r] = __longlong_as_double (((long long) t.y << 32) | (unsigned) t.x);
+      ok = ok && (unsigned) t.z == ep && (unsigned) t.w == seq;
+    }
+    if (__builtin_amdgcn_ballot_w64 (!ok) == 0) return;
+    if (it >= XSPIN) break;
+    __builtin_amdgcn_s_sleep (2);
+  }
+  dead = true;
+#pragma unroll
+  for (int r = 0; r < NR; r++) v[r] = __builtin_nan ("");
+}
+
+template <int NR, int D>
+__global__ __launch_bounds__(UW * 64) void k_split_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
+    char* xch, unsigned* epochs)
+{
+  constexpr int H = D / 2, FB = D - H;              // near blocks 1..H, far H+1..D
+  constexpr int NA = UPW * (H + 1), NB = UPW * FB;  // pairs per wave
+  constexpr int NP = NA > NB ? NA : NB;
+  constexpr int MS = PB * 16;                       // bytes per record row (64 lanes)
+  extern __shared__ double sm[];
+  SweepSeg g;
+  if (!sweep_seg (g, NR, D, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, 2))
+    return;
+  const int n = g.n, rs = g.rs, role = g.role;
+  double* bg = g.bg;
+  const int nb = (n + PB - 1) / PB;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
+  const size_t PS = (size_t) (D + 1) * PB * PB;
+  // this chain's rings: [dir][XR][NR] rows of 64 records; dir 0: z (near ->
+  // far), dir 1: p (far -> near)
+  const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc (
+      xch + (size_t) g.chain * 2 * XR * NR * MS, (short) 0, 2 * XR * NR * MS, 0x00020000);
+  auto xoff = [&] (int dir, int K) __attribute__ ((always_inline)) {
+    return ((dir * XR + (K & (XR - 1))) * NR) * MS + 16 * lane;
+  };
+  const unsigned ep = __hip_atomic_load (epochs + g.chain, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+  // the top bit keeps a never-written (zero) record from matching
+  auto seq = [&] (int pass, int K) __attribute__ ((always_inline)) {
+    return 0x80000000u | ((unsigned) pass << 24) | (unsigned) K;
+  };
+  bool dead = false;
+  double2 ua[NP], ub[NP];
+  double ba[NR], bb[NR];
+  bool ran = false;                                  // a previous pass ran
+
+  for (int pass = 0; pass < 2; pass++) {
+    const int J0 = pass ? g.b0 : g.f0, J1 = pass ? g.b1 : g.f1;
+    const bool wout = pass ? g.wb : g.wf;
+    if (J0 >= J1) continue;
+    const double* P = (pass == 0 ? PF : PR) + (size_t) g.mode * pstride + g.poff;
+    auto gidx = [&] (int j) __attribute__ ((always_inline)) {
+      j = min (j, n - 1);
+      return pass ? n - 1 - j : j;
+    };
+    const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc (
+        (void*) P, (short) 0, (int) (sizeof (double) * (size_t) (nb + 1) * PS), 0x00020000);
+    constexpr int OOB = 0x7f000000;
+    auto pld = [&] (int voff, int soff) __attribute__ ((always_inline)) {
+      typedef int v4i __attribute__ ((ext_vector_type (4)));
+      const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (prs, voff, soff, 0);
+      return __builtin_bit_cast (double2, t);
+    };
+    auto vbase = [&] (int Jn) __attribute__ ((always_inline)) {
+      return Jn <= nb ? (int) (sizeof (double) * (size_t) Jn * PS) + 1024 * UPW * w + 16 * lane
+                      : OOB;
+    };
+    const bool dneed = lane <= kd - PB * D + PB - 1;
+    auto pvoff = [&] (int vb, int d, int k) __attribute__ ((always_inline)) {
+      if (d == 0) return lane >= 8 * w + 2 * k ? vb : OOB;
+      if (d == D) return dneed ? vb : OOB;
+      return vb;
+    };
+
+    if (role == 0) {
+      // ---------------- near side: T_J, W_{J,J+1..J+H} ----------------
+      constexpr int RE = H + 1;                      // ring entries: blocks J..J+H
+      double* ring = sm;                             // [RE][UW][64][NR]
+      double* part = ring + RE * UW * PB * NR;       // [2][UW][64][NR]
+      auto ring_at = [&] (int K, int s) __attribute__ ((always_inline)) {
+        return ring + ((size_t) ((K % RE) * UW + s) * PB + lane) * NR;
+      };
+      auto part_at = [&] (int J, int s) __attribute__ ((always_inline)) {
+        return part + ((size_t) ((J & 1) * UW + s) * PB + lane) * NR;
+      };
+      auto put_y = [&] (int J) __attribute__ ((always_inline)) {
+        const int j = J * PB + lane;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          double y = part_at (J, 0)[r];
+#pragma unroll
+          for (int s = 1; s < UW; s++) y += part_at (J, s)[r];
+          if (w == 0 && j < n) bg[(size_t) r * rs + gidx (j)] = y;
+        }
+      };
+      // window: blocks J0..J0+H-1 from b, one wave per block
+      if (w < H) {
+        const int K = J0 + w;
+        const bool in = K * PB + lane < n;
+#pragma unroll
+        for (int s = 0; s < UW; s++)
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            ring_at (K, s)[r] = (in && s == 0) ? bg[(size_t) r * rs + gidx (K * PB + lane)] : 0.0;
+      }
+      {
+        const int v0 = vbase (J0), v1 = vbase (J0 + 1);
+#pragma unroll
+        for (int k = 0; k < UPW; k++)
+#pragma unroll
+          for (int d = 0; d <= H; d++)
+            ua[k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+#pragma unroll
+        for (int r = 0; r < NR; r++) ba[r] = bg[(size_t) r * rs + gidx ((J0 + H) * PB + lane)];
+#pragma unroll
+        for (int k = 0; k < UPW; k++)
+#pragma unroll
+          for (int d = 0; d <= H; d++)
+            ub[k * (H + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
+#pragma unroll
+        for (int r = 0; r < NR; r++) bb[r] = bg[(size_t) r * rs + gidx ((J0 + H + 1) * PB + lane)];
+      }
+      __builtin_amdgcn_s_waitcnt (0);
+      __syncthreads ();
+
+      auto step = [&] (int J, double2 (&u)[NP], double (&bq)[NR]) __attribute__ ((always_inline)) {
+        double pv[NR], zs[NR][2 * UPW];
+        msg_wait<NR> (xrs, xoff (1, J), MS, ep, seq (pass, J), pv, dead);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          double z = pv[r];
+#pragma unroll
+          for (int s = 0; s < UW; s++) z += ring_at (J, s)[r];
+          if (w == 0) msg_put (xrs, xoff (0, J) + r * MS, z, ep, seq (pass, J));
+#pragma unroll
+          for (int j = 0; j < 2 * UPW; j++) zs[r][j] = lane_bcast (z, 8 * w + j);
+        }
+        if (J > J0) put_y (J - 1);
+        double acc[H + 1][NR];
+#pragma unroll
+        for (int d = 0; d <= H; d++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
+        const int vb = vbase (J + 2);
+#pragma unroll
+        for (int k = 0; k < UPW; k++) {
+#pragma unroll
+          for (int d = 0; d <= H; d++) {
+            const double2 v = u[k * (H + 1) + d];
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              acc[d][r] = fma (v.y, zs[r][2 * k + 1], fma (v.x, zs[r][2 * k], acc[d][r]));
+            int so = 32768 * d + 1024 * k;
+#pragma unroll
+            for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(acc[d][r]));
+            u[k * (H + 1) + d] = pld (pvoff (vb, d, k), so);
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) part_at (J, w)[r] = acc[0][r];
+#pragma unroll
+        for (int d = 1; d < H; d++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) ring_at (J + d, w)[r] -= acc[d][r];
+        {
+          const bool in = w == 0 && (J + H) * PB + lane < n;
+#pragma unroll
+          for (int r = 0; r < NR; r++) ring_at (J + H, w)[r] = (in ? bq[r] : 0.0) - acc[H][r];
+        }
+        {
+          int off = 0;
+#pragma unroll
+          for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[H][r]), "v"(bq[r]));
+          const int gi = gidx ((J + H + 2) * PB + lane) + off;
+#pragma unroll
+          for (int r = 0; r < NR; r++) bq[r] = bg[(size_t) r * rs + gi];
+        }
+        lds_sync ();
+      };
+      int J = J0;
+      for (; J + 1 < J1; J += 2) {
+        step (J, ua, ba);
+        step (J + 1, ub, bb);
+      }
+      if (J < J1) step (J, ua, ba);
+      put_y (J1 - 1);
+      // the far side's last records: p_{J1..J1+D-1}; with them, the window
+      // handed on (blocks that never entered the near ring keep b)
+      for (int K = J1; K < J1 + D; K++) {
+        double pv[NR];
+        msg_wait<NR> (xrs, xoff (1, K), MS, ep, seq (pass, K), pv, dead);
+        if (wout && w == K - J1 && K * PB + lane < n) {
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            double v = pv[r];
+            if (K < J1 + H) {
+#pragma unroll
+              for (int s = 0; s < UW; s++) v += ring_at (K, s)[r];
+            } else {
+              v += bg[(size_t) r * rs + gidx (K * PB + lane)];
+            }
+            bg[(size_t) r * rs + gidx (K * PB + lane)] = v;
+          }
+        }
+      }
+      // the far side has read every z of this pass: tell it the pass is over
+      if (w == 0)
+#pragma unroll
+        for (int r = 0; r < NR; r++) msg_put (xrs, xoff (0, J1) + r * MS, 0.0, ep, seq (pass, J1));
+    } else {
+      // ---------------- far side: W_{J,J+H+1..J+D} ----------------
+      constexpr int RE = FB + 1;                     // ring entries: blocks J+H..J+D
+      double* ring = sm;                             // [RE][UW][64][NR]
+      auto ring_at = [&] (int K, int s) __attribute__ ((always_inline)) {
+        return ring + ((size_t) ((K % RE) * UW + s) * PB + lane) * NR;
+      };
+      // p_K of a completed block, published by wave 0
+      auto put_p = [&] (int K) __attribute__ ((always_inline)) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          double v = ring_at (K, 0)[r];
+#pragma unroll
+          for (int s = 1; s < UW; s++) v += ring_at (K, s)[r];
+          msg_put (xrs, xoff (1, K) + r * MS, v, ep, seq (pass, K));
+        }
+      };
+      // the near side must be done with the previous pass's records
+      if (ran) {
+        double dv[NR];
+        const int Jp = pass ? g.f1 : 0;
+        msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (pass - 1, Jp), dv, dead);
+      }
+      // blocks J0+H+1..J0+D-1 start empty here (their earlier far terms are
+      // in the window the near side reads from b); p_K = 0 for K <= J0+H
+      if (w < FB - 1) {
+        const int K = J0 + H + 1 + w;
+#pragma unroll
+        for (int s = 0; s < UW; s++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) ring_at (K, s)[r] = 0.0;
+      }
+      if (w == 0)
+        for (int K = J0; K <= J0 + H; K++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) msg_put (xrs, xoff (1, K) + r * MS, 0.0, ep, seq (pass, K));
+      {
+        const int v0 = vbase (J0), v1 = vbase (J0 + 1);
+#pragma unroll
+        for (int k = 0; k < UPW; k++)
+#pragma unroll
+          for (int d = 0; d < FB; d++)
+            ua[k * FB + d] = pld (pvoff (v0, H + 1 + d, k), 32768 * (H + 1 + d) + 1024 * k);
+#pragma unroll
+        for (int k = 0; k < UPW; k++)
+#pragma unroll
+          for (int d = 0; d < FB; d++)
+            ub[k * FB + d] = pld (pvoff (v1, H + 1 + d, k), 32768 * (H + 1 + d) + 1024 * k);
+      }
+      __builtin_amdgcn_s_waitcnt (0);
+      __syncthreads ();
+
+      auto step = [&] (int J, double2 (&u)[NP]) __attribute__ ((always_inline)) {
+        if (J > J0 && w == 0) put_p (J + H);
+        double zv[NR], zs[NR][2 * UPW];
+        msg_wait<NR> (xrs, xoff (0, J), MS, ep, seq (pass, J), zv, dead);
+#pragma unroll
+        for (int r = 0; r < NR; r++)
+#pragma unroll
+          for (int j = 0; j < 2 * UPW; j++) zs[r][j] = lane_bcast (zv[r], 8 * w + j);
+        double acc[FB][NR];
+#pragma unroll
+        for (int d = 0; d < FB; d++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
+        const int vb = vbase (J + 2);
+#pragma unroll
+        for (int k = 0; k < UPW; k++) {
+#pragma unroll
+          for (int d = 0; d < FB; d++) {
+            const double2 v = u[k * FB + d];
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              acc[d][r] = fma (v.y, zs[r][2 * k + 1], fma (v.x, zs[r][2 * k], acc[d][r]));
+            int so = 32768 * (H + 1 + d) + 1024 * k;
+#pragma unroll
+            for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(acc[d][r]));
+            u[k * FB + d] = pld (pvoff (vb, H + 1 + d, k), so);
+          }
+        }
+#pragma unroll
+        for (int d = 0; d < FB - 1; d++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) ring_at (J + H + 1 + d, w)[r] -= acc[d][r];
+#pragma unroll
+        for (int r = 0; r < NR; r++) ring_at (J + D, w)[r] = -acc[FB - 1][r];
+        lds_sync ();
+      };
+      int J = J0;
+      for (; J + 1 < J1; J += 2) {
+        step (J, ua);
+        step (J + 1, ub);
+      }
+      if (J < J1) step (J, ua);
+      // the block completed by the last step and the partial sums still
+      // pending: p_{J1+H..J1+D-1}
+      if (w == 0)
+        for (int K = J1 + H; K < J1 + D; K++) put_p (K);
+      ran = true;
+    }
+    __threadfence ();
+    __syncthreads ();
+  }
+  // the near side closes the launch: the next one uses the next epoch
+  if (role == 0 && threadIdx.x == 0)
+    __hip_atomic_store (epochs + g.chain, ep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // W_{J,J+d} = T_J U_{J,J+d} in place of U_{J,J+d}, d = 1..D: one workgroup
 // per (J, d, mode), T_J and the block staged in LDS, sums in a fixed order
 __global__ __launch_bounds__(256) void k_panel_unit (double* P, size_t pstride, int D, int nbp)
@@ -1096,6 +1477,81 @@ int launch_unit (const sx_msys* s, int G, int stage, hipStream_t st)
   }
   set_error ("unit panel sweep: unsupported block count");
   return SX_ERR_LIMIT;
+}
+
+template <int NR, int D>
+size_t split_lds ()
+{
+  constexpr int H = D / 2, FB = D - H;
+  return sizeof (double) * (size_t) std::max (H + 3, FB + 1) * UW * PB * NR;
+}
+
+// workgroups of k_split_sweeps<NR, D> the device holds at once
+template <int NR, int D>
+int split_capacity ()
+{
+  static int cap = -1;
+  if (cap < 0) {
+    int dev = 0, cus = 0, per = 0;
+    auto kern = k_split_sweeps<NR, D>;
+    const int lds = (int) split_lds<NR, D> ();
+    if (hipGetDevice (&dev) != hipSuccess ||
+        hipDeviceGetAttribute (&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipFuncSetAttribute ((const void*) kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             lds) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor (&per, (const void*) kern, UW * 64,
+                                                      (size_t) lds) != hipSuccess)
+      cap = 0;
+    else
+      cap = cus * per;
+  }
+  return cap;
+}
+
+template <int NR, int D>
+int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool probe)
+{
+  const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * 2 * pieces;
+  if (probe) return nblocks <= split_capacity<NR, D> () ? SX_OK : SX_ERR_LIMIT;
+  const size_t lds = split_lds<NR, D> ();
+  auto kern = k_split_sweeps<NR, D>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
+                      s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G,
+                      s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb, s->m->nband - 1,
+                      s->d_xch, s->d_xep);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// probe: SX_OK iff every workgroup of the launch fits on the device at once
+template <int NR>
+int launch_split (const sx_msys* s, int G, int stage, hipStream_t st, bool probe = false)
+{
+  switch (s->sw_D) {
+  case 3: return launch_split_d<NR, 3> (s, G, stage, st, probe);
+  case 4: return launch_split_d<NR, 4> (s, G, stage, st, probe);
+  case 5: return launch_split_d<NR, 5> (s, G, stage, st, probe);
+  default: break;
+  }
+  set_error ("split panel sweep: unsupported block count");
+  return SX_ERR_LIMIT;
+}
+
+// message rings and epochs of the split sweeps: up to 2 pieces x nmodes x 6
+// right-hand sides (one ring pair per right-hand side and chain)
+constexpr int XMAXR = 6;
+int split_alloc (sx_msys* s, hipStream_t st)
+{
+  const size_t rows = (size_t) 2 * s->nmodes * XMAXR;
+  s->xch_bytes = rows * 2 * XR * PB * 16;
+  SX_HIP (hipMalloc (&s->d_xch, s->xch_bytes));
+  SX_HIP (hipMalloc (&s->d_xep, sizeof (unsigned) * rows));
+  SX_HIP (hipMemsetAsync (s->d_xch, 0, s->xch_bytes, st));
+  SX_HIP (hipMemsetAsync (s->d_xep, 0, sizeof (unsigned) * rows, st));
+  return SX_OK;
 }
 
 // scale the off-diagonal blocks of nbp panels per mode to W = T_J U
@@ -1277,6 +1733,8 @@ static int band_factor_twisted (sx_msys* s, const std::vector<int>& tw, hipStrea
   s->sw_panel = 1;
   s->sw_D = D;
   s->sw_unit = use_unit (D);
+  if (s->sw_unit)
+    if (int rc = split_alloc (s, st)) return rc;
   s->sw_block = PB;
   s->tw = 1;
   s->tw_S = S;
@@ -1322,6 +1780,8 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
     s->sw_panel = 1;
     s->sw_D = D;
     s->sw_unit = use_unit (D);
+    if (s->sw_unit)
+      if (int rc = split_alloc (s, st)) return rc;
     s->sw_block = PB;
     s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
       SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
@@ -1383,8 +1843,38 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
         if (nr % c == 0 && slots * (nr / c) <= 256) { per = c; break; }
     if (s->sw_unit && per > 2) per = nr % 2 == 0 ? 2 : 1;
     if (per < 1 || nr % per) per = nr;
+    // Split sweeps (two workgroups per chain) for twisted systems, where the
+    // pieces leave the chain short: the right-hand sides per workgroup
+    // (1, 2 or 3) bring the grid closest to 128 workgroups, the count past
+    // which the sweeps measured slower (memory-system bound), among the
+    // launches that fit on the device at once.  SX_BAND_SPLIT=0/1 in the
+    // environment forces them off/on (on: also for single-piece systems).
+    int sper = 0;
+    if (s->sw_unit && s->d_xch && nr <= XMAXR) {
+      const char* e = std::getenv ("SX_BAND_SPLIT");
+      const bool want = e ? std::atoi (e) != 0 : s->tw != 0;
+      if (want) {
+        const char* ep = std::getenv ("SX_PANEL_NR");
+        const int stg = s->tw ? 1 : 0, slots1 = (s->nmodes + 7) / 8 * 8 * (s->tw ? 2 : 1);
+        int best = 1 << 30;
+        for (int c : {1, 2, 3}) {
+          if (nr % c || (ep && std::atoi (ep) != c)) continue;
+          const int fits = c == 1 ? launch_split<1> (s, nr, stg, st, true)
+                         : c == 2 ? launch_split<2> (s, nr / 2, stg, st, true)
+                                  : launch_split<3> (s, nr / 3, stg, st, true);
+          const int dist = std::abs (slots1 * (nr / c) * 2 - 128);
+          if (fits == SX_OK && dist <= best) { best = dist; sper = c; }
+        }
+      }
+    }
+    if (sper) per = sper;
     const int G = nr / per;
     auto sweep = [&] (int stage) {
+      if (sper) {
+        if (per == 1) return launch_split<1> (s, G, stage, st);
+        if (per == 2) return launch_split<2> (s, G, stage, st);
+        return launch_split<3> (s, G, stage, st);
+      }
       if (s->sw_unit) {
         if (per == 1) return launch_unit<1> (s, G, stage, st);
         if (per == 2) return launch_unit<2> (s, G, stage, st);

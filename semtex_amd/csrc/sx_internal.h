@@ -143,6 +143,9 @@ struct sx_msys {
   int sw_panel = 0;            // 1: right-looking panel sweeps (sw_D off-diagonal blocks)
   int sw_D = 0;
   int sw_unit = 0;             // 1: off-diagonal panels scaled to W = T_J U (unit diagonal blocks)
+  char* d_xch = nullptr;       // split sweeps: per-chain message rings
+  unsigned* d_xep = nullptr;   //   and epochs
+  size_t xch_bytes = 0;
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks

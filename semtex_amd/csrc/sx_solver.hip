@@ -982,7 +982,7 @@ extern "C" int sx_msys_destroy (sx_msys_t s)
   void* ptrs[] = {s->d_betak2, s->d_nsolve, s->d_Bm, s->d_Hinv, s->d_Mrec, s->d_Mcon,
                   s->d_Mrec32, s->d_Mcon32, s->d_Bm32,
                   s->d_Hinv32, s->d_bandU, s->d_swF, s->d_swR, s->d_rhs, s->d_G,
-                  s->d_work, s->d_pc, s->d_tw, s->d_rhsb};
+                  s->d_work, s->d_pc, s->d_tw, s->d_rhsb, s->d_xch, s->d_xep};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete s;
   return SX_OK;
