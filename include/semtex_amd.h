@@ -92,7 +92,11 @@ int sx_msys_nsolve (sx_msys_t s, int mode);
 /* Layout of the factor's solve copies (diagnostic; no reference
  * counterpart): info[0] 1 = 64x64 panel sweeps, 0 = offset-major blocks;
  * info[1] off-diagonal panel blocks D; info[2] 1 = twisted two-piece solve;
- * info[3] band width nband.  Fills min(n, 4) entries. */
+ * info[3] band width nband; info[4] 1 = unit-diagonal panels (W = T_J U);
+ * info[5] kernel of the last band solve: 0 none yet, 1 panel sweeps with
+ * the triangle in the chain, 2 unit-diagonal sweeps, 3 split sweeps (two
+ * workgroups per chain); info[6] its right-hand sides per workgroup.
+ * Fills min(n, 7) entries. */
 int sx_msys_layout (sx_msys_t s, int* info, int n);
 
 /* Field::solve (src/field.cpp:598-709, DIRECT branch) over nz planes.

@@ -1869,6 +1869,8 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
     }
     if (sper) per = sper;
     const int G = nr / per;
+    s->last_kernel = sper ? 3 : s->sw_unit ? 2 : 1;
+    s->last_nr = per;
     auto sweep = [&] (int stage) {
       if (sper) {
         if (per == 1) return launch_split<1> (s, G, stage, st);

@@ -146,6 +146,10 @@ struct sx_msys {
   char* d_xch = nullptr;       // split sweeps: per-chain message rings
   unsigned* d_xep = nullptr;   //   and epochs
   size_t xch_bytes = 0;
+  // the panel sweep kernel of the last band solve (diagnostic): 0 none,
+  // 1 k_panel_sweeps, 2 k_unit_sweeps, 3 k_split_sweeps; right-hand sides
+  // per workgroup
+  mutable int last_kernel = 0, last_nr = 0;
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks

@@ -1000,8 +1000,9 @@ extern "C" int sx_msys_layout (sx_msys_t s, int* info, int n)
     sx::set_error ("sx_msys_layout: null system or output");
     return SX_ERR_ARG;
   }
-  const int v[4] = {s->sw_panel, s->sw_D, s->tw, s->m->nband};
-  for (int i = 0; i < n && i < 4; i++) info[i] = v[i];
+  const int v[7] = {s->sw_panel, s->sw_D, s->tw, s->m->nband, s->sw_unit, s->last_kernel,
+                    s->last_nr};
+  for (int i = 0; i < n && i < 7; i++) info[i] = v[i];
   return SX_OK;
 }
 

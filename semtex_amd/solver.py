@@ -83,10 +83,14 @@ class ModalMatrixSys:
         return _lib.load().sx_msys_nsolve(self.handle, mode)
 
     def layout(self):
-        """{"panels", "D", "twisted", "nband"} of the factor's solve copies."""
-        info = (C.c_int * 4)()
-        _lib.check(_lib.load().sx_msys_layout(self.handle, info, 4), "sx_msys_layout")
-        return {"panels": info[0], "D": info[1], "twisted": info[2], "nband": info[3]}
+        """{"panels", "D", "twisted", "nband", "unit", "kernel", "nr"} of the
+        factor's solve copies and of the last band solve (kernel: 0 none yet,
+        1 panel sweeps, 2 unit-diagonal sweeps, 3 split sweeps; nr:
+        right-hand sides per workgroup)."""
+        info = (C.c_int * 7)()
+        _lib.check(_lib.load().sx_msys_layout(self.handle, info, 7), "sx_msys_layout")
+        keys = ("panels", "D", "twisted", "nband", "unit", "kernel", "nr")
+        return dict(zip(keys, info))
 
     def __len__(self):
         return self.num_modes
