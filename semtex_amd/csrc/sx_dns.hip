@@ -295,10 +295,14 @@ __global__ void k_ew_addc (double* x, double c, size_t n)
 // waveProp (dns/integrate.cpp:204-239): H = sum_q -alpha[q+1] Us[q] + beta[q] dt Uf[q]
 __global__ void k_ew_wave (double* H, const double* us0, const double* us1,
                            const double* uf0, const double* uf1, double a1,
-                           double a2, double b0, double b1, int Je, size_t n)
+                           double a2, double b0, double b1, int Je, size_t n,
+                           double* unyq, size_t S)
 {
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // unyq (when given): the Nyquist plane 1 of us0 carried into u, which the
+  // solves leave alone
+  if (unyq && i >= S && i < 2 * S) unyq[i] = us0[i];
   double h = 0.0;
   h += -a1 * us0[i];
   h += b0 * uf0[i];
@@ -582,10 +586,10 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
 {
   const int nc = d->ncom;
   int rc;
-  for (int i = 0; i < nc; i++) {
-    SX_HIP (hipMemsetAsync (N[i], 0, sizeof (double) * d->n, st));
+  // N needs no clearing: k_adv_xy writes every element entry of it (the
+  // plane padding is never written and stays zero from sx_dns_set_state)
+  for (int i = 0; i < nc; i++)
     if ((rc = transform_to (d, U[i], d->phys[i], -1, st))) return rc;
-  }
   const bool conv = d->advection == SX_SKEW || d->toggle == 1;
   const bool cons = d->advection == SX_SKEW || d->toggle == 0;
   const sx_mesh* m = d->mu;
@@ -667,20 +671,17 @@ int dns_step (sx_dns* d, hipStream_t st)
     double* un2 = d->un[(d->un_head + 1) % nord];
     const int unnew_head = (d->un_head + nord - 1) % nord;
     double* unnew = d->un[unnew_head];
-    SX_HIP (hipMemsetAsync (hnew, 0, sizeof (double) * d->nz * d->nline, st));
-    // un_new is written into the level that is about to be dropped (un2)
-    // only after un2 has been read: use a separate scratch when nord == 2
-    double* unscr = d->plines;   // reused as scratch before the lines are formed
-    SX_HIP (hipMemsetAsync (unscr, 0, sizeof (double) * d->nz * d->nline, st));
+    // k_hopbc writes every entry of the HOPBC segments of hnew and unnew
+    // outright (the other lines are never written and stay zero from
+    // sx_dns_set_state); unnew is the level being dropped (un2 when
+    // nord == 2), which each thread reads before it writes the same entry
     const int nmodes = std::max (1, d->nz / 2);
     hipLaunchKernelGGL (k_hopbc, dim3 (d->nb, nmodes), dim3 (64), 0, st, d->mu->np,
                         d->seg_elmt, d->seg_idx, d->seg_nx, d->seg_ny, d->mu->d_geo,
                         d->mu->d_D, U[0], U[1], nc == 3 ? U[2] : U[0], N[0], N[1], hnew,
-                        unscr, un1, un2, Je, SSc[Je][0], SSc[Je][1], SSc[Je][2],
+                        unnew, un1, un2, Je, SSc[Je][0], SSc[Je][1], SSc[Je][2],
                         1.0 / d->dt, d->nu, d->beta, d->nz, d->nline, d->S);
     SX_HIP (hipGetLastError ());
-    SX_HIP (hipMemcpyAsync (unnew, unscr, sizeof (double) * d->nz * d->nline,
-                            hipMemcpyDeviceToDevice, st));
     d->un_head = unnew_head;
     const int Jb = std::min (d->step, nord);
     hipLaunchKernelGGL (k_plines, dim3 (nb256 ((size_t) d->nz * d->nline)), dim3 (256), 0,
@@ -692,14 +693,11 @@ int dns_step (sx_dns* d, hipStream_t st)
   // -- waveProp
   const int Je = std::min (d->step, nord);
   for (int i = 0; i < nc; i++)
+    // D->u after swapData holds u^n = Us[0]: only the Nyquist plane, which
+    // the solves leave alone, has to be carried over (by the same kernel)
     EW (k_ew_wave, d->H[i], d->Us[0][i], d->Us[1][i], d->Uf[0][i], d->Uf[1][i],
-        SSc[Je][1], SSc[Je][2], EXc[Je][0] * d->dt, EXc[Je][1] * d->dt, Je, d->n);
-  // D->u after swapData holds u^n = Us[0]: only the Nyquist plane, which
-  // the solves leave alone, has to be carried over
-  if (d->nz > 1)
-    for (int i = 0; i < nc; i++)
-      SX_HIP (hipMemcpyAsync (d->u[i] + d->S, d->Us[0][i] + d->S, sizeof (double) * d->S,
-                              hipMemcpyDeviceToDevice, st));
+        SSc[Je][1], SSc[Je][2], EXc[Je][0] * d->dt, EXc[Je][1] * d->dt, Je, d->n,
+        d->nz > 1 ? d->u[i] : nullptr, (size_t) d->S);
   // rollm (Uf)
   for (int i = 0; i < nc; i++) std::swap (d->Uf[0][i], d->Uf[1][i]);
   // -- setPForce: div u^ / dt
