@@ -336,16 +336,34 @@ typedef double v4d __attribute__ ((ext_vector_type (4)));
 // lane l supplies A[l%16][k + l/16] = M[k + l/16][a0 + l%16] and
 // B[k + l/16][l%16] = X[k + l/16][e0 + l%16]; D[4r + l/16][l%16] is
 // component r of lane l
+// The operands of 8 k-steps are read from LDS before their products, which
+// alternate between two accumulators: the LDS latency and the MFMA latency
+// overlap instead of adding up step by step.
+constexpr int MF_U = 8;
 __device__ __forceinline__ v4d mfma_tile (const double* __restrict__ M, int NA,
                                           const double* __restrict__ X, int K, int lane)
 {
-  v4d acc = {0.0, 0.0, 0.0, 0.0};
+  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
   const double* A = M + (lane >> 4) * NA + (lane & 15);
   const double* B = X + (lane >> 4) * CG_EP + (lane & 15);
-  for (int k = 0; k < K; k += 4)
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc,
-                                                0, 0, 0);
-  return acc;
+  int k = 0;
+  for (; k + 4 * MF_U <= K; k += 4 * MF_U) {
+    double a[MF_U], b[MF_U];
+#pragma unroll
+    for (int u = 0; u < MF_U; u++) {
+      a[u] = A[(size_t) (k + 4 * u) * NA];
+      b[u] = B[(size_t) (k + 4 * u) * CG_EP];
+    }
+#pragma unroll
+    for (int u = 0; u < MF_U; u += 2) {
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[u], b[u], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[u + 1], b[u + 1], acc1, 0, 0, 0);
+    }
+  }
+  for (; k < K; k += 4)
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc0,
+                                                 0, 0, 0);
+  return acc0 + acc1;
 }
 
 // dst(i) <- src(i) for i < n with SU loads in flight per thread (the loads
@@ -399,13 +417,27 @@ typedef float v4f __attribute__ ((ext_vector_type (4)));
 __device__ __forceinline__ v4f mfma_tile32 (const float* __restrict__ M, int NA,
                                             const float* __restrict__ X, int K, int lane)
 {
-  v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  v4f acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
   const float* A = M + (lane >> 4) * NA + (lane & 15);
   const float* B = X + (lane >> 4) * CG_EP + (lane & 15);
-  for (int k = 0; k < K; k += 4)
-    acc = __builtin_amdgcn_mfma_f32_16x16x4f32 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc,
-                                                0, 0, 0);
-  return acc;
+  int k = 0;
+  for (; k + 4 * MF_U <= K; k += 4 * MF_U) {
+    float a[MF_U], b[MF_U];
+#pragma unroll
+    for (int u = 0; u < MF_U; u++) {
+      a[u] = A[(size_t) (k + 4 * u) * NA];
+      b[u] = B[(size_t) (k + 4 * u) * CG_EP];
+    }
+#pragma unroll
+    for (int u = 0; u < MF_U; u += 2) {
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32 (a[u], b[u], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32 (a[u + 1], b[u + 1], acc1, 0, 0, 0);
+    }
+  }
+  for (; k < K; k += 4)
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc0,
+                                                 0, 0, 0);
+  return acc0 + acc1;
 }
 
 // dst[i] = src[i] for i < n floats (n a multiple of 4, 16-byte aligned)
