@@ -1850,7 +1850,7 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
     // launches that fit on the device at once.  SX_BAND_SPLIT=0/1 in the
     // environment forces them off/on (on: also for single-piece systems).
     int sper = 0;
-    if (s->sw_unit && s->d_xch && nr <= XMAXR) {
+    if (s->sw_unit && s->d_xch && nr <= XMAXR && !s->no_split) {
       const char* e = std::getenv ("SX_BAND_SPLIT");
       const bool want = e ? std::atoi (e) != 0 : s->tw != 0;
       if (want) {

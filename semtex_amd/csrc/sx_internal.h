@@ -150,6 +150,10 @@ struct sx_msys {
   // 1 k_panel_sweeps, 2 k_unit_sweeps, 3 k_split_sweeps; right-hand sides
   // per workgroup
   mutable int last_kernel = 0, last_nr = 0;
+  // 1: solves may run concurrently with other kernels (a batch graph's
+  // branches), so the split sweeps, which need their whole grid resident,
+  // are not used
+  int no_split = 0;
   size_t sw_stride = 0;        // doubles per mode of each sweep copy
   double* d_swF = nullptr;     // [nmodes][nblk][nband][B] forward sweep blocks
   double* d_swR = nullptr;     // [nmodes][nblk][nband][B] reversed (backward) blocks

@@ -1,6 +1,6 @@
 """Uneven batch: independent 2D problems of different order, size, aspect,
 lambda2, Cartesian and cylindrical (axisymmetric, axis on y = 0), solved by
-one captured HIP graph (sx_batch_*).
+one captured HIP graph (sx_batch_*), its problems on concurrent branches.
 
 * FP64 graph replays (three, with eager work and synchronisation between
   them) equal the eager per-problem solves exactly (same kernels);
@@ -50,10 +50,14 @@ def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems
     from semtex_amd.solver import SX_FP64, SX_MIXED, Batch
     print("case: uneven batch of", len(batch_problems), flush=True)
     built = _build(torch, batch_problems, SX_FP64)
+    # the batch first: its systems then solve as its graph does (no split
+    # band sweeps, whose grids must be resident alone), eagerly as well
+    B = Batch([(M, u, src) for (M, u, src, _) in built])
     eager = []
     for (M, u, src, _) in built:
         u.solve(src.clone(), M)
         eager.append(u.data.clone())
+        assert M.layout()["kernel"] != 3, M.layout()
     torch.cuda.synchronize()
     # FP64 oracle parity of the eager solves
     for p, e in zip(batch_problems, eager):
@@ -62,7 +66,6 @@ def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems
         uo = oracle.solve_modes(m, p["lambda2"], 0.0, p["forcing"][:, :n])[0]
         got = e[0, :n].cpu().numpy()
         assert np.linalg.norm(got - uo) / np.linalg.norm(uo) < 1e-12, (p["np"], p["cyl"])
-    B = Batch([(M, u, src) for (M, u, src, _) in built])
     for rep in range(3):
         for (_, u, _, _) in built:
             u.data.zero_()
@@ -83,3 +86,32 @@ def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems
             n = p["mesh"].nel * p["mesh"].npnp
             rel = float(torch.linalg.norm(u.data[0, :n] - e[0, :n]) / torch.linalg.norm(e[0, :n]))
             assert rel < 1e-10, (rep, p["np"], p["lambda2"], p["cyl"], rel)
+
+
+def test_batch_problems_sharing_a_system(torch_cuda, batch_problems):
+    """Problems that share a system (and so its workspace) run in order on
+    one branch of the graph: three forcings on one system, two on another,
+    each equal to its eager solve."""
+    import torch
+    from semtex_amd.solver import SX_FP64, Batch, Field
+    print("case: batch, 5 problems on 2 systems", flush=True)
+    built = _build(torch, batch_problems[:2], SX_FP64)
+    entries, refs = [], []
+    for j, (M, _, src, gm) in enumerate(built):
+        for r in range(3 - j):
+            u = Field(gm, 1)
+            s = src * (1.0 + r) + 0.25 * r
+            entries.append((M, u, s))
+    B = Batch(entries)
+    for (M, u, s) in entries:
+        v = Field(M.gmesh, 1)
+        v.solve(s.clone(), M)
+        refs.append(v.data.clone())
+    torch.cuda.synchronize()
+    for rep in range(2):
+        for (_, u, _) in entries:
+            u.data.zero_()
+        B.run()
+        torch.cuda.synchronize()
+        for (_, u, _), ref in zip(entries, refs):
+            assert torch.equal(u.data, ref), rep
