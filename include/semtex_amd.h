@@ -135,7 +135,7 @@ int sx_field_solve_pcg (sx_msys_t s, int nz, int plane_stride, double* d_f, doub
  * sx_field_solve.  The buffers are baked into the graph: refill
  * d_f_src / d_lines in place between runs.  sx_batch_run launches the
  * whole batch on `stream`.  Problems of different systems run concurrently
- * (branches of the graph over up to 8 streams), so their d_f_work and d_u
+ * (branches of the graph over 4 streams), so their d_f_work and d_u
  * buffers must not overlap; problems that share a system run in order.
  * Systems passed here no longer use the split band sweeps, which need their
  * whole grid resident at once.  No reference counterpart: for callers that
