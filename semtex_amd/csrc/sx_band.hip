@@ -1069,6 +1069,7 @@ __global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
 // the solution rather than as a hang.
 constexpr int XR = 8;                               // ring entries per direction
 constexpr int XSPIN = 1 << 19;                      // polls before a wait gives up
+                                                    // (SX_SPLIT_SPIN overrides)
 
 __device__ __forceinline__ void msg_put (__amdgpu_buffer_rsrc_t rs, int off, double v,
                                          unsigned ep, unsigned seq)
@@ -1087,10 +1088,12 @@ __device__ __forceinline__ void msg_put (__amdgpu_buffer_rsrc_t rs, int off, dou
 template <int NR>
 __device__ __forceinline__ void msg_wait (__amdgpu_buffer_rsrc_t rs, int off0, int rstride,
                                           unsigned ep, unsigned seq, double (&v)[NR],
-                                          bool& dead)
+                                          bool& dead, int spin)
 {
   typedef int v4i __attribute__ ((ext_vector_type (4)));
-  if (dead) {
+  // spin < 0: every wait gives up at once (the host forces the failure path)
+  if (dead || spin < 0) {
+    dead = true;
 #pragma unroll
     for (int r = 0; r < NR; r++) v[r] = __builtin_nan ("");
     return;
@@ -1105,7 +1108,7 @@ __device__ __forceinline__ void msg_wait (__amdgpu_buffer_rsrc_t rs, int off0, i
       ok = ok && (unsigned) t.z == ep && (unsigned) t.w == seq;
     }
     if (__builtin_amdgcn_ballot_w64 (!ok) == 0) return;
-    if (it >= XSPIN) break;
+    if (it >= spin) break;
     __builtin_amdgcn_s_sleep (2);
   }
   dead = true;
@@ -1118,7 +1121,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
-    char* xch, unsigned* epochs)
+    char* xch, unsigned* epochs, int spin)
 {
   constexpr int H = D / 2, FB = D - H;              // near blocks 1..H, far H+1..D
   constexpr int NA = UPW * (H + 1), NB = UPW * FB;  // pairs per wave
@@ -1233,7 +1236,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
 
       auto step = [&] (int J, double2 (&u)[NP], double (&bq)[NR]) __attribute__ ((always_inline)) {
         double pv[NR], zs[NR][2 * UPW];
-        msg_wait<NR> (xrs, xoff (1, J), MS, ep, seq (pass, J), pv, dead);
+        msg_wait<NR> (xrs, xoff (1, J), MS, ep, seq (pass, J), pv, dead, spin);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
           double z = pv[r];
@@ -1296,7 +1299,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
       // handed on (blocks that never entered the near ring keep b)
       for (int K = J1; K < J1 + D; K++) {
         double pv[NR];
-        msg_wait<NR> (xrs, xoff (1, K), MS, ep, seq (pass, K), pv, dead);
+        msg_wait<NR> (xrs, xoff (1, K), MS, ep, seq (pass, K), pv, dead, spin);
         if (wout && w == K - J1 && K * PB + lane < n) {
 #pragma unroll
           for (int r = 0; r < NR; r++) {
@@ -1336,7 +1339,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
       if (ran) {
         double dv[NR];
         const int Jp = pass ? g.f1 : 0;
-        msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (pass - 1, Jp), dv, dead);
+        msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (pass - 1, Jp), dv, dead, spin);
       }
       // blocks J0+H+1..J0+D-1 start empty here (their earlier far terms are
       // in the window the near side reads from b); p_K = 0 for K <= J0+H
@@ -1370,7 +1373,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
       auto step = [&] (int J, double2 (&u)[NP]) __attribute__ ((always_inline)) {
         if (J > J0 && w == 0) put_p (J + H);
         double zv[NR], zs[NR][2 * UPW];
-        msg_wait<NR> (xrs, xoff (0, J), MS, ep, seq (pass, J), zv, dead);
+        msg_wait<NR> (xrs, xoff (0, J), MS, ep, seq (pass, J), zv, dead, spin);
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -1479,6 +1482,15 @@ int launch_unit (const sx_msys* s, int G, int stage, hipStream_t st)
   return SX_ERR_LIMIT;
 }
 
+// polls before a wait of the split sweeps gives up: XSPIN, or SX_SPLIT_SPIN
+// in the environment (-1: every wait gives up at once, which tests use to
+// force the failure path)
+int split_spin ()
+{
+  const char* e = std::getenv ("SX_SPLIT_SPIN");
+  return e ? std::max (-1, std::atoi (e)) : XSPIN;
+}
+
 template <int NR, int D>
 size_t split_lds ()
 {
@@ -1521,7 +1533,7 @@ int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool pro
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
                       s->sw_stride, s->d_nsolve, s->d_rhs, s->nmax, s->nmodes, G,
                       s->d_tw, stage, s->tw_boff, s->d_rhsb, s->nmaxb, s->m->nband - 1,
-                      s->d_xch, s->d_xep);
+                      s->d_xch, s->d_xep, split_spin ());
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }

@@ -138,3 +138,23 @@ def test_dns_split_three_rhs_per_workgroup(torch_cuda, monkeypatch):
     for i in range(4):
         rel = np.linalg.norm(a[i][planes] - b[i][planes]) / np.linalg.norm(b[i][planes])
         assert rel < 1e-11, (i, rel)
+
+
+def test_split_wait_failure_gives_nan_then_recovers(torch_cuda, problems, monkeypatch):
+    """The split sweeps' bounded waits, forced from the host to give up at
+    once (SX_SPLIT_SPIN=-1): the kernels still exit, the solution is NaN
+    (the failure shows in the result), and the next solve with the normal
+    bound is correct again (epochs and rings stay consistent)."""
+    m, nz, F, lines, uo, _ = problems["D5"]
+    print("case: D5 split, forced wait failure then a normal solve", flush=True)
+    monkeypatch.setenv("SX_BAND_SPLIT", "1")
+    monkeypatch.setenv("SX_SPLIT_SPIN", "-1")
+    (bad,), lay = _solve(torch_cuda, m, nz, F, lines)
+    assert lay["kernel"] == 3, lay
+    n = m.nel * m.npnp
+    assert np.isnan(bad[0, :n]).any()
+    monkeypatch.delenv("SX_SPLIT_SPIN")
+    (good,), _ = _solve(torch_cuda, m, nz, F, lines)
+    for k in (0, 2, 3):
+        rel = np.linalg.norm(good[k, :n] - uo[k]) / np.linalg.norm(uo[k])
+        assert rel < 1e-12, (k, rel)
