@@ -1624,16 +1624,21 @@ static bool use_unit (int D)
   return unit_ok (D);
 }
 
-static bool use_panels (int nband)
+// The panel kernels address a piece's panels with 32-bit byte offsets (and
+// a buffer descriptor of 32-bit size): systems whose panels per mode would
+// pass 2^31 bytes (about 700,000 rows at D = 5) use the offset-major sweeps.
+static bool use_panels (int nband, int nmax)
 {
   if (const char* e = std::getenv ("SX_BAND_SWEEP"))
     if (std::string (e) == "offset") return false;
-  return panel_D (nband) <= PMAXD;
+  const size_t bytes = sizeof (double) * ((size_t) (nmax + PB - 1) / PB + 3) *
+                       (panel_D (nband) + 1) * PB * PB;
+  return panel_D (nband) <= PMAXD && bytes < ((size_t) 1 << 31);
 }
 
 size_t band_sweep_bytes_per_mode (int nmax, int nband)
 {
-  if (use_panels (nband))
+  if (use_panels (nband, nmax))
     return (size_t) ((nmax + PB - 1) / PB + 1) * (panel_D (nband) + 1) * PB * PB;
   const int B = band_block (nband);
   return (size_t) ((nmax + B - 1) / B) * nband * B;
@@ -1772,7 +1777,7 @@ static int band_factor_twisted (sx_msys* s, const std::vector<int>& tw, hipStrea
 int band_factor (sx_msys* s, hipStream_t st, int* d_status)
 {
   const int ld = s->m->nband;
-  const bool panels = use_panels (ld);
+  const bool panels = use_panels (ld, s->nmax);
   std::vector<int> tw;
   if (panels && twist_split (s, tw)) return band_factor_twisted (s, tw, st, d_status);
   hipLaunchKernelGGL (k_band_factor, dim3 (s->nmodes), dim3 (1024),
