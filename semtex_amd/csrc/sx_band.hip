@@ -443,7 +443,7 @@ enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
 // (and piece), NR of its G*NR right-hand sides each; they get block indices
 // congruent mod 8, i.e. the same XCD and L2.  False: nothing to do.
 struct SweepSeg {
-  int mode, n, f0, f1, b0, b1, rs, role, chain;
+  int mode, n, f0, f1, b0, b1, rs, role, chain, grp;
   bool wf, wb;
   size_t poff;
   double* bg;
@@ -463,6 +463,7 @@ __device__ __forceinline__ bool sweep_seg (SweepSeg& g, int NR, int D,
   const int grp = gr / roles;
   if (mode >= nmodes) return false;
   g.role = gr % roles;
+  g.grp = grp;
   g.chain = (piece * nmodes + mode) * G + grp;
   g.mode = mode;
   g.f0 = g.f1 = g.b0 = g.b1 = 0;
@@ -1133,6 +1134,20 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     return;
   const int n = g.n, rs = g.rs, role = g.role;
   double* bg = g.bg;
+  // Stage 2 of a twisted solve (the separator S = rows [m, m+S) of the top
+  // piece) folds in the two moves around it: S's right-hand side is read
+  // with the bottom piece's residual added (row m+a of b plus row nbot-1-a
+  // of the bottom piece's vector), and x_S is written to both.
+  double* rbS = nullptr;
+  int tm = 0, tnb = 0;
+  if (stage == 2) {
+    tm = tw[TW * g.mode + TW_M];
+    tnb = tw[TW * g.mode + TW_NBOT];
+    rbS = rhsb + ((size_t) g.mode * G + g.grp) * NR * nmaxb;
+  }
+  auto bS = [&] (int r, int row) __attribute__ ((always_inline)) {
+    return rbS ? rbS[(size_t) r * nmaxb + (tnb - 1 - (row - tm))] : 0.0;
+  };
   const int nb = (n + PB - 1) / PB;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
@@ -1201,7 +1216,10 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
           double y = part_at (J, 0)[r];
 #pragma unroll
           for (int s = 1; s < UW; s++) y += part_at (J, s)[r];
-          if (w == 0 && j < n) bg[(size_t) r * rs + gidx (j)] = y;
+          if (w == 0 && j < n) {
+            bg[(size_t) r * rs + gidx (j)] = y;
+            if (rbS && pass) rbS[(size_t) r * nmaxb + (tnb - 1 - (gidx (j) - tm))] = y;
+          }
         }
       };
       // window: blocks J0..J0+H-1 from b, one wave per block
@@ -1212,7 +1230,9 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
         for (int s = 0; s < UW; s++)
 #pragma unroll
           for (int r = 0; r < NR; r++)
-            ring_at (K, s)[r] = (in && s == 0) ? bg[(size_t) r * rs + gidx (K * PB + lane)] : 0.0;
+            ring_at (K, s)[r] = (in && s == 0) ? bg[(size_t) r * rs + gidx (K * PB + lane)] +
+                                                   (pass ? 0.0 : bS (r, gidx (K * PB + lane)))
+                                               : 0.0;
       }
       {
         const int v0 = vbase (J0), v1 = vbase (J0 + 1);
@@ -1222,14 +1242,18 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
           for (int d = 0; d <= H; d++)
             ua[k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
 #pragma unroll
-        for (int r = 0; r < NR; r++) ba[r] = bg[(size_t) r * rs + gidx ((J0 + H) * PB + lane)];
+        for (int r = 0; r < NR; r++)
+          ba[r] = bg[(size_t) r * rs + gidx ((J0 + H) * PB + lane)] +
+                  (pass ? 0.0 : bS (r, gidx ((J0 + H) * PB + lane)));
 #pragma unroll
         for (int k = 0; k < UPW; k++)
 #pragma unroll
           for (int d = 0; d <= H; d++)
             ub[k * (H + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
 #pragma unroll
-        for (int r = 0; r < NR; r++) bb[r] = bg[(size_t) r * rs + gidx ((J0 + H + 1) * PB + lane)];
+        for (int r = 0; r < NR; r++)
+          bb[r] = bg[(size_t) r * rs + gidx ((J0 + H + 1) * PB + lane)] +
+                  (pass ? 0.0 : bS (r, gidx ((J0 + H + 1) * PB + lane)));
       }
       __builtin_amdgcn_s_waitcnt (0);
       __syncthreads ();
@@ -1284,7 +1308,8 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
           for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[H][r]), "v"(bq[r]));
           const int gi = gidx ((J + H + 2) * PB + lane) + off;
 #pragma unroll
-          for (int r = 0; r < NR; r++) bq[r] = bg[(size_t) r * rs + gi];
+          for (int r = 0; r < NR; r++)
+            bq[r] = bg[(size_t) r * rs + gi] + (pass ? 0.0 : bS (r, gi));
         }
         lds_sync ();
       };
@@ -1922,6 +1947,13 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
       return e == hipSuccess ? (int) SX_OK : hip_fail (e, "k_twist_move");
     };
     int rc;
+    if (sper) {
+      // the split sweeps fold the separator's combine and x_S copy into stage 2
+      if ((rc = move (0)) || (rc = sweep (1)) || (rc = sweep (2)) || (rc = sweep (3)) ||
+          (rc = move (3)))
+        return rc;
+      return SX_OK;
+    }
     if ((rc = move (0)) || (rc = sweep (1)) || (rc = move (1)) || (rc = sweep (2)) ||
         (rc = move (2)) || (rc = sweep (3)) || (rc = move (3)))
       return rc;
