@@ -1169,6 +1169,33 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
   double2 ua[NP], ub[NP];
   double ba[NR], bb[NR];
   bool ran = false;                                  // a previous pass ran
+  // The next step's records, read at the start of a step before its panel
+  // reloads: a wait for them then leaves those reloads in flight (vector
+  // loads retire in order, so a wait for a load issued after them would
+  // not).  Taken when their tags match, else the step polls.
+  typedef int v4i __attribute__ ((ext_vector_type (4)));
+  v4i spec[NR];
+  int spec_k = -1;                                   // step whose records spec holds
+  auto spec_read = [&] (int dir, int K) __attribute__ ((always_inline)) {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      spec[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, xoff (dir, K) + r * MS, 0, 17);
+    spec_k = K;
+  };
+  // this step's records: the speculative ones when all lanes carry the tag
+  auto take = [&] (int dir, int K, unsigned sq, double (&v)[NR]) __attribute__ ((always_inline)) {
+    bool ok = spec_k == K && !dead;
+    if (ok) {
+      bool lane_ok = true;
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        v[r] = __longlong_as_double (((long long) spec[r].y << 32) | (unsigned) spec[r].x);
+        lane_ok = lane_ok && (unsigned) spec[r].z == ep && (unsigned) spec[r].w == sq;
+      }
+      ok = __builtin_amdgcn_ballot_w64 (!lane_ok) == 0;
+    }
+    if (!ok) msg_wait<NR> (xrs, xoff (dir, K), MS, ep, sq, v, dead, spin);
+  };
 
   for (int pass = 0; pass < 2; pass++) {
     const int J0 = pass ? g.b0 : g.f0, J1 = pass ? g.b1 : g.f1;
@@ -1260,7 +1287,8 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
 
       auto step = [&] (int J, double2 (&u)[NP], double (&bq)[NR]) __attribute__ ((always_inline)) {
         double pv[NR], zs[NR][2 * UPW];
-        msg_wait<NR> (xrs, xoff (1, J), MS, ep, seq (pass, J), pv, dead, spin);
+        take (1, J, seq (pass, J), pv);
+        if (J + 1 < J1) spec_read (1, J + 1);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
           double z = pv[r];
@@ -1398,7 +1426,8 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
       auto step = [&] (int J, double2 (&u)[NP]) __attribute__ ((always_inline)) {
         if (J > J0 && w == 0) put_p (J + H);
         double zv[NR], zs[NR][2 * UPW];
-        msg_wait<NR> (xrs, xoff (0, J), MS, ep, seq (pass, J), zv, dead, spin);
+        take (0, J, seq (pass, J), zv);
+        if (J + 1 < J1) spec_read (0, J + 1);
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
