@@ -310,30 +310,29 @@ __global__ void k_ew_wave (double* H, const double* us0, const double* us1,
   H[i] = h;
 }
 // setPForce (dns/integrate.cpp:242-257): f = (g0 + g1 [+ g2]) / dt
-// Field::smooth: ds[k][g] = imass[g] sum Q4 f over the element-boundary
-// slots of global node g, then written back to every slot.
-__global__ void k_smooth_sum (int npnp, int nglobal, const int* __restrict__ gptr,
-                              const int* __restrict__ gsrc, const int* __restrict__ emap,
-                              const double* __restrict__ geo, const double* __restrict__ imass,
-                              const double* __restrict__ f, double* ds, int S)
+// Field::smooth: the average imass[g] sum Q4 f over the element-boundary
+// slots of global node g, written back to every one of those slots.  The
+// slot lists of the global nodes are disjoint, so each thread reads and
+// writes only its own slots (one pass, no intermediate global array).
+__global__ void k_smooth (int npnp, int nglobal, const int* __restrict__ gptr,
+                          const int* __restrict__ gsrc, const int* __restrict__ emap,
+                          const double* __restrict__ geo, const double* __restrict__ imass,
+                          double* f, int S)
 {
   const int g = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (g >= nglobal) return;
+  double* fk = f + (size_t) k * S;
+  const int c0 = gptr[g], c1 = gptr[g + 1];
   double s = 0.0;
-  for (int c = gptr[g]; c < gptr[g + 1]; c++) {
+  for (int c = c0; c < c1; c++) {
     const int slot = gsrc[c], e = slot / npnp, ij = emap[slot % npnp];
-    s += geo[((size_t) e * NGEO + G_Q4) * npnp + ij] * f[(size_t) k * S + (size_t) e * npnp + ij];
+    s += geo[((size_t) e * NGEO + G_Q4) * npnp + ij] * fk[(size_t) e * npnp + ij];
   }
-  ds[(size_t) k * nglobal + g] = s * imass[g];
-}
-__global__ void k_smooth_put (int npnp, int next, int nglobal, const int* __restrict__ btog,
-                              const int* __restrict__ emap, const double* __restrict__ ds,
-                              double* f, int nel, int S)
-{
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-  if (idx >= nel * next) return;
-  const int e = idx / next, i = idx % next;
-  f[(size_t) k * S + (size_t) e * npnp + emap[i]] = ds[(size_t) k * nglobal + btog[idx]];
+  s *= imass[g];
+  for (int c = c0; c < c1; c++) {
+    const int slot = gsrc[c];
+    fk[(size_t) (slot / npnp) * npnp + emap[slot % npnp]] = s;
+  }
 }
 
 // Computed pressure BC terms on boundary segment b, Fourier mode m
@@ -510,7 +509,7 @@ struct sx_dns {
   double *u[3] = {}, *p = nullptr;
   double *Us[2][3] = {}, *Uf[2][3] = {};
   double *phys[3] = {}, *tmp = nullptr, *tmp2 = nullptr, *H[3] = {}, *F[3] = {};
-  double *imass = nullptr, *ds = nullptr;
+  double* imass = nullptr;
   int nb = 0, nline = 0, hop_head = 0, un_head = 0;
   int *seg_elmt = nullptr, *seg_idx = nullptr, *seg_kind = nullptr;
   double *seg_nx = nullptr, *seg_ny = nullptr;
@@ -566,12 +565,9 @@ int grad (sx_dns* d, const double* in, double* out, int dir, hipStream_t st)
 int smooth (sx_dns* d, double* f, hipStream_t st)
 {
   const sx_mesh* m = d->mu;
-  hipLaunchKernelGGL (k_smooth_sum, dim3 ((m->nglobal + 255) / 256, d->nz), dim3 (256), 0,
-                      st, m->npnp, m->nglobal, m->d_gptr, m->d_gsrc, m->d_emap, m->d_geo,
-                      d->imass, f, d->ds, d->S);
-  hipLaunchKernelGGL (k_smooth_put, dim3 ((m->nel * m->next + 255) / 256, d->nz),
-                      dim3 (256), 0, st, m->npnp, m->next, m->nglobal, m->d_btog, m->d_emap,
-                      d->ds, f, m->nel, d->S);
+  hipLaunchKernelGGL (k_smooth, dim3 ((m->nglobal + 255) / 256, d->nz), dim3 (256), 0, st,
+                      m->npnp, m->nglobal, m->d_gptr, m->d_gsrc, m->d_emap, m->d_geo, d->imass,
+                      f, d->S);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -822,7 +818,7 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
         mass[mu->btog[(size_t) e * mu->next + i]] += geo[G_Q4 * npnp + mu->emap[i]];
     }
     for (double& v : mass) v = 1.0 / v;
-    if (!(d->imass = d->alloc (mass.size ())) || !(d->ds = d->alloc ((size_t) nz * mu->nglobal)))
+    if (!(d->imass = d->alloc (mass.size ())))
       return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
     if (hipMemcpy (d->imass, mass.data (), sizeof (double) * mass.size (), hipMemcpyHostToDevice))
       return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
