@@ -47,7 +47,7 @@ def oracle_solve(oracle, mesh, lambda2, beta, nz, F, lines=None):
     return oracle.solve_modes(mesh, lambda2, beta, planes, lines)
 
 
-def test_config1_square_parity(torch_cuda, oracle):
+def test_square_4x4_np8_parity(torch_cuda, oracle):
     from semtex_amd.problems import helmholtz_2d_square
     P = helmholtz_2d_square()
     m = P["mesh"]
@@ -211,7 +211,7 @@ def test_full_fourier_batch_parity_and_residual(torch_cuda, oracle):
 def test_mixed_precision_refinement_reaches_fp64(torch_cuda, oracle):
     """SX_MIXED: FP32 elemental condensation/recovery + FP64 band solve +
     FP64 residual refinement agrees with the FP64 path to <= 1e-10 relative
-    on the full 16x16 N_P=10 N_Z=64 batch, and with
+    (the mixed path's accuracy bar) on the full 16x16 N_P=10 N_Z=64 batch, and with
     boundary data on a small periodic problem."""
     import torch
     from semtex_amd.mesh import rect_mesh

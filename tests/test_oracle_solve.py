@@ -139,7 +139,7 @@ def test_session_laplace4_neumann_top(oracle):
     assert np.max(np.abs(u - ex)) < 1e-10
 
 
-def test_config1_manufactured(oracle):
+def test_square_4x4_np8_manufactured(oracle):
     """4x4 elements, N_P=8, lambda2=1, u = sin(pi x) sin(pi y)."""
     m = rect_mesh(4, 4, 8)
     ex = np.sin(np.pi * m.x) * np.sin(np.pi * m.y)
