@@ -393,11 +393,8 @@ int launch_fft2_t (int S, const double* in, double* out, int sign, const double2
 // planes of 18,432 columns); SX_FFT_THREADS=256 selects the larger tile.
 int fft_threads ()
 {
-  static const int t = [] {
-    const char* e = getenv ("SX_FFT_THREADS");
-    return e && atoi (e) == 256 ? 256 : 128;
-  }();
-  return t;
+  const char* e = getenv ("SX_FFT_THREADS");
+  return e && atoi (e) == 256 ? 256 : 128;
 }
 
 template <int N, int R1>

@@ -126,3 +126,44 @@ def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup, nz):
             planes = [k for k in range(nz) if k != 1]
             rel = np.linalg.norm(g[planes] - o[planes]) / np.linalg.norm(o[planes])
             assert rel < 1e-11, (s, i, rel)
+
+
+def _channel_run(torch, monkeypatch, tile, nz=32, steps=3):
+    from semtex_amd.mesh import edgemaps, rect_mesh
+    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh
+    monkeypatch.setenv("SX_FFT_THREADS", tile)
+    bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
+    mu = rect_mesh(4, 6, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
+    mp = rect_mesh(4, 6, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
+    beta, nu, dt = 1.0, 1.0 / 100, 2e-3
+    D = DNS(GpuMesh(mu), GpuMesh(mp), nz, beta, nu, dt, const_force=(2 * nu, 0.0, 0.0),
+            advection=SX_SKEW)
+    n, S = mu.nel * mu.npnp, D.stride
+    rng = np.random.default_rng(7)
+    wall = 1.0 - mu.y.ravel() ** 2
+    em, _ = edgemaps(mu.np)
+    u0 = np.zeros((3, nz, S))
+    for i in range(3):
+        u0[i, :, :n] = (wall if i == 0 else 0.0) + 0.1 * rng.uniform(-1, 1, (nz, n)) * wall
+        for k in range(nz):
+            f = u0[i, k, :n].reshape(mu.nel, -1)
+            g = np.zeros(mu.nglobal)
+            g[mu.btog] = f[:, em[:mu.next]]
+            f[:, em[:mu.next]] = g[mu.btog]
+            u0[i, k, :n] = f.ravel()
+    D.set_state(torch.from_numpy(u0).cuda())
+    D.step(steps)
+    torch.cuda.synchronize()
+    return [D.field(i).cpu().numpy()[:, :n] for i in range(4)]
+
+
+def test_dns_transform_tiles_agree(torch_cuda, monkeypatch):
+    """The DNS step's fused z transforms on 128- and 256-thread tiles (the
+    tile only regroups columns; each column's arithmetic is the same)."""
+    print("case: channel 4x6 np=6 nz=32 skew, 3 steps, transform tiles 128 vs 256", flush=True)
+    a = _channel_run(torch_cuda, monkeypatch, "128")
+    b = _channel_run(torch_cuda, monkeypatch, "256")
+    planes = [0] + list(range(2, 32))
+    for i in range(4):
+        rel = np.linalg.norm(a[i][planes] - b[i][planes]) / np.linalg.norm(b[i][planes])
+        assert rel < 1e-13, (i, rel)

@@ -18,13 +18,20 @@ def torch_cuda():
     return torch
 
 
-@pytest.mark.parametrize("nz,S", [(8, 484), (16, 1000), (32, 777), (64, 130), (128, 18432),
-                                  (128, 35), (256, 333), (1024, 70), (12, 500), (2, 64)])
-def test_transform_matches_oracle(torch_cuda, oracle, nz, S):
+SIZES = [(8, 484), (16, 1000), (32, 777), (64, 130), (128, 18432), (128, 35), (256, 333),
+         (1024, 70), (12, 500), (2, 64)]
+# the two-level transform (nz 32..256) also on its 256-thread tile
+CASES = [(nz, S, "128") for nz, S in SIZES] + \
+        [(nz, S, "256") for nz, S in SIZES if nz in (32, 64, 128, 256)]
+
+
+@pytest.mark.parametrize("nz,S,tile", CASES)
+def test_transform_matches_oracle(torch_cuda, oracle, monkeypatch, nz, S, tile):
     import torch
     from oracle import oracle as O
     from semtex_amd.solver import fourier_transform
-    print("case: fourier nz", nz, "S", S, flush=True)
+    monkeypatch.setenv("SX_FFT_THREADS", tile)
+    print("case: fourier nz", nz, "S", S, "tile", tile, flush=True)
     rng = np.random.default_rng(nz * 7 + S)
     x = rng.standard_normal((nz, S))
     want_f = O.dft(x, nz, S, +1)
