@@ -31,6 +31,8 @@ extern "C" {
 #define SX_ERR_GEOM     -3   /* non-positive element Jacobian              */
 #define SX_ERR_FACTOR   -4   /* Cholesky breakdown (matrix not SPD)        */
 #define SX_ERR_LIMIT    -5   /* problem exceeds an implemented size limit  */
+#define SX_ERR_SYNC     -6   /* a split band sweep's bounded wait gave up:  */
+                             /* the affected solutions hold NaN            */
 
 #define SX_ESSENTIAL 0       /* boundary kinds (src/condition.h)           */
 #define SX_NATURAL   1
@@ -87,6 +89,20 @@ int sx_msys_create (sx_mesh_t m, double lambda2, double beta, int base_mode,
                     int num_modes, int precision, void* stream,
                     sx_msys_t* out);
 int sx_msys_destroy (sx_msys_t s);
+/* Status of the solves issued on this system since the last query:
+ * synchronises `stream`, returns SX_OK, or SX_ERR_SYNC (and clears the
+ * status) when a wait of the split band sweeps gave up.  The split sweeps
+ * run each chain of block steps on two workgroups that exchange partial
+ * sums and need both resident at once; the library launches them only when
+ * the whole grid fits on the device, and if another process's kernels keep
+ * a partner workgroup off the device past the wait bound, that solve's
+ * results are NaN instead of the kernel hanging.  sx_field_solve and
+ * sx_field_solve_mixed run asynchronously on their stream, so a caller that
+ * keeps the reference's message(ERROR) behaviour (veclib/util.c:24-46) calls
+ * this after the stream work it depends on; sx_field_solve_host checks it
+ * itself.  No reference counterpart (the CPU band solve cannot fail this
+ * way). */
+int sx_msys_status (sx_msys_t s, void* stream);
 /* Unknowns solved for mode index `mode` (0-based within the system). */
 int sx_msys_nsolve (sx_msys_t s, int mode);
 /* Layout of the factor's solve copies (diagnostic; no reference
@@ -137,8 +153,9 @@ int sx_field_solve_pcg (sx_msys_t s, int nz, int plane_stride, double* d_f, doub
  * whole batch on `stream`.  Problems of different systems run concurrently
  * (branches of the graph over 4 streams), so their d_f_work and d_u
  * buffers must not overlap; problems that share a system run in order.
- * Systems passed here no longer use the split band sweeps, which need their
- * whole grid resident at once.  No reference counterpart: for callers that
+ * The recorded solves use no split band sweeps (they need their whole grid
+ * resident at once); sx_field_solve on the same systems outside the batch
+ * keeps them.  No reference counterpart: for callers that
  * solve many independent small problems (uneven batches). */
 typedef struct sx_batch* sx_batch_t;
 int sx_batch_create (int n, const sx_msys_t* systems, const int* nz, const int* plane_stride,
@@ -188,15 +205,19 @@ int sx_fourier_transform (int nz, int plane_stride, const double* d_in, double* 
 /* ------------------------------------------------------------------------
  * DNS time step: integrate() (dns/integrate.cpp:66-201) for Cartesian
  * 2D (nz == 1: u, v, p) or 2.5D (nz even >= 2: u, v, w, p) flows.
- * mesh_u: velocity numbering (its essential segments are no-slip walls
- * with zero velocity), mesh_p: pressure numbering, same boundary segments
- * in the same order; natural segments of mesh_p carry the computed
- * high-order pressure condition (BCmgr::maintainFourier/evaluateCNBCp,
+ * mesh_u: velocity numbering (its essential segments carry the velocity
+ * boundary values of sx_dns_set_velocity_bc, zero by default: no-slip
+ * walls), mesh_p: pressure numbering, same boundary segments in the same
+ * order; natural segments of mesh_p carry the computed high-order
+ * pressure condition (BCmgr::maintainFourier/evaluateCNBCp,
  * src/bcmgr.cpp:1037-1335).  advection: SX_SKEW (skewSymmetric, "-S") or
  * SX_ALTSKEW (altSkewSymmetric, the driver default), dns/nonlinear.cpp.
  * const_force: ConstForce CONST_X/Y/Z added to mode 0 (may be NULL).
- * The first nord-1 steps use the first-order direct solve (the limit of
- * the reference's start-up Jacobi-PCG, dns/integrate.cpp:340-353).
+ * nord: N_TIME, 1..3 (Integration::StifflyStable/Extrapolation,
+ * src/integration.cpp:99-164).  Step s < nord runs at order s: by default
+ * with the direct solve of that order's system (the limit of the
+ * reference's start-up Jacobi-PCG, dns/integrate.cpp:340-353), or with
+ * JACPCG after sx_dns_set_startup.
  * ---------------------------------------------------------------------- */
 #define SX_SKEW    0
 #define SX_ALTSKEW 1
@@ -209,6 +230,13 @@ int sx_dns_destroy (sx_dns_t d);
  * [ncom][nz][plane_stride] (device) -> forward transform, Nyquist plane
  * zeroed; pressure and history cleared, step = 0. */
 int sx_dns_set_state (sx_dns_t d, const double* d_u_phys, void* stream);
+/* Steady velocity boundary values: d_lines_phys [ncom][nz][nbound*np]
+ * (device) in physical space, segment b's np values in CCW traverse order,
+ * as Field::evaluateBoundaries fills _line (src/field.cpp:206-240); they
+ * are forward transformed once as Field::bTransform does
+ * (src/field.cpp:174-203) and used by every velocity solve (only the
+ * essential segments' values matter).  NULL resets them to zero. */
+int sx_dns_set_velocity_bc (sx_dns_t d, const double* d_lines_phys, void* stream);
 /* Advance nsteps steps of the integrate() loop body. */
 int sx_dns_step (sx_dns_t d, int nsteps, void* stream);
 /* Velocity solves of the start-up steps (step < nord): SX_FP64 (direct,

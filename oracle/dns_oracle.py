@@ -133,6 +133,9 @@ class DnsOracle:
         self.Uf = [np.zeros((nc, nz, self.n)) for _ in range(nord)]
         self.un = np.zeros((nord, nz, self.nline))
         self.hopbc = np.zeros((nord, nz, self.nline))
+        # velocity boundary values in Fourier space, [ncom][nz][nline], or
+        # None for zero data (the essential segments are then no-slip walls)
+        self.ulines = None
 
     # ---------------------------------------------------------------- ops
     def grad_xy(self, f, d):
@@ -237,6 +240,22 @@ class DnsOracle:
     def physical(self, F):
         return inv(F)
 
+    def set_velocity_bc(self, lines_phys):
+        """Steady velocity boundary values given per z plane in physical
+        space, [ncom][nz][nline] (segment b's np values in CCW traverse
+        order): evaluated, then forward transformed as Field::bTransform
+        does (src/field.cpp:174-203: for nz == 2 the first plane keeps its
+        physical values and the second is zeroed).  Only the essential
+        segments' values are used."""
+        L = np.asarray(lines_phys, float)
+        out = np.zeros_like(L)
+        for i in range(self.ncom):
+            if self.nz == 2:
+                out[i][0] = L[i][0]
+            else:
+                out[i] = fwd(L[i])
+        self.ulines = out
+
     # -------------------------------------------------------------- HOPBC
     def side_vals(self, F, b):
         e, idx = self.seg_nodes[b]
@@ -285,9 +304,12 @@ class DnsOracle:
                     continue
                 un_new[k, sl] = self.side_vals(U[0][k], b) * nx + self.side_vals(U[1][k], b) * ny
                 if Je:
+                    # as src/bcmgr.cpp:1124-1130 forms it: every alpha[q+1]
+                    # multiplies the previous step's u.n (_un[0], before the
+                    # roll), not level q
                     work = alpha[0] * un_new[k, sl]
                     for q in range(Je):
-                        work = work + alpha[q + 1] * self.un[q, k, sl]
+                        work = work + alpha[q + 1] * self.un[0, k, sl]
                     self.hopbc[0, k, sl] = -work / self.dt
                 self.hopbc[0, k, sl] += (self.side_vals(N[0][k], b) * nx +
                                          self.side_vals(N[1][k], b) * ny)
@@ -361,13 +383,14 @@ class DnsOracle:
         # viscous substep (first-order constant until the history is full)
         lam2 = SS[Je][0] / (dt * nu)
         for i in range(self.ncom):
+            li = None if self.ulines is None else self.ulines[i]
             if self.startup == "pcg" and self.step_no < nord:
-                self.u[i] = self.solve_pcg(lam2, F[i], u_old[i])
+                self.u[i] = self.solve_pcg(lam2, F[i], u_old[i], li)
             else:
-                self.u[i] = self.solve("u", lam2, F[i], None, u_old[i])
+                self.u[i] = self.solve("u", lam2, F[i], li, u_old[i])
         return self
 
-    def solve_pcg(self, lam2, F, u0):
+    def solve_pcg(self, lam2, F, u0, lines=None):
         """JACPCG velocity solve of every plane from the current field u0
         (Field::solve, src/field.cpp:711-805); the Nyquist plane keeps u0."""
         out = u0.copy()
@@ -377,6 +400,6 @@ class DnsOracle:
             key = ("upcg", lam2, k >> 1)
             if key not in self._sys:
                 self._sys[key] = O.OracleMsys(self.om_u, lam2, (self.beta * (k >> 1)) ** 2)
-            out[k], _ = self._sys[key].solve_plane_pcg(F[k], None, u0[k], self.tol_rel,
-                                                       self.step_max)
+            out[k], _ = self._sys[key].solve_plane_pcg(F[k], None if lines is None else lines[k],
+                                                       u0[k], self.tol_rel, self.step_max)
         return out

@@ -12,6 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsemtex_amd.so")
 
 SX_OK = 0
+SX_ERR_ARG, SX_ERR_HIP, SX_ERR_GEOM, SX_ERR_FACTOR, SX_ERR_LIMIT, SX_ERR_SYNC = -1, -2, -3, -4, -5, -6
 SX_FP64, SX_MIXED, SX_JACPCG = 0, 1, 2
 SX_SKEW, SX_ALTSKEW = 0, 1
 
@@ -29,6 +30,7 @@ SIGNATURES = {
     "sx_msys_destroy": (_i, [_vp]),
     "sx_msys_nsolve": (_i, [_vp, _i]),
     "sx_msys_layout": (_i, [_vp, C.POINTER(C.c_int), _i]),
+    "sx_msys_status": (_i, [_vp, _vp]),
     "sx_field_solve": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
     "sx_field_solve_mixed": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i,
                                   C.POINTER(_i), _vp]),
@@ -36,6 +38,7 @@ SIGNATURES = {
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
     "sx_fourier_transform": (_i, [_i, _i, _dp, _dp, _i, _vp]),
     "sx_dns_set_startup": (_i, [_vp, _i, _d, _i, _vp]),
+    "sx_dns_set_velocity_bc": (_i, [_vp, _dp, _vp]),
     "sx_field_solve_weak": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
     "sx_batch_create": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
     "sx_batch_run": (_i, [_vp, _vp]),
@@ -59,7 +62,9 @@ _lib = None
 
 
 class SxError(RuntimeError):
-    pass
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load():
@@ -80,5 +85,5 @@ def load():
 def check(rc, what=""):
     if rc != SX_OK:
         msg = load().sx_last_error().decode()
-        raise SxError(f"{what} failed ({rc}): {msg}")
+        raise SxError(f"{what} failed ({rc}): {msg}", rc)
     return rc

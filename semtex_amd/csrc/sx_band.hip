@@ -1118,11 +1118,11 @@ __device__ __forceinline__ void msg_wait (__amdgpu_buffer_rsrc_t rs, int off0, i
 }
 
 template <int NR, int D>
-__global__ __launch_bounds__(UW * 64) void k_split_sweeps (
+__device__ __forceinline__ void split_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
-    char* xch, unsigned* epochs, int spin)
+    char* xch, unsigned ep, int spin, bool& dead)
 {
   constexpr int H = D / 2, FB = D - H;              // near blocks 1..H, far H+1..D
   constexpr int NA = UPW * (H + 1), NB = UPW * FB;  // pairs per wave
@@ -1159,13 +1159,10 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
   auto xoff = [&] (int dir, int K) __attribute__ ((always_inline)) {
     return ((dir * XR + (K & (XR - 1))) * NR) * MS + 16 * lane;
   };
-  const unsigned ep = __hip_atomic_load (epochs + g.chain, __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
   // the top bit keeps a never-written (zero) record from matching
   auto seq = [&] (int pass, int K) __attribute__ ((always_inline)) {
     return 0x80000000u | ((unsigned) pass << 24) | (unsigned) K;
   };
-  bool dead = false;
   double2 ua[NP], ub[NP];
   double ba[NR], bb[NR];
   bool ran = false;                                  // a previous pass ran
@@ -1475,9 +1472,37 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     __threadfence ();
     __syncthreads ();
   }
-  // the near side closes the launch: the next one uses the next epoch
-  if (role == 0 && threadIdx.x == 0)
-    __hip_atomic_store (epochs + g.chain, ep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// sync[0]: the epoch of the system's current launch, sync[1]: workgroups of
+// that launch that have finished, sync[2]: status (bit 0: some wait gave up
+// since the host last read it).  Every workgroup reads the epoch when it
+// starts; the last one to finish advances it, so both workgroups of a chain
+// always use one epoch (also when one of them starts only after its partner
+// has given up and finished), and a record written under an earlier launch
+// of any layout never carries the current epoch.
+template <int NR, int D>
+__global__ __launch_bounds__(UW * 64) void k_split_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
+    char* xch, unsigned* sync, int spin)
+{
+  const unsigned ep = __hip_atomic_load (sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  bool dead = false;
+  split_body<NR, D> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
+                     nmaxb, kd, xch, ep, spin, dead);
+  const int any_dead = __syncthreads_or (dead ? 1 : 0);
+  if (threadIdx.x == 0) {
+    if (any_dead)
+      __hip_atomic_fetch_or (sync + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned done = __hip_atomic_fetch_add (sync + 1, 1u, __ATOMIC_ACQ_REL,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    if (done == gridDim.x - 1) {
+      __hip_atomic_store (sync + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store (sync, ep + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 }
 
 // W_{J,J+d} = T_J U_{J,J+d} in place of U_{J,J+d}, d = 1..D: one workgroup
@@ -1606,17 +1631,18 @@ int launch_split (const sx_msys* s, int G, int stage, hipStream_t st, bool probe
   return SX_ERR_LIMIT;
 }
 
-// message rings and epochs of the split sweeps: up to 2 pieces x nmodes x 6
-// right-hand sides (one ring pair per right-hand side and chain)
+// message rings of the split sweeps: up to 2 pieces x nmodes x 6
+// right-hand sides (one ring pair per right-hand side and chain), and the
+// system's epoch, arrival count and status words (k_split_sweeps)
 constexpr int XMAXR = 6;
 int split_alloc (sx_msys* s, hipStream_t st)
 {
   const size_t rows = (size_t) 2 * s->nmodes * XMAXR;
   s->xch_bytes = rows * 2 * XR * PB * 16;
   SX_HIP (hipMalloc (&s->d_xch, s->xch_bytes));
-  SX_HIP (hipMalloc (&s->d_xep, sizeof (unsigned) * rows));
+  SX_HIP (hipMalloc (&s->d_xep, sizeof (unsigned) * 4));
   SX_HIP (hipMemsetAsync (s->d_xch, 0, s->xch_bytes, st));
-  SX_HIP (hipMemsetAsync (s->d_xep, 0, sizeof (unsigned) * rows, st));
+  SX_HIP (hipMemsetAsync (s->d_xep, 0, sizeof (unsigned) * 4, st));
   return SX_OK;
 }
 
@@ -2010,5 +2036,23 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
              " outside the instantiated range (nband <= 768)");
   return SX_ERR_LIMIT;
 }
+
+// status word of the split sweeps: SX_OK, or SX_ERR_SYNC (and the word
+// cleared) when a wait of some launch since the last query gave up
+int split_status (sx_msys* s, hipStream_t st)
+{
+  if (!s->d_xep) return SX_OK;
+  unsigned w[4] = {0, 0, 0, 0};
+  SX_HIP (hipMemcpyAsync (w, s->d_xep, sizeof w, hipMemcpyDeviceToHost, st));
+  SX_HIP (hipStreamSynchronize (st));
+  if (!(w[2] & 1u)) return SX_OK;
+  const unsigned zero = 0;
+  SX_HIP (hipMemcpyAsync (s->d_xep + 2, &zero, sizeof zero, hipMemcpyHostToDevice, st));
+  SX_HIP (hipStreamSynchronize (st));
+  set_error ("band solve: a wait of the split sweeps gave up (a partner workgroup was not "
+             "resident); the affected solutions hold NaN");
+  return SX_ERR_SYNC;
+}
+
 
 }  // namespace sx

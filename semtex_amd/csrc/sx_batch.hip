@@ -107,9 +107,17 @@ extern "C" int sx_batch_create (int n, const sx_msys_t* systems, const int* nz,
     if (hipStreamCreate (&b->lane[l]) != hipSuccess ||
         hipEventCreateWithFlags (&b->join[l], hipEventDisableTiming) != hipSuccess)
       return fail (hip_fail (hipErrorUnknown, "lane stream"));
-  // the branches run concurrently: no split sweeps (they need their whole
-  // grid resident at once)
-  for (int i = 0; i < n; i++) systems[i]->no_split = 1;
+  // the branches run concurrently: the recorded launches use no split
+  // sweeps (they need their whole grid resident at once); the systems'
+  // own setting is restored once the graph is recorded
+  struct NoSplit {
+    std::vector<std::pair<sx_msys*, int>> saved;
+    ~NoSplit () { for (auto it = saved.rbegin (); it != saved.rend (); ++it) it->first->no_split = it->second; }
+  } guard;
+  for (int i = 0; i < n; i++) {
+    guard.saved.emplace_back (systems[i], systems[i]->no_split);
+    systems[i]->no_split = 1;
+  }
   // one eager pass first: workspaces and kernel attributes are set up
   // outside the capture
   int rc = record (b, n, systems, nz, plane_stride, d_f_src, d_f_work, d_u, d_lines,

@@ -292,21 +292,27 @@ __global__ void k_ew_addc (double* x, double c, size_t n)
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += c;
 }
-// waveProp (dns/integrate.cpp:204-239): H = sum_q -alpha[q+1] Us[q] + beta[q] dt Uf[q]
-__global__ void k_ew_wave (double* H, const double* us0, const double* us1,
-                           const double* uf0, const double* uf1, double a1,
-                           double a2, double b0, double b1, int Je, size_t n,
-                           double* unyq, size_t S)
+// Time levels q = 0..Je-1 of one field (Je <= 3) and their coefficients.
+struct Levels {
+  const double* f[3];
+  double c[3];
+};
+struct Coef3 { double c[3]; };
+// waveProp (dns/integrate.cpp:204-239): H = sum_q -alpha[q+1] Us[q] + beta[q] dt Uf[q],
+// summed in the reference's order (q ascending, the Us term first)
+__global__ void k_ew_wave (double* H, Levels us, Levels uf, int Je, size_t n, double* unyq,
+                           size_t S)
 {
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   // unyq (when given): the Nyquist plane 1 of us0 carried into u, which the
   // solves leave alone
-  if (unyq && i >= S && i < 2 * S) unyq[i] = us0[i];
+  if (unyq && i >= S && i < 2 * S) unyq[i] = us.f[0][i];
   double h = 0.0;
-  h += -a1 * us0[i];
-  h += b0 * uf0[i];
-  if (Je > 1) { h += -a2 * us1[i]; h += b1 * uf1[i]; }
+  for (int q = 0; q < Je; q++) {
+    h += us.c[q] * us.f[q][i];
+    h += uf.c[q] * uf.f[q][i];
+  }
   H[i] = h;
 }
 // setPForce (dns/integrate.cpp:242-257): f = (g0 + g1 [+ g2]) / dt
@@ -337,8 +343,12 @@ __global__ void k_smooth (int npnp, int nglobal, const int* __restrict__ gptr,
 
 // Computed pressure BC terms on boundary segment b, Fourier mode m
 // (BCmgr::maintainFourier, src/bcmgr.cpp:1037-1296; Edge::curlCurl,
-// src/edge.cpp:98-220), Cartesian.  hop_new and un_new are level 0 of
-// the rolled storage; un1, un2 the previous two levels of u.n.
+// src/edge.cpp:98-220), Cartesian.  hop_new is level 0 of the rolled
+// storage, un_new receives this step's u.n and un1 holds the previous
+// step's.  The time-derivative term is formed as the reference forms it
+// (src/bcmgr.cpp:1124-1130): alpha_0 u.n + sum_{q<Je} alpha_{q+1} times
+// the *previous* level for every q, i.e. a0 u.n + asum un1 with asum =
+// alpha_1 + .. + alpha_Je.
 __global__ __launch_bounds__(64) void k_hopbc (
     int np, const int* __restrict__ seg_elmt, const int* __restrict__ seg_idx,
     const double* __restrict__ seg_nx, const double* __restrict__ seg_ny,
@@ -346,9 +356,8 @@ __global__ __launch_bounds__(64) void k_hopbc (
     const double* __restrict__ U0, const double* __restrict__ U1,
     const double* __restrict__ U2, const double* __restrict__ N0,
     const double* __restrict__ N1, double* hop_new, double* un_new,
-    const double* __restrict__ un1, const double* __restrict__ un2, int Je,
-    double a0, double a1, double a2, double inv_dt, double nu, double beta,
-    int nz, int nline, int S)
+    const double* __restrict__ un1, int Je, double a0, Coef3 acoef, double inv_dt, double nu,
+    double beta, int nz, int nline, int S)
 {
   const int npnp = np * np, b = blockIdx.x, m = blockIdx.y, lane = threadIdx.x;
   const int e = seg_elmt[b];
@@ -423,8 +432,7 @@ __global__ __launch_bounds__(64) void k_hopbc (
       double h = 0.0;
       if (Je) {
         double w = a0 * un;
-        w += a1 * un1[(size_t) k * nline + jl];
-        if (Je > 1) w += a2 * un2[(size_t) k * nline + jl];
+        for (int q = 0; q < Je; q++) w += acoef.c[q] * un1[(size_t) k * nline + jl];
         h = -inv_dt * w;
       }
       double nn = 0.0;
@@ -474,18 +482,15 @@ __global__ __launch_bounds__(64) void k_hopbc (
 
 // evaluateCNBCp: lines[k] = sum_q beta_q hop[q][k]; Nyquist plane and
 // essential pressure segments zero.
-__global__ void k_plines (double* lines, const double* h0, const double* h1,
-                          double b0, double b1, int Je, int nz, int nline, int np,
+__global__ void k_plines (double* lines, Levels hop, int Je, int nz, int nline, int np,
                           const int* __restrict__ kind)
 {
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t) nz * nline) return;
   const int k = (int) (i / nline), j = (int) (i % nline);
   double v = 0.0;
-  if (!(nz > 1 && k == 1) && kind[j / np] == SX_NATURAL) {
-    v += b0 * h0[i];
-    if (Je > 1) v += b1 * h1[i];
-  }
+  if (!(nz > 1 && k == 1) && kind[j / np] == SX_NATURAL)
+    for (int q = 0; q < Je; q++) v += hop.c[q] * hop.f[q][i];
   lines[i] = v;
 }
 
@@ -499,21 +504,22 @@ struct sx_dns {
   int nz = 1, ncom = 2, S = 0, advection = SX_SKEW, nord = 2, toggle = 1, step = 0;
   double beta = 0, nu = 0, dt = 0, cf[3] = {0, 0, 0};
   size_t n = 0;                        // nz * S
-  sx_msys *Mv[2] = {nullptr, nullptr}, *Mp = nullptr;
+  sx_msys *Mv[3] = {nullptr, nullptr, nullptr}, *Mp = nullptr;   // Mv[Je-1]: order Je
   bool group_solve = false;            // velocity solves share one band launch
   int startup = SX_FP64;               // start-up velocity solves: direct or SX_JACPCG
   double tol_rel = 1e-8;
   int step_max = 500;
-  sx_msys* Mpcg = nullptr;
+  sx_msys* Mpcg[2] = {nullptr, nullptr};   // start-up JACPCG systems, order 1, 2
   std::vector<double*> bufs;           // every device allocation
   double *u[3] = {}, *p = nullptr;
-  double *Us[2][3] = {}, *Uf[2][3] = {};
+  double *Us[3][3] = {}, *Uf[3][3] = {};   // [level][component], nord levels
   double *phys[3] = {}, *tmp = nullptr, *tmp2 = nullptr, *H[3] = {}, *F[3] = {};
   double* imass = nullptr;
   int nb = 0, nline = 0, hop_head = 0, un_head = 0;
   int *seg_elmt = nullptr, *seg_idx = nullptr, *seg_kind = nullptr;
   double *seg_nx = nullptr, *seg_ny = nullptr;
-  double *hop[2] = {}, *un[2] = {}, *plines = nullptr;
+  double *hop[3] = {}, *un[2] = {}, *plines = nullptr;
+  double* ulines[3] = {};              // velocity boundary values, Fourier [nz][nline]
 
   double* alloc (size_t cnt)
   {
@@ -642,15 +648,34 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
   return SX_OK;
 }
 
-const double SSc[4][4] = {{0, 0, 0, 0}, {1.0, -1.0, 0, 0}, {1.5, -2.0, 0.5, 0}, {0, 0, 0, 0}};
-const double EXc[4][3] = {{0, 0, 0}, {1.0, 0, 0}, {2.0, -1.0, 0}, {0, 0, 0}};
+// Integration::StifflyStable / Extrapolation (src/integration.cpp:99-164)
+const double SSc[4][4] = {{0, 0, 0, 0}, {1.0, -1.0, 0, 0}, {1.5, -2.0, 0.5, 0},
+                          {11.0 / 6.0, -3.0, 1.5, -1.0 / 3.0}};
+const double EXc[4][3] = {{0, 0, 0}, {1.0, 0, 0}, {2.0, -1.0, 0}, {3.0, -3.0, 1.0}};
+
+// rollm (dns/integrate.cpp): level q moves to q+1, the oldest to level 0
+void roll (double* (&X)[3][3], int nord, int nc)
+{
+  for (int i = 0; i < nc; i++) {
+    double* last = X[nord - 1][i];
+    for (int q = nord - 1; q > 0; q--) X[q][i] = X[q - 1][i];
+    X[0][i] = last;
+  }
+}
+
+int startup_system (sx_dns* d, int Je, sx_msys** M, hipStream_t)
+{
+  *M = (Je >= 1 && Je <= 2) ? d->Mpcg[Je - 1] : nullptr;
+  if (!*M) { set_error ("sx_dns_step: start-up JACPCG system missing"); return SX_ERR_ARG; }
+  return SX_OK;
+}
 
 int dns_step (sx_dns* d, hipStream_t st)
 {
   const int nc = d->ncom, nord = d->nord;
   int rc;
-  // Us[0] <- u: the level about to be overwritten holds nothing needed, so
-  // the buffers are exchanged; u is rewritten by the viscous solves (its
+  // Us[0] <- u: Us[0] holds the oldest level, which nothing needs any more,
+  // so the buffers are exchanged; u is rewritten by the viscous solves (its
   // Nyquist plane, which they skip, is copied back from U below)
   for (int i = 0; i < nc; i++) std::swap (d->u[i], d->Us[0][i]);
   double** U = d->Us[0];
@@ -663,39 +688,45 @@ int dns_step (sx_dns* d, hipStream_t st)
     const int Je = std::min (d->step - 1, nord);
     d->hop_head = (d->hop_head + nord - 1) % nord;     // roll: new level 0
     double* hnew = d->hop[d->hop_head];
-    double* un1 = d->un[d->un_head];                   // previous level 0
-    double* un2 = d->un[(d->un_head + 1) % nord];
-    const int unnew_head = (d->un_head + nord - 1) % nord;
-    double* unnew = d->un[unnew_head];
-    // k_hopbc writes every entry of the HOPBC segments of hnew and unnew
-    // outright (the other lines are never written and stay zero from
-    // sx_dns_set_state); unnew is the level being dropped (un2 when
-    // nord == 2), which each thread reads before it writes the same entry
+    double* un1 = d->un[d->un_head];                   // the previous step's u.n
+    double* unnew = d->un[1 - d->un_head];
+    // k_hopbc writes every entry of the segments of hnew and unnew outright
+    // (the plane padding is never written and stays zero)
     const int nmodes = std::max (1, d->nz / 2);
+    Coef3 ac = {{SSc[Je][1], SSc[Je][2], SSc[Je][3]}};
     hipLaunchKernelGGL (k_hopbc, dim3 (d->nb, nmodes), dim3 (64), 0, st, d->mu->np,
                         d->seg_elmt, d->seg_idx, d->seg_nx, d->seg_ny, d->mu->d_geo,
                         d->mu->d_D, U[0], U[1], nc == 3 ? U[2] : U[0], N[0], N[1], hnew,
-                        unnew, un1, un2, Je, SSc[Je][0], SSc[Je][1], SSc[Je][2],
-                        1.0 / d->dt, d->nu, d->beta, d->nz, d->nline, d->S);
+                        unnew, un1, Je, SSc[Je][0], ac, 1.0 / d->dt, d->nu, d->beta, d->nz,
+                        d->nline, d->S);
     SX_HIP (hipGetLastError ());
-    d->un_head = unnew_head;
+    d->un_head = 1 - d->un_head;
     const int Jb = std::min (d->step, nord);
+    Levels hl = {};
+    for (int q = 0; q < Jb; q++) {
+      hl.f[q] = d->hop[(d->hop_head + q) % nord];
+      hl.c[q] = EXc[Jb][q];
+    }
     hipLaunchKernelGGL (k_plines, dim3 (nb256 ((size_t) d->nz * d->nline)), dim3 (256), 0,
-                        st, d->plines, d->hop[d->hop_head], d->hop[(d->hop_head + 1) % nord],
-                        EXc[Jb][0], EXc[Jb][1], Jb, d->nz, d->nline, d->mu->np, d->seg_kind);
+                        st, d->plines, hl, Jb, d->nz, d->nline, d->mu->np, d->seg_kind);
     SX_HIP (hipGetLastError ());
     plines = d->plines;
   }
   // -- waveProp
   const int Je = std::min (d->step, nord);
-  for (int i = 0; i < nc; i++)
+  for (int i = 0; i < nc; i++) {
     // D->u after swapData holds u^n = Us[0]: only the Nyquist plane, which
     // the solves leave alone, has to be carried over (by the same kernel)
-    EW (k_ew_wave, d->H[i], d->Us[0][i], d->Us[1][i], d->Uf[0][i], d->Uf[1][i],
-        SSc[Je][1], SSc[Je][2], EXc[Je][0] * d->dt, EXc[Je][1] * d->dt, Je, d->n,
-        d->nz > 1 ? d->u[i] : nullptr, (size_t) d->S);
-  // rollm (Uf)
-  for (int i = 0; i < nc; i++) std::swap (d->Uf[0][i], d->Uf[1][i]);
+    Levels ls = {}, lf = {};
+    for (int q = 0; q < Je; q++) {
+      ls.f[q] = d->Us[q][i];
+      ls.c[q] = -SSc[Je][q + 1];
+      lf.f[q] = d->Uf[q][i];
+      lf.c[q] = EXc[Je][q] * d->dt;
+    }
+    EW (k_ew_wave, d->H[i], ls, lf, Je, d->n, d->nz > 1 ? d->u[i] : nullptr, (size_t) d->S);
+  }
+  roll (d->Uf, nord, nc);
   // -- setPForce: div u^ / dt
   const sx_mesh* m = d->mu;
   hipLaunchKernelGGL (k_div_h, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo, m->d_D,
@@ -710,25 +741,30 @@ int dns_step (sx_dns* d, hipStream_t st)
                       m->d_D, d->p, d->H[0], d->H[1], nc == 3 ? d->H[2] : nullptr, d->F[0],
                       d->F[1], nc == 3 ? d->F[2] : nullptr, d->S, d->beta, alpha, betap);
   SX_HIP (hipGetLastError ());
-  // -- Us: store u^n and roll
-  for (int i = 0; i < nc; i++) std::swap (d->Us[0][i], d->Us[1][i]);
-  // -- viscous substep
+  // -- Us: u^n is already in Us[0]; roll
+  roll (d->Us, nord, nc);
+  // -- viscous substep, with the velocity boundary values (bTransform'd
+  //    once by sx_dns_set_velocity_bc; steady)
+  const double* lines[3] = {d->ulines[0], d->ulines[1], d->ulines[2]};
   if (d->startup == SX_JACPCG && d->step < nord) {
-    // Solve, dns/integrate.cpp:336-350: a JACPCG system for the start-up
-    // steps, iterating from the current velocity u^n (now in Us[1])
+    // Solve, dns/integrate.cpp:336-350: a JACPCG system of this step's
+    // order for the start-up steps, iterating from the current velocity
+    // u^n (now in Us[1])
+    sx_msys* Mpcg = nullptr;
+    if ((rc = startup_system (d, Je, &Mpcg, st))) return rc;
     for (int i = 0; i < nc; i++) {
       SX_HIP (hipMemcpyAsync (d->u[i], d->Us[1][i], sizeof (double) * d->n,
                               hipMemcpyDeviceToDevice, st));
-      if ((rc = sx_field_solve_pcg (d->Mpcg, d->nz, d->S, d->F[i], d->u[i], nullptr,
+      if ((rc = sx_field_solve_pcg (Mpcg, d->nz, d->S, d->F[i], d->u[i], lines[i],
                                     d->tol_rel, d->step_max, nullptr, st)))
         return rc;
     }
     return SX_OK;
   }
   sx_msys* Mv = d->Mv[Je - 1];
-  if (d->group_solve) return field_solve_group (Mv, nc, d->nz, d->S, d->F, d->u, nullptr, st);
+  if (d->group_solve) return field_solve_group (Mv, nc, d->nz, d->S, d->F, d->u, lines, st);
   for (int i = 0; i < nc; i++)
-    if ((rc = sx_field_solve (Mv, d->nz, d->S, d->F[i], d->u[i], nullptr, st))) return rc;
+    if ((rc = sx_field_solve (Mv, d->nz, d->S, d->F[i], d->u[i], lines[i], st))) return rc;
   return SX_OK;
 }
 
@@ -740,7 +776,8 @@ extern "C" int sx_dns_destroy (sx_dns_t d)
   for (double* q : d->bufs) (void) hipFree (q);
   void* ip[] = {d->seg_elmt, d->seg_idx, d->seg_kind};
   for (void* q : ip) if (q) (void) hipFree (q);
-  for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mp, d->Mpcg}) if (M) sx_msys_destroy (M);
+  for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mv[2], d->Mp, d->Mpcg[0], d->Mpcg[1]})
+    if (M) sx_msys_destroy (M);
   delete d;
   return SX_OK;
 }
@@ -752,9 +789,10 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
                               int nord, void* stream, sx_dns_t* out)
 {
   if (!mu || !mp || !out || nz < 1 || (nz > 1 && (nz & 1)) || kinvis <= 0 || dt <= 0 ||
-      nord != 2 || (advection != SX_SKEW && advection != SX_ALTSKEW) || mu->nel != mp->nel ||
-      mu->np != mp->np || mu->nbound != mp->nbound || mu->cyl || mp->cyl || mu->np > 16) {
-    set_error ("sx_dns_create: bad arguments (Cartesian, nz 1 or even, N_TIME = 2, "
+      nord < 1 || nord > 3 || (advection != SX_SKEW && advection != SX_ALTSKEW) ||
+      mu->nel != mp->nel || mu->np != mp->np || mu->nbound != mp->nbound || mu->cyl ||
+      mp->cyl || mu->np > 16) {
+    set_error ("sx_dns_create: bad arguments (Cartesian, nz 1 or even, N_TIME 1..3, "
                "same mesh and boundary segments for velocity and pressure)");
     return SX_ERR_ARG;
   }
@@ -775,12 +813,13 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
   int rc;
   auto fail = [&] (int r) { sx_dns_destroy (d); return r; };
   const int nmodes = std::max (1, nz / 2);
-  // Helmholtz systems: velocity for the first-order start and the
-  // second-order steps, pressure (preSolve, dns/integrate.cpp:292-325)
-  if ((rc = sx_msys_create (mu, 1.0 / (dt * kinvis), beta, 0, nmodes, SX_FP64, stream, &d->Mv[0])))
-    return fail (rc);
-  if ((rc = sx_msys_create (mu, 1.5 / (dt * kinvis), beta, 0, nmodes, SX_FP64, stream, &d->Mv[1])))
-    return fail (rc);
+  // Helmholtz systems: velocity for every order 1..nord (the start-up
+  // steps and the full-order ones), pressure (preSolve,
+  // dns/integrate.cpp:292-325)
+  for (int q = 0; q < nord; q++)
+    if ((rc = sx_msys_create (mu, SSc[q + 1][0] / (dt * kinvis), beta, 0, nmodes, SX_FP64,
+                              stream, &d->Mv[q])))
+      return fail (rc);
   if ((rc = sx_msys_create (mp, 0.0, beta, 0, nmodes, SX_FP64, stream, &d->Mp)))
     return fail (rc);
   // fields
@@ -789,7 +828,7 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
   // all of them (falls back to one solve per component when the band is
   // too wide for the panel sweeps)
   d->group_solve = true;
-  for (int q = 0; q < 2; q++) {
+  for (int q = 0; q < nord; q++) {
     rc = msys_reserve_rhs (d->Mv[q], 2 * nc);
     if (rc == SX_ERR_LIMIT) d->group_solve = false;
     else if (rc) return fail (rc);
@@ -798,7 +837,7 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
     if (!(d->u[i] = d->alloc (d->n)) || !(d->phys[i] = d->alloc (d->n)) ||
         !(d->H[i] = d->alloc (d->n)) || !(d->F[i] = d->alloc (d->n)))
       return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
-    for (int q = 0; q < 2; q++)
+    for (int q = 0; q < nord; q++)
       if (!(d->Us[q][i] = d->alloc (d->n)) || !(d->Uf[q][i] = d->alloc (d->n)))
         return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
   }
@@ -855,8 +894,11 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
       return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
     if (!(d->seg_nx = d->alloc (nxv.size ())) || !(d->seg_ny = d->alloc (nyv.size ())))
       return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
+    for (int q = 0; q < nord; q++)
+      if (!(d->hop[q] = d->alloc ((size_t) nz * d->nline)))
+        return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
     for (int q = 0; q < 2; q++)
-      if (!(d->hop[q] = d->alloc ((size_t) nz * d->nline)) || !(d->un[q] = d->alloc ((size_t) nz * d->nline)))
+      if (!(d->un[q] = d->alloc ((size_t) nz * d->nline)))
         return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
     if (!(d->plines = d->alloc ((size_t) nz * d->nline)))
       return fail (hip_fail (hipErrorOutOfMemory, "sx_dns_create"));
@@ -866,11 +908,9 @@ extern "C" int sx_dns_create (sx_mesh_t mu, sx_mesh_t mp, int nz, double beta,
         hipMemcpy (d->seg_nx, nxv.data (), sizeof (double) * nxv.size (), hipMemcpyHostToDevice) ||
         hipMemcpy (d->seg_ny, nyv.data (), sizeof (double) * nyv.size (), hipMemcpyHostToDevice))
       return fail (hip_fail (hipErrorUnknown, "sx_dns_create copy"));
-    for (int q = 0; q < 2; q++) {
-      if (hipMemsetAsync (d->hop[q], 0, sizeof (double) * nz * d->nline, st) ||
-          hipMemsetAsync (d->un[q], 0, sizeof (double) * nz * d->nline, st))
+    for (double* q : {d->hop[0], d->hop[1], d->hop[2], d->un[0], d->un[1]})
+      if (q && hipMemsetAsync (q, 0, sizeof (double) * nz * d->nline, st))
         return fail (hip_fail (hipErrorUnknown, "sx_dns_create memset"));
-    }
   }
   if (hipStreamSynchronize (st) != hipSuccess) return fail (hip_fail (hipErrorUnknown, "sync"));
   *out = d;
@@ -887,17 +927,42 @@ extern "C" int sx_dns_set_state (sx_dns_t d, const double* u_phys, void* stream)
                             hipMemcpyDeviceToDevice, st));
     if ((rc = transform (d, d->u[i], +1, st))) return rc;
     if (d->nz > 1) SX_HIP (hipMemsetAsync (d->u[i] + d->S, 0, sizeof (double) * d->S, st));
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < d->nord; q++) {
       SX_HIP (hipMemsetAsync (d->Us[q][i], 0, sizeof (double) * d->n, st));
       SX_HIP (hipMemsetAsync (d->Uf[q][i], 0, sizeof (double) * d->n, st));
     }
   }
   SX_HIP (hipMemsetAsync (d->p, 0, sizeof (double) * d->n, st));
-  for (int q = 0; q < 2 && d->nb; q++) {
-    SX_HIP (hipMemsetAsync (d->hop[q], 0, sizeof (double) * d->nz * d->nline, st));
-    SX_HIP (hipMemsetAsync (d->un[q], 0, sizeof (double) * d->nz * d->nline, st));
-  }
+  for (double* q : {d->hop[0], d->hop[1], d->hop[2], d->un[0], d->un[1]})
+    if (q) SX_HIP (hipMemsetAsync (q, 0, sizeof (double) * d->nz * d->nline, st));
   d->step = 0; d->toggle = 1; d->hop_head = 0; d->un_head = 0;
+  return SX_OK;
+}
+
+extern "C" int sx_dns_set_velocity_bc (sx_dns_t d, const double* d_lines_phys, void* stream)
+{
+  if (!d) { set_error ("sx_dns_set_velocity_bc: bad arguments"); return SX_ERR_ARG; }
+  hipStream_t st = (hipStream_t) stream;
+  const size_t len = (size_t) d->nz * d->nline;
+  if (!d_lines_phys || !len) {
+    for (int i = 0; i < d->ncom; i++)
+      if (d->ulines[i]) SX_HIP (hipMemsetAsync (d->ulines[i], 0, sizeof (double) * len, st));
+    return SX_OK;
+  }
+  for (int i = 0; i < d->ncom; i++) {
+    if (!d->ulines[i] && !(d->ulines[i] = d->alloc (len)))
+      return hip_fail (hipErrorOutOfMemory, "sx_dns_set_velocity_bc");
+    const double* src = d_lines_phys + (size_t) i * len;
+    if (d->nz == 2) {
+      // Field::bTransform for nz == 2: the first plane keeps its values,
+      // the second is zeroed
+      SX_HIP (hipMemcpyAsync (d->ulines[i], src, sizeof (double) * d->nline,
+                              hipMemcpyDeviceToDevice, st));
+      SX_HIP (hipMemsetAsync (d->ulines[i] + d->nline, 0, sizeof (double) * d->nline, st));
+    } else if (int rc = fourier_z (d->nz, d->nline, src, d->ulines[i], +1, st)) {
+      return rc;
+    }
+  }
   return SX_OK;
 }
 
@@ -908,12 +973,14 @@ extern "C" int sx_dns_set_startup (sx_dns_t d, int method, double tol_rel, int s
     set_error ("sx_dns_set_startup: bad arguments");
     return SX_ERR_ARG;
   }
-  if (method == SX_JACPCG && !d->Mpcg) {
-    // the first-order constant of the start-up step, alpha_0(1) / (dt nu)
-    const int rc = sx_msys_create (d->mu, SSc[1][0] / (d->dt * d->nu), d->beta, 0,
-                                   std::max (1, d->nz / 2), SX_JACPCG, stream, &d->Mpcg);
-    if (rc) return rc;
-  }
+  // the start-up steps 1..nord-1 run at orders Je = step, constant
+  // alpha_0(Je) / (dt nu) (dns/integrate.cpp:336-350)
+  for (int q = 0; method == SX_JACPCG && q + 1 < d->nord; q++)
+    if (!d->Mpcg[q]) {
+      const int rc = sx_msys_create (d->mu, SSc[q + 1][0] / (d->dt * d->nu), d->beta, 0,
+                                     std::max (1, d->nz / 2), SX_JACPCG, stream, &d->Mpcg[q]);
+      if (rc) return rc;
+    }
   d->startup = method;
   d->tol_rel = tol_rel;
   d->step_max = step_max;

@@ -48,6 +48,9 @@ int    band_block (int nband);
 size_t band_sweep_bytes_per_mode (int nmax, int nband);
 int    band_factor (sx_msys* s, hipStream_t st, int* d_status);
 int    band_solve (const sx_msys* s, int nr, hipStream_t st);   // nr RHS per mode
+// SX_OK, or SX_ERR_SYNC when a split-sweep wait gave up since the last call
+// (synchronises st; clears the status word)
+int    split_status (sx_msys* s, hipStream_t st);
 // Grow the per-mode right-hand-side workspace to nr (2, 4 or 6) columns.
 int    msys_reserve_rhs (sx_msys* s, int nr);
 // Field::solve for nf fields sharing one system (the DNS velocity
@@ -144,7 +147,7 @@ struct sx_msys {
   int sw_D = 0;
   int sw_unit = 0;             // 1: off-diagonal panels scaled to W = T_J U (unit diagonal blocks)
   char* d_xch = nullptr;       // split sweeps: per-chain message rings
-  unsigned* d_xep = nullptr;   //   and epochs
+  unsigned* d_xep = nullptr;   //   [4] epoch, arrivals, status (bit 0: a wait gave up)
   size_t xch_bytes = 0;
   // the panel sweep kernel of the last band solve (diagnostic): 0 none,
   // 1 k_panel_sweeps, 2 k_unit_sweeps, 3 k_split_sweeps; right-hand sides

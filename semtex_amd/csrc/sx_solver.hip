@@ -1438,11 +1438,18 @@ extern "C" int sx_field_solve_host (sx_msys_t s, int nz, int stride, const doubl
     SX_HIP (hipMemcpy (dl, lines, lb, hipMemcpyHostToDevice));
   }
   rc = sx_field_solve (s, nz, stride, df, du, dl, nullptr);
+  if (rc == SX_OK) rc = split_status (s, nullptr);
   if (rc == SX_OK) {
     SX_HIP (hipMemcpy (u, du, nb, hipMemcpyDeviceToHost));
   }
   (void) hipFree (df); (void) hipFree (du); if (dl) (void) hipFree (dl);
   return rc;
+}
+
+extern "C" int sx_msys_status (sx_msys_t s, void* stream)
+{
+  if (!s) { set_error ("sx_msys_status: null system"); return SX_ERR_ARG; }
+  return split_status (s, (hipStream_t) stream);
 }
 
 extern "C" int sx_helmholtz_op (sx_msys_t s, int nz, int stride, const double* d_u,

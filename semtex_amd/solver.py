@@ -92,6 +92,13 @@ class ModalMatrixSys:
         keys = ("panels", "D", "twisted", "nband", "unit", "kernel", "nr")
         return dict(zip(keys, info))
 
+    def status(self):
+        """Synchronises the current stream; raises SxError (code SX_ERR_SYNC)
+        when a split band sweep's wait gave up in a solve since the last
+        call (its solution holds NaN), as the reference's message(ERROR)
+        would end the run."""
+        _lib.check(_lib.load().sx_msys_status(self.handle, _stream()), "sx_msys_status")
+
     def __len__(self):
         return self.num_modes
 
@@ -203,6 +210,22 @@ class DNS:
 
     def step(self, n=1):
         _lib.check(_lib.load().sx_dns_step(self.handle, int(n), _stream()), "sx_dns_step")
+        return self
+
+    def set_velocity_bc(self, lines_phys):
+        """Steady velocity boundary values, [ncom][nz][nbound*np] CUDA float64
+        in physical space (Field::evaluateBoundaries + bTransform); None
+        resets them to zero."""
+        import torch
+        if lines_phys is None:
+            ptr = None
+        else:
+            assert lines_phys.dtype == torch.float64 and lines_phys.is_cuda
+            assert lines_phys.is_contiguous()
+            self._ulines = lines_phys
+            ptr = C.c_void_p(lines_phys.data_ptr())
+        _lib.check(_lib.load().sx_dns_set_velocity_bc(self.handle, ptr, _stream()),
+                   "sx_dns_set_velocity_bc")
         return self
 
     def set_startup(self, method, tol_rel=1e-8, step_max=500):

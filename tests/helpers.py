@@ -79,3 +79,33 @@ def boundary_lines(mesh, func):
         xs, ys = mesh.side_xy(b)
         line[b * mesh.np:(b + 1) * mesh.np] = func(xs, ys)
     return line
+
+
+def kovas5_case():
+    """mesh/kovas5: Kovasznay flow in the y-z plane (regress/kovas5.ok).
+    Returns the velocity and pressure meshes, the DNS parameters, the
+    exact-solution function ex(X, Y, Z) -> [u, v, w], the z planes and the
+    physical-space velocity boundary lines [3][nz][nbound*np]."""
+    from semtex_amd.mesh import rect_mesh
+    Re = 40.0
+    lam = Re / 2 - np.sqrt(0.25 * Re * Re + 4 * np.pi ** 2)
+    nz, Lz, dt, np_ = 4, 1.0, 0.005, 8
+    xs, ys = [-0.5, 0.0, 0.5], [-0.5, 0.0, 1.0]
+    mu = rect_mesh(2, 2, np_, bc=("D", "D", "D", "D"), xs=xs, ys=ys)
+    mp = rect_mesh(2, 2, np_, bc=("N", "N", "N", "N"), xs=xs, ys=ys)
+
+    def ex(X, Y, Z):
+        tp = 2 * np.pi
+        return [0 * X, 1 - np.exp(lam * Y) * np.cos(tp * Z),
+                lam / tp * np.exp(lam * Y) * np.sin(tp * Z)]
+
+    z = np.arange(nz) * Lz / nz
+    lines = np.zeros((3, nz, mu.nbound * np_))
+    for b in range(mu.nbound):
+        xb, yb = mu.side_xy(b)
+        for k in range(nz):
+            e = ex(xb, yb, np.full_like(xb, z[k]))
+            for i in range(3):
+                lines[i, k, b * np_:(b + 1) * np_] = e[i]
+    return dict(mesh_u=mu, mesh_p=mp, nz=nz, beta=2 * np.pi / Lz, kinvis=1.0 / Re, dt=dt,
+                steps=200, ex=ex, z=z, lines=lines)

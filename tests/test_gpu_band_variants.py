@@ -140,21 +140,69 @@ def test_dns_split_three_rhs_per_workgroup(torch_cuda, monkeypatch):
         assert rel < 1e-11, (i, rel)
 
 
-def test_split_wait_failure_gives_nan_then_recovers(torch_cuda, problems, monkeypatch):
+def test_split_wait_failure_reports_status_then_recovers(torch_cuda, problems, monkeypatch):
     """The split sweeps' bounded waits, forced from the host to give up at
-    once (SX_SPLIT_SPIN=-1): the kernels still exit, the solution is NaN
-    (the failure shows in the result), and the next solve with the normal
-    bound is correct again (epochs and rings stay consistent)."""
+    once (SX_SPLIT_SPIN=-1): the kernels still exit, the solution is NaN,
+    sx_msys_status reports SX_ERR_SYNC once (then clears), and the next
+    solves on the same system with the normal bound are exact: every
+    workgroup of the failed launch used one epoch and the epoch advanced
+    only after all of them finished, so no record of the failed launch
+    matches a later one."""
+    from semtex_amd import _lib
+    from semtex_amd.solver import Field, GpuMesh, ModalMatrixSys
     m, nz, F, lines, uo, _ = problems["D5"]
-    print("case: D5 split, forced wait failure then a normal solve", flush=True)
+    print("case: D5 split, forced wait failure, status, then normal solves", flush=True)
     monkeypatch.setenv("SX_BAND_SPLIT", "1")
-    monkeypatch.setenv("SX_SPLIT_SPIN", "-1")
-    (bad,), lay = _solve(torch_cuda, m, nz, F, lines)
-    assert lay["kernel"] == 3, lay
+    gm = GpuMesh(m)
+    M = ModalMatrixSys(2.5, 1.0, 0, nz // 2, gm)
+    u = Field(gm, nz)
+    u.lines.copy_(torch_cuda.from_numpy(lines))
     n = m.nel * m.npnp
-    assert np.isnan(bad[0, :n]).any()
+    M.status()                                        # clean before anything ran
+    monkeypatch.setenv("SX_SPLIT_SPIN", "-1")
+    u.solve(torch_cuda.from_numpy(F).cuda(), M)
+    torch_cuda.cuda.synchronize()
+    assert M.layout()["kernel"] == 3, M.layout()
+    assert np.isnan(u.data.cpu().numpy()[0, :n]).any()
+    with pytest.raises(_lib.SxError) as ei:
+        M.status()
+    assert ei.value.code == _lib.SX_ERR_SYNC
+    M.status()                                        # reported once, then cleared
     monkeypatch.delenv("SX_SPLIT_SPIN")
-    (good,), _ = _solve(torch_cuda, m, nz, F, lines)
+    for rep in range(3):
+        u.solve(torch_cuda.from_numpy(F).cuda(), M)
+        torch_cuda.cuda.synchronize()
+        M.status()
+        good = u.data.cpu().numpy()
+        for k in (0, 2, 3):
+            rel = np.linalg.norm(good[k, :n] - uo[k]) / np.linalg.norm(uo[k])
+            assert rel < 1e-12, (rep, k, rel)
+
+
+def test_split_status_through_host_entry_point(torch_cuda, problems, monkeypatch):
+    """sx_field_solve_host checks the status itself: a forced wait failure
+    returns SX_ERR_SYNC instead of SX_OK."""
+    import ctypes as C
+
+    from semtex_amd import _lib
+    from semtex_amd.solver import GpuMesh, ModalMatrixSys
+    m, nz, F, lines, uo, _ = problems["D5"]
+    print("case: D5 split, sx_field_solve_host with a forced wait failure", flush=True)
+    monkeypatch.setenv("SX_BAND_SPLIT", "1")
+    gm = GpuMesh(m)
+    M = ModalMatrixSys(2.5, 1.0, 0, nz // 2, gm)
+    L = _lib.load()
+    f = np.ascontiguousarray(F)
+    u = np.zeros_like(f)
+    ln = np.ascontiguousarray(lines)
+    p = lambda a: C.c_void_p(a.ctypes.data)  # noqa: E731
+    monkeypatch.setenv("SX_SPLIT_SPIN", "-1")
+    rc = L.sx_field_solve_host(M.handle, nz, f.shape[1], p(f), p(u), p(ln))
+    assert rc == _lib.SX_ERR_SYNC, rc
+    monkeypatch.delenv("SX_SPLIT_SPIN")
+    rc = L.sx_field_solve_host(M.handle, nz, f.shape[1], p(f), p(u), p(ln))
+    assert rc == _lib.SX_OK, L.sx_last_error()
+    n = m.nel * m.npnp
     for k in (0, 2, 3):
-        rel = np.linalg.norm(good[k, :n] - uo[k]) / np.linalg.norm(uo[k])
+        rel = np.linalg.norm(u[k, :n] - uo[k]) / np.linalg.norm(uo[k])
         assert rel < 1e-12, (k, rel)

@@ -4,10 +4,14 @@ and, step by step, against the DNS oracle.
 * regress/taylor2.ok and taylor4.ok: 20 steps of 2D / 3D Taylor flow with
   the default alternating skew-symmetric advection on the GPU reproduce
   norm_inf(exact - u) to the 4 printed digits.
+* regress/kovas5.ok: z-varying Dirichlet velocity on every side and
+  computed pressure conditions on every side, 200 steps, JACPCG start-up:
+  the same 4 printed digits for u, v, w.
 * A walled, forced 3D channel (no-slip walls, computed pressure BCs,
-  CONST_X, full skew-symmetric advection, random perturbation): every
-  Fourier plane of u, v, w (and p, Nyquist excluded) after each of 8 steps
-  agrees with the oracle to 1e-11 relative L2.
+  CONST_X, full skew-symmetric advection, random perturbation), N_TIME 1,
+  2 and 3: every Fourier plane of u, v, w (and p, Nyquist excluded) after
+  each of 8 steps agrees with the oracle to 1e-11 relative L2; the same
+  with z-varying Dirichlet data on the walls.
 """
 import json
 import os
@@ -78,22 +82,23 @@ def test_gpu_dns_reproduces_regress_goldens(torch_cuda, oracle, case, startup):
             assert f"{err:.4e}" == f"{want:.4e}", (comp, err, want)
 
 
-@pytest.mark.parametrize("startup,nz", [("direct", 8), ("pcg", 8), ("direct", 32)])
-def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup, nz):
+@pytest.mark.parametrize("startup,nz,nord", [("direct", 8, 2), ("pcg", 8, 2), ("direct", 32, 2),
+                                             ("direct", 8, 1), ("direct", 8, 3), ("pcg", 8, 3)])
+def test_gpu_dns_channel_step_parity(torch_cuda, oracle, startup, nz, nord):
     import torch
     from oracle.dns_oracle import DnsOracle
     from semtex_amd.mesh import rect_mesh
     from semtex_amd.solver import DNS, SX_JACPCG, SX_SKEW, GpuMesh
-    print(f"case: channel 3x4 np=6 nz={nz} skew, 8 steps", startup, flush=True)
+    print(f"case: channel 3x4 np=6 nz={nz} skew N_TIME={nord}, 8 steps", startup, flush=True)
     bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
     mu = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
     mp = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
     beta, nu, dt = 1.0, 1.0 / 50, 5e-3
     cf = (2 * nu, 0.0, 0.0)
     gu, gp = GpuMesh(mu), GpuMesh(mp)
-    D = DNS(gu, gp, nz, beta, nu, dt, const_force=cf, advection=SX_SKEW)
+    D = DNS(gu, gp, nz, beta, nu, dt, const_force=cf, advection=SX_SKEW, nord=nord)
     O = DnsOracle(mu, mp, nz, beta, nu, dt, const_force=cf, advection="skew", startup=startup,
-                  tol_rel=1e-12)
+                  tol_rel=1e-12, nord=nord)
     if startup == "pcg":
         D.set_startup(SX_JACPCG, tol_rel=1e-12, step_max=500)
     n, S = mu.nel * mu.npnp, D.stride
@@ -167,3 +172,77 @@ def test_dns_transform_tiles_agree(torch_cuda, monkeypatch):
     for i in range(4):
         rel = np.linalg.norm(a[i][planes] - b[i][planes]) / np.linalg.norm(b[i][planes])
         assert rel < 1e-13, (i, rel)
+
+
+def test_gpu_dns_reproduces_kovas5_golden(torch_cuda, oracle):
+    import torch
+    from helpers import kovas5_case
+    from semtex_amd.solver import DNS, SX_ALTSKEW, SX_JACPCG, GpuMesh
+    print("case: kovas5, 200 steps, JACPCG start-up", flush=True)
+    K = kovas5_case()
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "regress_kovas.json")))["kovas5"]
+    mu, nz = K["mesh_u"], K["nz"]
+    D = DNS(GpuMesh(mu), GpuMesh(K["mesh_p"]), nz, K["beta"], K["kinvis"], K["dt"],
+            advection=SX_ALTSKEW)
+    D.set_startup(SX_JACPCG, tol_rel=1e-8, step_max=500)
+    D.set_velocity_bc(torch.from_numpy(K["lines"]).cuda())
+    n, S = mu.nel * mu.npnp, D.stride
+    X = np.broadcast_to(mu.x.ravel(), (nz, n))
+    Y = np.broadcast_to(mu.y.ravel(), (nz, n))
+    Z = np.broadcast_to(K["z"][:, None], (nz, n))
+    E = K["ex"](X, Y, Z)
+    u0 = np.zeros((3, nz, S))
+    for i in range(3):
+        u0[i, :, :n] = E[i]
+    D.set_state(torch.from_numpy(u0).cuda())
+    D.step(K["steps"])
+    torch.cuda.synchronize()
+    for i, c in enumerate("uvw"):
+        got = phys_of(D.field(i).cpu().numpy()[:, :n])
+        err = float(np.max(np.abs(got - E[i])))
+        assert f"{err:.4e}" == f"{gold[c]:.4e}", (c, err, gold[c])
+
+
+def test_gpu_dns_channel_with_wall_data_step_parity(torch_cuda, oracle):
+    """Walls with z- and x-varying Dirichlet velocity (a tangential wall
+    motion u = 0.1 sin(x) cos(z), w = 0.05 cos(x); v = 0) against the oracle
+    every step."""
+    import torch
+    from oracle.dns_oracle import DnsOracle
+    from semtex_amd.mesh import rect_mesh
+    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh
+    print("case: channel 3x4 np=6 nz=8 with wall data, 6 steps", flush=True)
+    bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
+    mu = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
+    mp = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
+    nz, beta, nu, dt = 8, 1.0, 1.0 / 50, 5e-3
+    cf = (2 * nu, 0.0, 0.0)
+    D = DNS(GpuMesh(mu), GpuMesh(mp), nz, beta, nu, dt, const_force=cf, advection=SX_SKEW)
+    O = DnsOracle(mu, mp, nz, beta, nu, dt, const_force=cf, advection="skew")
+    z = np.arange(nz) * 2 * np.pi / (beta * nz)
+    lines = np.zeros((3, nz, mu.nbound * mu.np))
+    for b in range(mu.nbound):
+        xb, _ = mu.side_xy(b)
+        for k in range(nz):
+            sl = slice(b * mu.np, (b + 1) * mu.np)
+            lines[0, k, sl] = 0.1 * np.sin(xb) * np.cos(z[k])
+            lines[2, k, sl] = 0.05 * np.cos(xb)
+    O.set_velocity_bc(lines)
+    D.set_velocity_bc(torch.from_numpy(lines).cuda())
+    n, S = mu.nel * mu.npnp, D.stride
+    y = mu.y.ravel()
+    u0 = np.zeros((3, nz, S))
+    u0[0, :, :n] = 1.0 - y ** 2
+    O.set_initial(u0[:, :, :n])
+    D.set_state(torch.from_numpy(u0).cuda())
+    planes = [k for k in range(nz) if k != 1]
+    for s in range(6):
+        O.step()
+        D.step(1)
+        torch.cuda.synchronize()
+        for i in range(4):
+            g = D.field(i).cpu().numpy()[:, :n]
+            o = O.u[i] if i < 3 else O.p
+            rel = np.linalg.norm(g[planes] - o[planes]) / np.linalg.norm(o[planes])
+            assert rel < 1e-11, (s, i, rel)

@@ -45,3 +45,12 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.SxError):
         _lib.load()
+
+
+def test_cpp_glue_compiles_and_links():
+    """The C++ caller of the ABI (tests/abi_glue.cpp, the calls of the
+    drop-in glue) compiles against include/semtex_amd.h alone and links to
+    the library; it is run against the oracle under -m gpu."""
+    from glue_build import build_glue
+    exe = build_glue(force=True)
+    assert os.access(exe, os.X_OK)

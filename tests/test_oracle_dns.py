@@ -139,3 +139,106 @@ def test_hopbc_curl_curl_closed_form(oracle):
     D2.maintain_fourier(np.zeros_like(U), N2)
     for b in range(D2.nb):
         assert np.allclose(D2.hopbc[0, 0, b * D2.np_:(b + 1) * D2.np_], 2.5 * D2.seg_ny[b], atol=1e-14)
+
+
+def test_dns_oracle_reproduces_kovas5_golden(oracle):
+    """regress/kovas5.ok: z-varying Dirichlet velocity on every side (the
+    lines forward transformed as Field::bTransform does), high-order
+    pressure conditions on every side (pure Neumann pressure, singular
+    mode 0), N_TIME = 2, start-up JACPCG at TOL_REL = 1e-8, 200 steps:
+    norm_inf(exact - u) to the 4 printed digits for u, v, w."""
+    from helpers import kovas5_case
+    from oracle.dns_oracle import DnsOracle
+    K = kovas5_case()
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                       "regress_kovas.json")))["kovas5"]
+    mu, nz = K["mesh_u"], K["nz"]
+    D = DnsOracle(mu, K["mesh_p"], nz, K["beta"], K["kinvis"], K["dt"], advection="alt",
+                  startup="pcg", tol_rel=1e-8)
+    D.set_velocity_bc(K["lines"])
+    n = mu.nel * mu.npnp
+    X = np.broadcast_to(mu.x.ravel(), (nz, n))
+    Y = np.broadcast_to(mu.y.ravel(), (nz, n))
+    Z = np.broadcast_to(K["z"][:, None], (nz, n))
+    E = K["ex"](X, Y, Z)
+    D.set_initial(E)
+    for _ in range(K["steps"]):
+        D.step()
+    for i, c in enumerate("uvw"):
+        err = float(np.max(np.abs(D.physical(D.u[i]) - E[i])))
+        assert f"{err:.4e}" == f"{gold[c]:.4e}", (c, err, gold[c])
+
+
+def test_hopbc_time_levels_closed_form(oracle):
+    """The time-level bookkeeping of the computed pressure condition
+    (BCmgr::maintainFourier + evaluateCNBCp, src/bcmgr.cpp:1037-1335)
+    against a closed form.  A spatially uniform velocity v(t) = a + b t +
+    g t^2 (u = w = 0) and N = 0 leave only the -d(u.n)/dt term: at step n
+    (velocity at t_{n-1}) the reference forms
+        h_n = -(alpha_0 un(t_{n-1}) + sum_{q<Je} alpha_{q+1} un(t_{n-2})) / dt,
+    i.e. -1.5 (v(t_{n-1}) - v(t_{n-2})) n_y / dt for N_TIME = 2 (every
+    alpha_{q+1} multiplies the previous level, src/bcmgr.cpp:1127-1128),
+    and dp/dn = 2 h_n - h_{n-1} (Extrapolation, order 2).  A slip in the
+    roll of the levels, the coefficients or the extrapolation changes
+    the values."""
+    from oracle.dns_oracle import DnsOracle
+    mu, mp = channel(nx=2, ny=2, np_=5, nz=4)
+    nz, dt = 4, 0.1
+    a, b, g = 0.3, -1.7, 2.9
+    D = DnsOracle(mu, mp, nz, 1.0, 0.05, dt)
+    vt = lambda t: a + b * t + g * t * t  # noqa: E731
+    n = mu.nel * mu.npnp
+
+    def h_exact(step):                  # step >= 3: Je = 2
+        t1, t0 = (step - 1) * dt, (step - 2) * dt
+        return -1.5 * (vt(t1) - vt(t0)) / dt
+
+    for step in range(1, 7):
+        U = np.zeros((3, nz, n))
+        U[1][0] = vt((step - 1) * dt)     # mode 0 only: uniform in z
+        D.step_no = step
+        D.maintain_fourier(U, np.zeros_like(U))
+        lines = D.pressure_lines()
+        if step >= 4:
+            want = 2 * h_exact(step) - h_exact(step - 1)
+            for bb in range(D.nb):
+                sl = slice(bb * D.np_, (bb + 1) * D.np_)
+                assert np.allclose(D.hopbc[0, 0, sl], h_exact(step) * D.seg_ny[bb],
+                                   rtol=1e-12, atol=1e-12)
+                assert np.allclose(lines[0, sl], want * D.seg_ny[bb], rtol=1e-12, atol=1e-12)
+            assert np.max(np.abs(lines[2:])) < 1e-14
+
+
+def test_walled_channel_converges_at_second_order(oracle):
+    """Self-convergence of the walled step in time (computed pressure
+    conditions on the walls, N_TIME = 2): the differences of the solution
+    at t = 0.08 between dt, dt/2 and dt/4 shrink by ~4, which a slip in the
+    extrapolation of the pressure condition or in the stiffly-stable
+    levels would break."""
+    from oracle.dns_oracle import DnsOracle
+    # resolved in space (N_P = 11): at N_P = 7 the unresolved products'
+    # aliasing, not the time stepping, dominates the differences
+    mu, mp = channel(nx=2, ny=3, np_=11, nz=4)
+    nz, nu, T = 4, 0.05, 0.08
+    n = mu.nel * mu.npnp
+    X = np.broadcast_to(mu.x.ravel(), (nz, n))
+    Y = np.broadcast_to(mu.y.ravel(), (nz, n))
+    z = np.arange(nz) * 2 * np.pi / nz
+    Z = np.broadcast_to(z[:, None], (nz, n))
+    # divergence-free perturbation of Poiseuille flow from the stream
+    # function psi = (1 - y^2)^2 sin x (1 + 0.5 cos z): u = psi_y, v = -psi_x,
+    # both zero on the walls
+    wall = 1.0 - Y ** 2
+    amp = 0.3 * (1.0 + 0.5 * np.cos(Z))
+    u0 = [wall - amp * 4 * Y * wall * np.sin(X), -amp * wall * wall * np.cos(X), 0 * X]
+    sols = []
+    for dt in (0.004, 0.002, 0.001):
+        D = DnsOracle(mu, mp, nz, 1.0, nu, dt, const_force=(2 * nu, 0.0, 0.0))
+        D.set_initial(u0)
+        for _ in range(int(round(T / dt))):
+            D.step()
+        sols.append(np.array([D.physical(D.u[i]) for i in range(3)]))
+    d1 = np.max(np.abs(sols[0] - sols[1]))
+    d2 = np.max(np.abs(sols[1] - sols[2]))
+    ratio = d1 / d2
+    assert 3.3 < ratio < 4.7, (d1, d2, ratio)
