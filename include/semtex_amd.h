@@ -237,6 +237,16 @@ int sx_dns_set_state (sx_dns_t d, const double* d_u_phys, void* stream);
  * (src/field.cpp:174-203) and used by every velocity solve (only the
  * essential segments' values matter).  NULL resets them to zero. */
 int sx_dns_set_velocity_bc (sx_dns_t d, const double* d_lines_phys, void* stream);
+/* 3/2-rule dealiasing of the z products of the skew-symmetric nonlinear
+ * term (on = 1), the treatment AuxField::transform32 provides
+ * (src/auxfield.cpp:1197-1258; the reference's nonlinear terms no longer
+ * call it, dns/nonlinear.cpp:72-76, so it is off by default and the
+ * parity mode is the reference's): the velocity and its z derivative are
+ * zero-padded onto M = 3 nz / 2 physical planes, the products formed there
+ * and the result truncated to modes 0..nz/2-1 (the Nyquist plane of the
+ * nonlinear term is zero).  The z transforms run as FP64 GEMMs with the
+ * packed DFT matrices.  3D fields with nz a multiple of 4, SX_SKEW. */
+int sx_dns_set_dealias (sx_dns_t d, int on, void* stream);
 /* Advance nsteps steps of the integrate() loop body. */
 int sx_dns_step (sx_dns_t d, int nsteps, void* stream);
 /* Velocity solves of the start-up steps (step < nord): SX_FP64 (direct,
@@ -250,6 +260,53 @@ int sx_dns_set_startup (sx_dns_t d, int method, double tol_rel, int step_max, vo
 int sx_dns_get (sx_dns_t d, int which, double* d_out, int* step, void* stream);
 /* Plane stride of the DNS fields (Geometry::planeSize()). */
 int sx_dns_plane_stride (sx_dns_t d);
+
+/* ------------------------------------------------------------------------
+ * Field files (binary nekton format, host memory): the 10-line header of
+ * writeField (src/misc.cpp:106-180) / Header (src/data2df.cpp:437-516)
+ * followed, for each field named in `fields` (one character each, in
+ * order), by nz planes of nel*nr*ns doubles in the machine's byte order
+ * (AuxField operator<<, src/auxfield.cpp:1021-1070).  Dumps may be
+ * appended one after another (the reference's .fld with CHKPOINT = 0).
+ * ---------------------------------------------------------------------- */
+typedef struct {
+  char   session[64];     /* "Session" (the reader keeps 24 characters)    */
+  char   created[64];     /* "Created"; empty on write: the current time   */
+  int    nr, ns, nz, nel; /* "Nr, Ns, Nz, Elements"                        */
+  int    step;            /* "Step"                                        */
+  double time, dt;        /* "Time", "Time step"                           */
+  double kinvis, beta;    /* "Kinvis", "Beta"                              */
+  char   fields[32];      /* "Fields written", e.g. "uvwp"                 */
+  char   format[64];      /* "Format": "binary IEEE little-endian"; set on
+                             write from the machine                        */
+} sx_field_header;
+/* writeField: data[i] ([nz][plane_stride] host doubles) for fields[i];
+ * append = 1 adds a dump at the end of the file. */
+int sx_field_write (const char* path, const sx_field_header* hdr,
+                    const double* const* data, int plane_stride, int append);
+/* Header of dump number `dump` (0 = first) of a field file. */
+int sx_field_read_header (const char* path, int dump, sx_field_header* hdr);
+/* readField for one field: field `name` of dump `dump` into data
+ * [nz][plane_stride] (plane padding zeroed), byte-swapped when the file's
+ * format names the other endianness (Header::swab, src/data2df.cpp:527). */
+int sx_field_read (const char* path, int dump, char name, double* data, int plane_stride);
+
+/* session.num numbering files.  sx_num_write prints the enumerate summary
+ * and table (utility/enumerate.cpp:441-552): `fields` all field names,
+ * nset numbering systems, set_fields[j] the fields sharing system j,
+ * btog[j]/bmask[j] [nel][4(np-1)] per element side in emap order,
+ * nglobal/nsolve/optlev/nband per system.  sx_num_read parses it as
+ * BCmgr::buildnum does (src/bcmgr.cpp:586-780), checking nel and np:
+ * set_fields receives nset strings of field_cap chars, btog/bmask
+ * [nset][nel*4(np-1)], the rest [nset]. */
+int sx_num_write (const char* path, const char* fields, int nset,
+                  const char* const* set_fields, int nel, int np,
+                  const int* const* btog, const int* const* bmask,
+                  const int* nglobal, const int* nsolve, const int* optlev,
+                  const int* nband);
+int sx_num_read (const char* path, int nel, int np, int max_sets, int* nset,
+                 char* set_fields, int field_cap, int* btog, int* bmask,
+                 int* nglobal, int* nsolve, int* optlev, int* nband);
 
 /* Per-kernel timing of sx_field_solve with HIP events recorded on the
  * solve's stream (no counterpart in the reference, whose only timer is the CPU

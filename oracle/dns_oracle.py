@@ -64,6 +64,30 @@ def inv(X):
     return np.fft.irfft(F, n=nz, axis=0) * nz
 
 
+def inv32(X, M):
+    """AuxField::transform32(INVERSE) (src/auxfield.cpp:1197-1258) with the
+    3/2 rule: the N packed Fourier planes, modes 0..N/2-1 (the Nyquist plane
+    dropped), zero-padded and inverse transformed onto M = 3N/2 z points."""
+    N = X.shape[0]
+    F = np.zeros((M // 2 + 1,) + X.shape[1:], dtype=complex)
+    F[0] = X[0]
+    F[1:N // 2] = X[2::2] + 1j * X[3::2]
+    return np.fft.irfft(F, n=M, axis=0) * M
+
+
+def fwd32(x, N):
+    """AuxField::transform32(FORWARD) with the 3/2 rule: the real DFT of M
+    physical planes (scaled 1/M), truncated to modes 0..N/2-1 in the N-plane
+    packing; the Nyquist plane is zero (the truncation drops mode N/2)."""
+    M = x.shape[0]
+    F = np.fft.rfft(x, axis=0) / M
+    out = np.zeros((N,) + x.shape[1:])
+    out[0] = F[0].real
+    out[2::2] = F[1:N // 2].real
+    out[3::2] = F[1:N // 2].imag
+    return out
+
+
 class DnsOracle:
     """State and step of integrate() for a 2D mesh x N_Z Fourier planes.
 
@@ -136,6 +160,8 @@ class DnsOracle:
         # velocity boundary values in Fourier space, [ncom][nz][nline], or
         # None for zero data (the essential segments are then no-slip walls)
         self.ulines = None
+        # 3/2-rule dealiasing of the z products (advection "skew" only)
+        self.dealias = False
 
     # ---------------------------------------------------------------- ops
     def grad_xy(self, f, d):
@@ -198,6 +224,8 @@ class DnsOracle:
         Returns (U, N): the Fourier copy of the velocity and N."""
         nc = self.ncom
         U = self.u.copy()
+        if self.dealias:
+            return U, self.skew_symmetric_32(U)
         phys = np.array([inv(U[i]) for i in range(nc)])
         N = np.zeros_like(U)
         conv = self.advection == "skew" or self.toggle == 1
@@ -227,6 +255,36 @@ class DnsOracle:
         if self.advection == "alt":
             self.toggle = 1 - self.toggle
         return U, N
+
+    def skew_symmetric_32(self, U):
+        """The skew-symmetric nonlinear term with every z product formed on
+        M = 3N/2 physical planes (transform32, src/auxfield.cpp:1197-1258):
+        the product of two fields with modes |k| <= N/2-1 has modes |k| <=
+        N-2, which on M points alias onto |k| >= M-(N-2) = N/2+2, so the
+        retained modes 0..N/2-1 are free of aliasing.  Same term by term as
+        skewSymmetric (dns/nonlinear.cpp:162-193): the physical-space parts
+        are summed on M planes and truncated once; d_z(u_i w) is formed in
+        Fourier space from the truncated product."""
+        nz, nc = self.nz, self.ncom
+        Mz = 3 * nz // 2
+        physM = np.array([inv32(U[i], Mz) for i in range(nc)])
+        N = np.zeros_like(U)
+        for i in range(nc):
+            n_i = np.zeros((Mz, self.n))
+            for j in range(nc):
+                # convective u_j d_j u_i
+                tmp = inv32(self.grad_z(U[i]), Mz) if j == 2 else self.grad_xy(physM[i], j)
+                n_i -= physM[j] * tmp
+                # conservative d_j (u_i u_j), x and y on the M planes
+                if j < 2:
+                    n_i -= self.grad_xy(physM[i] * physM[j], j)
+            Ni = fwd32(n_i, nz)
+            Ni -= self.grad_z(fwd32(physM[i] * physM[2], nz))
+            Ni *= 0.5
+            if abs(self.const[i]) > 6e-14:
+                Ni[0] += self.const[i]
+            N[i] = self.smooth(Ni)
+        return N
 
     def set_initial(self, u_phys):
         """Domain::restart: physical IC, forward transform, Nyquist plane

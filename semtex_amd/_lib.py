@@ -39,6 +39,7 @@ SIGNATURES = {
     "sx_fourier_transform": (_i, [_i, _i, _dp, _dp, _i, _vp]),
     "sx_dns_set_startup": (_i, [_vp, _i, _d, _i, _vp]),
     "sx_dns_set_velocity_bc": (_i, [_vp, _dp, _vp]),
+    "sx_dns_set_dealias": (_i, [_vp, _i, _vp]),
     "sx_field_solve_weak": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
     "sx_batch_create": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
     "sx_batch_run": (_i, [_vp, _vp]),
@@ -52,6 +53,12 @@ SIGNATURES = {
     "sx_dns_step": (_i, [_vp, _i, _vp]),
     "sx_dns_get": (_i, [_vp, _i, _dp, C.POINTER(_i), _vp]),
     "sx_dns_plane_stride": (_i, [_vp]),
+    "sx_field_write": (_i, [C.c_char_p, _vp, _vp, _i, _i]),
+    "sx_field_read_header": (_i, [C.c_char_p, _i, _vp]),
+    "sx_field_read": (_i, [C.c_char_p, _i, C.c_char, _dp, _i]),
+    "sx_num_write": (_i, [C.c_char_p, C.c_char_p, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
+                          _vp]),
+    "sx_num_read": (_i, [C.c_char_p, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sx_timing_enable": (_i, [_i]),
     "sx_timing_read": (_i, [_dp, _ip, _i]),
 }

@@ -24,6 +24,8 @@
 #include <cmath>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "sx_internal.h"
 
 using namespace sx;
@@ -292,6 +294,20 @@ __global__ void k_ew_addc (double* x, double c, size_t n)
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] += c;
 }
+// N -= d_z F in Fourier space (planes 0 and 1 carry no z derivative):
+// Re_k -= -beta k Im_k(F), Im_k -= beta k Re_k(F)
+__global__ void k_sub_dz (double* N, const double* F, int nz, int S, double beta)
+{
+  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t half = (size_t) (nz / 2 - 1) * S;
+  if (i >= half) return;
+  const int k = 1 + (int) (i / S);
+  const size_t c = i % S, re = (size_t) 2 * k * S + c, im = re + S;
+  const double bk = beta * k, fr = F[re], fi = F[im];
+  N[re] += bk * fi;
+  N[im] -= bk * fr;
+}
+
 // Time levels q = 0..Je-1 of one field (Je <= 3) and their coefficients.
 struct Levels {
   const double* f[3];
@@ -520,6 +536,13 @@ struct sx_dns {
   double *seg_nx = nullptr, *seg_ny = nullptr;
   double *hop[3] = {}, *un[2] = {}, *plines = nullptr;
   double* ulines[3] = {};              // velocity boundary values, Fourier [nz][nline]
+  // 3/2-rule dealiasing: M = 3 nz / 2 physical planes, the transforms as
+  // FP64 GEMMs with the packed DFT matrices (column-major)
+  int dealias = 0, M = 0;
+  size_t nM = 0;                       // M * S
+  double *physM[3] = {}, *tmpM = nullptr, *tmpM2 = nullptr;
+  double *Wi = nullptr, *Wdz = nullptr, *Wf = nullptr;   // [N][M], [N][M], [M][N]
+  rocblas_handle blas = nullptr;
 
   double* alloc (size_t cnt)
   {
@@ -584,8 +607,66 @@ int smooth (sx_dns* d, double* f, hipStream_t st)
 // skewSymmetric / altSkewSymmetric (dns/nonlinear.cpp): U holds the
 // Fourier velocity (the reference's Us[0] copy of D->u), N -> Uf0; the
 // physical-space velocity goes to phys.
+// out (S x cols, ld S) = alpha X (S x inner, ld S) W (inner x cols, ld inner): a
+// z transform of all S columns of a field as one FP64 GEMM
+int zgemm (sx_dns* d, const double* X, const double* W, double* out, int inner, int cols,
+           double alpha)
+{
+  const double beta0 = 0.0;
+  const rocblas_status r = rocblas_dgemm (d->blas, rocblas_operation_none,
+                                          rocblas_operation_none, d->S, cols, inner, &alpha,
+                                          X, d->S, W, inner, &beta0, out, d->S);
+  if (r != rocblas_status_success) {
+    set_error (std::string ("rocblas_dgemm: ") + rocblas_status_to_string (r));
+    return SX_ERR_HIP;
+  }
+  return SX_OK;
+}
+
+// skewSymmetric (dns/nonlinear.cpp:162-193) with every product in z formed
+// on M = 3N/2 physical planes (AuxField::transform32,
+// src/auxfield.cpp:1197-1258, with the 3/2 rule): the velocity and its z
+// derivative are zero-padded and inverse transformed onto M planes, the x-y
+// terms and the convective z term summed there, then forward transformed
+// and truncated to modes 0..N/2-1; d_z(u_i w) is taken in Fourier space
+// from the truncated product.
+int advection32 (sx_dns* d, double** U, double** N, hipStream_t st)
+{
+  const int nc = d->ncom, nz = d->nz, M = d->M;
+  const sx_mesh* m = d->mu;
+  int rc;
+  if (rocblas_set_stream (d->blas, st) != rocblas_status_success) {
+    set_error ("rocblas_set_stream failed");
+    return SX_ERR_HIP;
+  }
+  for (int i = 0; i < nc; i++)
+    if ((rc = zgemm (d, U[i], d->Wi, d->physM[i], nz, M, 1.0))) return rc;
+  const unsigned gM = nb256 (d->nM);
+  for (int i = 0; i < nc; i++) {
+    hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, M), dim3 (64), 0, st, m->np, m->d_geo, m->d_D,
+                        d->physM[0], d->physM[1], d->physM[i], d->tmpM, d->S, 1, 1);
+    if ((rc = zgemm (d, U[i], d->Wdz, d->tmpM2, nz, M, 1.0))) return rc;
+    hipLaunchKernelGGL (k_ew_submul, dim3 (gM), dim3 (256), 0, st, d->tmpM, d->physM[2],
+                        d->tmpM2, d->nM);
+    hipLaunchKernelGGL (k_ew_mul, dim3 (gM), dim3 (256), 0, st, d->tmpM2, d->physM[i],
+                        d->physM[2], d->nM);
+    SX_HIP (hipGetLastError ());
+    if ((rc = zgemm (d, d->tmpM, d->Wf, N[i], M, nz, 0.5))) return rc;
+    if ((rc = zgemm (d, d->tmpM2, d->Wf, d->tmp, M, nz, 0.5))) return rc;
+    hipLaunchKernelGGL (k_sub_dz, dim3 (nb256 ((size_t) (nz / 2 - 1) * d->S)), dim3 (256), 0,
+                        st, N[i], d->tmp, nz, d->S, d->beta);
+    if (std::fabs (d->cf[i]) > EPSDP)
+      hipLaunchKernelGGL (k_ew_addc, dim3 (nb256 (d->S)), dim3 (256), 0, st, N[i], d->cf[i],
+                          (size_t) d->S);
+    SX_HIP (hipGetLastError ());
+    if ((rc = smooth (d, N[i], st))) return rc;
+  }
+  return SX_OK;
+}
+
 int advection (sx_dns* d, double** U, double** N, hipStream_t st)
 {
+  if (d->dealias) return advection32 (d, U, N, st);
   const int nc = d->ncom;
   int rc;
   // N needs no clearing: k_adv_xy writes every element entry of it (the
@@ -774,6 +855,7 @@ extern "C" int sx_dns_destroy (sx_dns_t d)
 {
   if (!d) return SX_OK;
   for (double* q : d->bufs) (void) hipFree (q);
+  if (d->blas) (void) rocblas_destroy_handle (d->blas);
   void* ip[] = {d->seg_elmt, d->seg_idx, d->seg_kind};
   for (void* q : ip) if (q) (void) hipFree (q);
   for (sx_msys* M : {d->Mv[0], d->Mv[1], d->Mv[2], d->Mp, d->Mpcg[0], d->Mpcg[1]})
@@ -963,6 +1045,58 @@ extern "C" int sx_dns_set_velocity_bc (sx_dns_t d, const double* d_lines_phys, v
       return rc;
     }
   }
+  return SX_OK;
+}
+
+extern "C" int sx_dns_set_dealias (sx_dns_t d, int on, void* stream)
+{
+  if (!d || (on && (d->ncom != 3 || d->nz < 4 || d->nz % 4 || d->advection != SX_SKEW))) {
+    set_error ("sx_dns_set_dealias: needs a 3D field (nz a multiple of 4) and the "
+               "skew-symmetric advection");
+    return SX_ERR_ARG;
+  }
+  (void) stream;
+  if (on && !d->blas) {
+    const int N = d->nz, M = 3 * N / 2;
+    d->M = M;
+    d->nM = (size_t) M * d->S;
+    if (rocblas_create_handle (&d->blas) != rocblas_status_success) {
+      set_error ("rocblas_create_handle failed");
+      return SX_ERR_HIP;
+    }
+    for (int i = 0; i < 3; i++)
+      if (!(d->physM[i] = d->alloc (d->nM))) return hip_fail (hipErrorOutOfMemory, "dealias");
+    if (!(d->tmpM = d->alloc (d->nM)) || !(d->tmpM2 = d->alloc (d->nM)) ||
+        !(d->Wi = d->alloc ((size_t) N * M)) || !(d->Wdz = d->alloc ((size_t) N * M)) ||
+        !(d->Wf = d->alloc ((size_t) N * M)))
+      return hip_fail (hipErrorOutOfMemory, "dealias");
+    // packed real DFT matrices (femlib/fourier.c packing, the Nyquist plane
+    // of the N planes dropped): Wi (N x M, ld N) evaluates modes 0..N/2-1
+    // at the M points, Wdz the same after d/dz (i beta k), Wf (M x N, ld M)
+    // the forward transform scaled 1/M truncated to those modes
+    std::vector<double> wi ((size_t) N * M, 0.0), wdz ((size_t) N * M, 0.0),
+        wf ((size_t) N * M, 0.0);
+    const double tp = 2.0 * M_PI;
+    for (int j = 0; j < M; j++) {
+      wi[(size_t) j * N] = 1.0;
+      wf[j] = 1.0 / M;
+      for (int k = 1; k < N / 2; k++) {
+        const double a = tp * (double) (((long long) j * k) % M) / M;
+        const double c = std::cos (a), sn = std::sin (a), bk = d->beta * k;
+        wi[(size_t) j * N + 2 * k] = 2.0 * c;
+        wi[(size_t) j * N + 2 * k + 1] = -2.0 * sn;
+        wdz[(size_t) j * N + 2 * k] = -2.0 * bk * sn;
+        wdz[(size_t) j * N + 2 * k + 1] = -2.0 * bk * c;
+        wf[(size_t) (2 * k) * M + j] = c / M;
+        wf[(size_t) (2 * k + 1) * M + j] = -sn / M;
+      }
+    }
+    if (hipMemcpy (d->Wi, wi.data (), sizeof (double) * wi.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->Wdz, wdz.data (), sizeof (double) * wdz.size (), hipMemcpyHostToDevice) ||
+        hipMemcpy (d->Wf, wf.data (), sizeof (double) * wf.size (), hipMemcpyHostToDevice))
+      return hip_fail (hipErrorUnknown, "dealias matrices");
+  }
+  d->dealias = on ? 1 : 0;
   return SX_OK;
 }
 

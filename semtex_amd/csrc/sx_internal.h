@@ -80,6 +80,12 @@ int    condense32_tiled (const sx_msys* s, int nz, int stride, const double* X, 
                          hipStream_t st);
 int    recover32_tiled (const sx_msys* s, int nz, int stride, const double* Fp, double* V,
                         hipStream_t st);
+// The DIRECT solve of sx_field_solve with the workspace `s` names (d_G,
+// d_rhs, d_rhsb; split sweeps unless no_split) -- sx_solver.hip; and the
+// argument check of the DIRECT entry points.
+int    field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                           const double* d_lines, hipStream_t st, bool timed);
+int    check_direct_solve_args (sx_msys* s, int nz, int stride, const void* f, const void* u);
 // JACPCG set-up: the Jacobi preconditioner of every mode (sx_pcg.hip).
 int    pcg_setup (sx_msys* s, hipStream_t st);
 inline bool fourier_in_place (int nz) { return nz >= 8 && nz <= 1024 && (nz & (nz - 1)) == 0; }

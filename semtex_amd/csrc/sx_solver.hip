@@ -967,7 +967,8 @@ struct PhaseScope {
   PhaseRec r;
   hipStream_t st;
   bool on;
-  PhaseScope (int phase, hipStream_t s) : st (s), on (g_timing.on)
+  PhaseScope (int phase, hipStream_t s, bool allowed = true)
+    : st (s), on (allowed && g_timing.on)
   {
     if (!on) return;
     r.phase = phase; r.a = g_timing.get (); r.b = g_timing.get ();
@@ -1170,8 +1171,10 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
   return SX_OK;
 }
 
+// direct: the entry point runs the DIRECT solve, which a JACPCG system
+// (preconditioner only, no condensed matrices or band factor) cannot do
 static int check_solve_args (sx_msys_t s, int nz, int stride, const void* f,
-                             const void* u)
+                             const void* u, bool direct = true)
 {
   if (!s || !f || !u || nz < 1 || stride < s->m->nel * s->m->npnp) {
     set_error ("sx_field_solve: bad arguments");
@@ -1181,8 +1184,19 @@ static int check_solve_args (sx_msys_t s, int nz, int stride, const void* f,
     set_error ("sx_field_solve: more planes than modes in the system");
     return SX_ERR_ARG;
   }
+  if (direct && s->precision == SX_JACPCG) {
+    set_error ("sx_field_solve: a JACPCG system is solved with sx_field_solve_pcg");
+    return SX_ERR_ARG;
+  }
   return SX_OK;
 }
+
+namespace sx {
+int check_direct_solve_args (sx_msys* s, int nz, int stride, const void* f, const void* u)
+{
+  return check_solve_args (s, nz, stride, f, u, true);
+}
+}  // namespace sx
 
 namespace {
 
@@ -1278,27 +1292,32 @@ int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, c
 
 }  // namespace
 
-extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
-                               double* d_u, const double* d_lines, void* stream)
+namespace sx {
+
+// The DIRECT solve of Field::solve over nz planes with the system's
+// workspace (d_G, d_rhs, d_rhsb, and the split sweeps' rings when
+// no_split is 0) as `s` names it: sx_field_solve passes the system itself,
+// sx_field_solve_noalloc a copy whose workspace pointers lead into its
+// static scratch.  timed: record the per-phase HIP events when timing is on.
+int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                        const double* d_lines, hipStream_t st, bool timed)
 {
-  int rc = check_solve_args (s, nz, stride, d_f, d_u);
-  if (rc) return rc;
-  hipStream_t st = (hipStream_t) stream;
+  int rc;
   const sx_mesh* m = s->m;
   const int npl = n_solved_planes (nz, s->base_mode);
   const int lstride = m->nbound * m->np;
   const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
   {
-    PhaseScope ps (0, st);
+    PhaseScope ps (0, st, timed);
     hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                         st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
   }
   {
-    PhaseScope ps (1, st);
+    PhaseScope ps (1, st, timed);
     if ((rc = condense (s, nz, stride, d_f, s->d_G, st))) return rc;
   }
   {
-    PhaseScope ps (2, st);
+    PhaseScope ps (2, st, timed);
     hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                         st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
                         d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
@@ -1306,15 +1325,25 @@ extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
   }
   SX_HIP (hipGetLastError ());
   {
-    PhaseScope ps (3, st);
+    PhaseScope ps (3, st, timed);
     if ((rc = band_solve (s, 2, st))) return rc;
   }
   {
-    PhaseScope ps (4, st);
+    PhaseScope ps (4, st, timed);
     if ((rc = recover (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st))) return rc;
   }
   SX_HIP (hipGetLastError ());
   return SX_OK;
+}
+
+}  // namespace sx
+
+extern "C" int sx_field_solve (sx_msys_t s, int nz, int stride, double* d_f,
+                               double* d_u, const double* d_lines, void* stream)
+{
+  const int rc = check_solve_args (s, nz, stride, d_f, d_u);
+  if (rc) return rc;
+  return field_solve_direct (s, nz, stride, d_f, d_u, d_lines, (hipStream_t) stream, true);
 }
 
 namespace sx {
@@ -1455,7 +1484,7 @@ extern "C" int sx_msys_status (sx_msys_t s, void* stream)
 extern "C" int sx_helmholtz_op (sx_msys_t s, int nz, int stride, const double* d_u,
                                 double* d_hu, void* stream)
 {
-  int rc = check_solve_args (s, nz, stride, d_u, d_hu);
+  int rc = check_solve_args (s, nz, stride, d_u, d_hu, false);
   if (rc) return rc;
   const sx_mesh* m = s->m;
   hipLaunchKernelGGL (k_helm_op, dim3 (m->nel, n_solved_planes (nz, s->base_mode)),

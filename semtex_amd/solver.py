@@ -180,6 +180,14 @@ def fourier_transform(x, sign, out=None):
 SX_SKEW, SX_ALTSKEW = _lib.SX_SKEW, _lib.SX_ALTSKEW
 
 
+class _FieldHeader(C.Structure):
+    """sx_field_header (include/semtex_amd.h)."""
+    _fields_ = [("session", C.c_char * 64), ("created", C.c_char * 64), ("nr", C.c_int),
+                ("ns", C.c_int), ("nz", C.c_int), ("nel", C.c_int), ("step", C.c_int),
+                ("time", C.c_double), ("dt", C.c_double), ("kinvis", C.c_double),
+                ("beta", C.c_double), ("fields", C.c_char * 32), ("format", C.c_char * 64)]
+
+
 class DNS:
     """integrate() (dns/integrate.cpp:66-201) on the GPU: the velocity and
     pressure fields and their time history live in HBM; each step runs the
@@ -199,6 +207,8 @@ class DNS:
                                    advection, nord, _stream(), C.byref(h)), "sx_dns_create")
         self.handle = h
         self.stride = L.sx_dns_plane_stride(h)
+        self.dt, self.kinvis, self.beta = float(dt), float(kinvis), float(beta)
+        self.step_no = 0
 
     def set_state(self, u_phys):
         """u_phys: [ncom][nz][stride] CUDA float64, physical space."""
@@ -210,6 +220,12 @@ class DNS:
 
     def step(self, n=1):
         _lib.check(_lib.load().sx_dns_step(self.handle, int(n), _stream()), "sx_dns_step")
+        return self
+
+    def set_dealias(self, on=True):
+        """3/2-rule dealiasing of the skew-symmetric term's z products."""
+        _lib.check(_lib.load().sx_dns_set_dealias(self.handle, int(bool(on)), _stream()),
+                   "sx_dns_set_dealias")
         return self
 
     def set_velocity_bc(self, lines_phys):
@@ -234,6 +250,42 @@ class DNS:
                                                   int(step_max), _stream()),
                    "sx_dns_set_startup")
         return self
+
+    def dump(self, path, session="session", append=False):
+        """Domain::dump (src/domain.cpp:160-199): the velocity and pressure,
+        transformed to physical space, written as one dump of a field file
+        (writeField, src/misc.cpp:106-180) with the step and time."""
+        import torch
+        nz, S = self.nz, self.stride
+        mesh = self.gu.mesh
+        names = "uv" + ("w" if self.ncom == 3 else "") + "p"
+        host = []
+        for i in range(self.ncom + 1):
+            f = self.field(i)
+            if nz > 1:
+                f = fourier_transform(f, -1, out=torch.empty_like(f))
+            host.append(np.ascontiguousarray(f.cpu().numpy()))
+        h = _FieldHeader()
+        h.session = session.encode()[:63]
+        h.nr = h.ns = mesh.np
+        h.nz, h.nel, h.step = nz, mesh.nel, self.step_no
+        h.time, h.dt, h.kinvis, h.beta = self.step_no * self.dt, self.dt, self.kinvis, self.beta
+        h.fields = names.encode()
+        ptrs = (C.c_void_p * len(host))(*[a.ctypes.data for a in host])
+        _lib.check(_lib.load().sx_field_write(str(path).encode(), C.byref(h), ptrs, S,
+                                              int(bool(append))), "sx_field_write")
+
+    def restart(self, path, dump=0):
+        """Domain::restart (src/domain.cpp:123-157): the velocity of dump
+        `dump` of a field file (physical space) becomes the state."""
+        import torch
+        nz, S = self.nz, self.stride
+        u = np.zeros((self.ncom, nz, S))
+        for i, c in enumerate("uvw"[:self.ncom]):
+            _lib.check(_lib.load().sx_field_read(str(path).encode(), int(dump), c.encode(),
+                                                 C.c_void_p(u[i].ctypes.data), S),
+                       "sx_field_read")
+        self.set_state(torch.from_numpy(u).cuda())
 
     def field(self, which):
         """Fourier-space copy of velocity component `which` (ncom: pressure)."""

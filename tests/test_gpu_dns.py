@@ -232,8 +232,18 @@ def test_gpu_dns_channel_with_wall_data_step_parity(torch_cuda, oracle):
     D.set_velocity_bc(torch.from_numpy(lines).cuda())
     n, S = mu.nel * mu.npnp, D.stride
     y = mu.y.ravel()
+    wall = 1.0 - y ** 2
+    rng = np.random.default_rng(31)
+    from semtex_amd.mesh import edgemaps
+    em, _ = edgemaps(mu.np)
     u0 = np.zeros((3, nz, S))
-    u0[0, :, :n] = 1.0 - y ** 2
+    for i in range(3):
+        u0[i, :, :n] = (wall if i == 0 else 0.0) + 0.2 * rng.uniform(-1, 1, (nz, n)) * wall
+        for k in range(nz):
+            f = u0[i, k, :n].reshape(mu.nel, -1)
+            g = np.zeros(mu.nglobal)
+            g[mu.btog] = f[:, em[:mu.next]]
+            f[:, em[:mu.next]] = g[mu.btog]
     O.set_initial(u0[:, :, :n])
     D.set_state(torch.from_numpy(u0).cuda())
     planes = [k for k in range(nz) if k != 1]
@@ -246,3 +256,84 @@ def test_gpu_dns_channel_with_wall_data_step_parity(torch_cuda, oracle):
             o = O.u[i] if i < 3 else O.p
             rel = np.linalg.norm(g[planes] - o[planes]) / np.linalg.norm(o[planes])
             assert rel < 1e-11, (s, i, rel)
+
+
+@pytest.mark.parametrize("nz", [16, 32])
+def test_gpu_dns_dealiased_step_parity(torch_cuda, oracle, nz):
+    """3/2-rule dealiasing (sx_dns_set_dealias): the walled channel against
+    the oracle's dealiased step (DnsOracle.dealias), every field after each
+    of 5 steps, at the DNS bar."""
+    import torch
+    from oracle.dns_oracle import DnsOracle
+    from semtex_amd.mesh import edgemaps, rect_mesh
+    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh
+    print(f"case: channel 3x4 np=6 nz={nz} skew dealiased, 5 steps", flush=True)
+    bcu, bcp = ("D", "N", "D", "N"), ("N", "N", "N", "N")
+    mu = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcu, periodic_x=True)
+    mp = rect_mesh(3, 4, 6, 0.0, 2 * np.pi, -1.0, 1.0, bc=bcp, periodic_x=True)
+    beta, nu, dt = 1.0, 1.0 / 50, 5e-3
+    cf = (2 * nu, 0.0, 0.0)
+    D = DNS(GpuMesh(mu), GpuMesh(mp), nz, beta, nu, dt, const_force=cf, advection=SX_SKEW)
+    D.set_dealias(True)
+    O = DnsOracle(mu, mp, nz, beta, nu, dt, const_force=cf, advection="skew")
+    O.dealias = True
+    n, S = mu.nel * mu.npnp, D.stride
+    rng = np.random.default_rng(77)
+    wall = 1.0 - mu.y.ravel() ** 2
+    em, _ = edgemaps(mu.np)
+    u0 = np.zeros((3, nz, S))
+    for i in range(3):
+        u0[i, :, :n] = (wall if i == 0 else 0.0) + 0.3 * rng.uniform(-1, 1, (nz, n)) * wall
+        for k in range(nz):
+            f = u0[i, k, :n].reshape(mu.nel, -1)
+            g = np.zeros(mu.nglobal)
+            g[mu.btog] = f[:, em[:mu.next]]
+            f[:, em[:mu.next]] = g[mu.btog]
+    O.set_initial(u0[:, :, :n])
+    D.set_state(torch.from_numpy(u0).cuda())
+    planes = [k for k in range(nz) if k != 1]
+    for s in range(5):
+        O.step()
+        D.step(1)
+        torch.cuda.synchronize()
+        for i in range(4):
+            g = D.field(i).cpu().numpy()[:, :n]
+            o = O.u[i] if i < 3 else O.p
+            rel = np.linalg.norm(g[planes] - o[planes]) / np.linalg.norm(o[planes])
+            assert rel < 1e-11, (s, i, rel)
+
+
+def test_gpu_dns_dump_and_restart_through_field_file(torch_cuda, tmp_path):
+    """Domain::dump / Domain::restart through a field file: the state after
+    3 steps, written in physical space (sx_field_write) and read back into
+    a fresh DNS (sx_field_read + sx_dns_set_state), equals the original
+    velocity to round-off; the header carries the step, time and fields."""
+    import ctypes as C
+
+    import torch
+    from semtex_amd import _lib
+    from semtex_amd.problems import dns_channel
+    from semtex_amd.solver import DNS, SX_SKEW, GpuMesh, _FieldHeader
+    print("case: channel 4x6 np=6 nz=16, dump after 3 steps and restart", flush=True)
+    P = dns_channel(nx=4, ny=6, np_=6, nz=16)
+    gu, gp = GpuMesh(P["mesh_u"]), GpuMesh(P["mesh_p"])
+    kw = dict(const_force=P["const_force"], advection=SX_SKEW)
+    A = DNS(gu, gp, P["nz"], P["beta"], P["kinvis"], P["dt"], **kw)
+    n = P["mesh_u"].nel * P["mesh_u"].npnp
+    u0 = torch.zeros((3, P["nz"], A.stride), dtype=torch.float64)
+    u0[:, :, :n] = torch.from_numpy(P["u0"])
+    A.set_state(u0.cuda())
+    A.step(3)
+    path = str(tmp_path / "chan.fld")
+    A.dump(path, session="chan")
+    h = _FieldHeader()
+    _lib.check(_lib.load().sx_field_read_header(path.encode(), 0, C.byref(h)), "header")
+    assert h.step == 3 and h.fields == b"uvwp" and h.nz == P["nz"] and h.nel == P["mesh_u"].nel
+    assert abs(h.time - 3 * P["dt"]) < 1e-15
+    B = DNS(gu, gp, P["nz"], P["beta"], P["kinvis"], P["dt"], **kw)
+    B.restart(path)
+    planes = [k for k in range(P["nz"]) if k != 1]
+    for i in range(3):
+        a = A.field(i).cpu().numpy()[planes, :n]
+        b = B.field(i).cpu().numpy()[planes, :n]
+        assert np.linalg.norm(a - b) < 1e-14 * np.linalg.norm(a), i

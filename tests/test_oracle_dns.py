@@ -242,3 +242,63 @@ def test_walled_channel_converges_at_second_order(oracle):
     d2 = np.max(np.abs(sols[1] - sols[2]))
     ratio = d1 / d2
     assert 3.3 < ratio < 4.7, (d1, d2, ratio)
+
+
+def test_dealiased_products_are_exact(oracle):
+    """transform32 with the 3/2 rule: the product of two band-limited
+    series (modes |k| < N/2) formed on M = 3N/2 planes and truncated equals
+    the exact product's modes 0..N/2-1 (formed on 4N planes), while the
+    plain N-plane product aliases."""
+    from oracle.dns_oracle import fwd, fwd32, inv, inv32
+    rng = np.random.default_rng(8)
+    N, n = 16, 7
+    A, B = np.zeros((N, n)), np.zeros((N, n))
+    A[0], B[0] = rng.standard_normal(n), rng.standard_normal(n)
+    A[2:], B[2:] = rng.standard_normal((N - 2, n)), rng.standard_normal((N - 2, n))
+
+    def pad(X, L):                        # same modes on L planes
+        Y = np.zeros((L, n))
+        Y[0] = X[0]
+        Y[2:N] = X[2:]
+        return Y
+    exact = fwd(inv(pad(A, 4 * N)) * inv(pad(B, 4 * N)))
+    want = np.zeros((N, n))
+    want[0], want[2:] = exact[0], exact[2:N]
+    got = fwd32(inv32(A, 3 * N // 2) * inv32(B, 3 * N // 2), N)
+    assert np.max(np.abs(got - want)) < 1e-13 * np.max(np.abs(want))
+    plain = fwd(inv(A) * inv(B))
+    plain[1] = 0.0
+    assert np.max(np.abs(plain - want)) > 1e-2 * np.max(np.abs(want))
+
+
+def test_dealiased_nonlinear_term_equals_resolved_one(oracle):
+    """The dealiased skew-symmetric term on N planes equals the plain term
+    computed on 2N planes (where no product of the N-plane modes aliases)
+    truncated to modes 0..N/2-1, for a band-limited walled-channel field;
+    the plain N-plane term differs."""
+    from oracle.dns_oracle import DnsOracle, fwd, inv
+    mu, mp = channel(nx=3, ny=3, np_=6, nz=8)
+    N, n = 8, mu.nel * mu.npnp
+    rng = np.random.default_rng(12)
+    wall = 1.0 - mu.y.ravel() ** 2
+    U = np.zeros((3, N, n))
+    for i in range(3):
+        U[i] = rng.standard_normal((N, n)) * wall
+        U[i][1] = 0.0
+    D32 = DnsOracle(mu, mp, N, 1.0, 0.01, 1e-3, const_force=(0.02, 0.0, 0.0))
+    D32.dealias = True
+    D32.u = U.copy()
+    _, N32 = D32.skew_symmetric()
+    D2 = DnsOracle(mu, mp, 2 * N, 1.0, 0.01, 1e-3, const_force=(0.02, 0.0, 0.0))
+    U2 = np.zeros((3, 2 * N, n))
+    U2[:, 0], U2[:, 2:N] = U[:, 0], U[:, 2:]
+    D2.u = U2
+    _, NB = D2.skew_symmetric()
+    want = np.zeros_like(N32)
+    want[:, 0], want[:, 2:] = NB[:, 0], NB[:, 2:N]
+    assert np.max(np.abs(N32 - want)) < 1e-12 * np.max(np.abs(want))
+    Dp = DnsOracle(mu, mp, N, 1.0, 0.01, 1e-3, const_force=(0.02, 0.0, 0.0))
+    Dp.u = U.copy()
+    _, NP = Dp.skew_symmetric()
+    NP[:, 1] = 0.0
+    assert np.max(np.abs(NP - want)) > 1e-3 * np.max(np.abs(want))
