@@ -291,6 +291,23 @@ int sx_field_read_header (const char* path, int dump, sx_field_header* hdr);
  * format names the other endianness (Header::swab, src/data2df.cpp:527). */
 int sx_field_read (const char* path, int dump, char name, double* data, int plane_stride);
 
+/* The numbering `enumerate -O<optlev>` generates for one field
+ * (utility/enumerate.cpp:110-199 with Mesh::buildMap/buildMask,
+ * src/mesh.cpp:884-1326): the element-boundary to global map of the
+ * condensed system, essential nodes numbered last, the unknowns renumbered
+ * by reverse Cuthill-McKee from a pseudo-peripheral root (optlev 1; 2 and
+ * 3 search roots for the least band width; 0 keeps the traversal order).
+ * elem_nodes: [nel][4] 0-based node indices (the session's NODES order) of
+ * each quadrilateral, counter-clockwise.  surf: [nsurf][5] the SURFACES in
+ * session order, 0-based: {elmt, side, 0, essential, 0} for a boundary
+ * side whose group's BC for this field is essential (<D>, or <A>/<O> as
+ * Mesh::buildMask decides), {elmt, side, 1, mate elmt, mate side} for a
+ * periodic pair.  Out: btog, bmask [nel][4(np-1)] (emap order), nglobal,
+ * nsolve, nband (band width with the diagonal, Nsys::globalBandwidth). */
+int sx_enumerate (int nel, int nnode, const int* elem_nodes, int nsurf, const int* surf,
+                  int np, int optlev, int* btog, int* bmask, int* nglobal, int* nsolve,
+                  int* nband);
+
 /* session.num numbering files.  sx_num_write prints the enumerate summary
  * and table (utility/enumerate.cpp:441-552): `fields` all field names,
  * nset numbering systems, set_fields[j] the fields sharing system j,

@@ -56,6 +56,7 @@ SIGNATURES = {
     "sx_field_write": (_i, [C.c_char_p, _vp, _vp, _i, _i]),
     "sx_field_read_header": (_i, [C.c_char_p, _i, _vp]),
     "sx_field_read": (_i, [C.c_char_p, _i, C.c_char, _dp, _i]),
+    "sx_enumerate": (_i, [_i, _i, _vp, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "sx_num_write": (_i, [C.c_char_p, C.c_char_p, _i, _vp, _i, _i, _vp, _vp, _vp, _vp, _vp,
                           _vp]),
     "sx_num_read": (_i, [C.c_char_p, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
