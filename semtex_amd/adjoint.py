@@ -270,8 +270,10 @@ class DnsTape:
         if Je:
             a = SS[Je]
             work = a[0] * un_new
+            # as src/bcmgr.cpp:1124-1130 forms it: every a[q+1] multiplies the
+            # previous step's u.n (un[0]), not level q
             for q in range(Je):
-                work = work + a[q + 1] * un[q]
+                work = work + a[q + 1] * un[0]
             h = h - work / dt
         # nu n.curl curl u on every plane
         w = self.grad_xy(U[1], 0) - self.grad_xy(U[0], 1)

@@ -3,7 +3,10 @@ lambda2, Cartesian and cylindrical (axisymmetric, axis on y = 0), solved by
 one captured HIP graph (sx_batch_*), its problems on concurrent branches.
 
 * FP64 graph replays (three, with eager work and synchronisation between
-  them) equal the eager per-problem solves exactly (same kernels);
+  them) equal the eager per-problem solves exactly (same kernels: the
+  graph records no split band sweeps, so the eager references run with
+  SX_BAND_SPLIT=0), and the systems keep their split sweeps outside the
+  batch;
 * the eager FP64 solves match the oracle to 1e-12;
 * the mixed-precision graph (FP32 element work, 3 FP64 refinement passes)
   is within 1e-10 of FP64 for every problem.
@@ -45,19 +48,29 @@ def _build(torch, probs, precision):
     return out
 
 
-def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems):
+def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems, monkeypatch):
     import torch
     from semtex_amd.solver import SX_FP64, SX_MIXED, Batch
     print("case: uneven batch of", len(batch_problems), flush=True)
     built = _build(torch, batch_problems, SX_FP64)
-    # the batch first: its systems then solve as its graph does (no split
-    # band sweeps, whose grids must be resident alone), eagerly as well
     B = Batch([(M, u, src) for (M, u, src, _) in built])
+    # the systems keep their own sweep choice outside the batch: a twisted
+    # system still gets the split sweeps (two workgroups per chain) ...
+    split_seen = False
+    for (M, u, src, _) in built:
+        u.solve(src.clone(), M)
+        split_seen = split_seen or M.layout()["kernel"] == 3
+    torch.cuda.synchronize()
+    assert split_seen, [M.layout() for (M, _, _, _) in built]
+    # ... while the graph's recorded solves use none (its branches run
+    # concurrently); eager solves without them are its exact reference
+    monkeypatch.setenv("SX_BAND_SPLIT", "0")
     eager = []
     for (M, u, src, _) in built:
         u.solve(src.clone(), M)
         eager.append(u.data.clone())
         assert M.layout()["kernel"] != 3, M.layout()
+    monkeypatch.delenv("SX_BAND_SPLIT")
     torch.cuda.synchronize()
     # FP64 oracle parity of the eager solves
     for p, e in zip(batch_problems, eager):
@@ -88,7 +101,7 @@ def test_batch_graph_matches_eager_and_oracle(torch_cuda, oracle, batch_problems
             assert rel < 1e-10, (rep, p["np"], p["lambda2"], p["cyl"], rel)
 
 
-def test_batch_problems_sharing_a_system(torch_cuda, batch_problems):
+def test_batch_problems_sharing_a_system(torch_cuda, batch_problems, monkeypatch):
     """Problems that share a system (and so its workspace) run in order on
     one branch of the graph: three forcings on one system, two on another,
     each equal to its eager solve."""
@@ -103,10 +116,12 @@ def test_batch_problems_sharing_a_system(torch_cuda, batch_problems):
             s = src * (1.0 + r) + 0.25 * r
             entries.append((M, u, s))
     B = Batch(entries)
+    monkeypatch.setenv("SX_BAND_SPLIT", "0")          # as the graph's recorded solves
     for (M, u, s) in entries:
         v = Field(M.gmesh, 1)
         v.solve(s.clone(), M)
         refs.append(v.data.clone())
+    monkeypatch.delenv("SX_BAND_SPLIT")
     torch.cuda.synchronize()
     for rep in range(2):
         for (_, u, _) in entries:
