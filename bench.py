@@ -119,8 +119,9 @@ def cpu_baseline(P, seconds):
         if el >= seconds:
             break
     return {"value": count / el, "unit": "mode-solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle Field::solve, modes 0-7 (15 planes) x {reps} repetitions, "
-                      f"{el:.1f} s, same mesh/numbering/forcing, 1 thread"}
+            "sample": f"oracle Field::solve (plain C, no BLAS), modes 0-7 (15 planes) x {reps} "
+                      f"repetitions, {el:.1f} s, same mesh/numbering/forcing, 1 thread; "
+                      f"CPU {cpu_model()}"}
 
 
 THREAD_CAPS = {"OMP_NUM_THREADS": "1", "OPENBLAS_NUM_THREADS": "1", "MKL_NUM_THREADS": "1"}

@@ -2,8 +2,10 @@
 src/matrix.cpp:269-310, src/element.cpp:1484-1533):
 
 * the Jacobi preconditioner is the inverse diagonal of the global matrix
-  assembled independently from dense element matrices (affine elements;
-  the reference's cross-term indexing only matters where Q3 != 0);
+  assembled independently from dense element matrices (affine elements);
+  on curved elements (Q3 != 0) it carries the reference's cross term
+  2 Q3 dvr[j*j] dvs[i*i] (linear indices, src/element.cpp:1519), checked
+  against the dense diagonal plus that term's difference from the true one;
 * with a tight tolerance the PCG solution equals the direct (condensed,
   banded Cholesky) oracle solve, with Dirichlet and Neumann data and
   Fourier constants, for two numberings;
@@ -80,3 +82,52 @@ def test_default_tolerance_converges_to_its_reach():
     up, it = S.solve_plane_pcg(f, line, tol_rel=1e-8)
     assert 0 < it < 50
     assert np.linalg.norm(up - ud) / np.linalg.norm(ud) < 1e-6
+
+
+@pytest.mark.parametrize("lam2,bk2", [(1.0, 0.0), (2.5, 4.0)])
+def test_preconditioner_cross_term_on_curved_elements(lam2, bk2):
+    """HelmholtzDiag's cross term on non-affine (curved) elements, where Q3
+    is nonzero.  The reference forms it as 2 Q3[ij] dvr[j*j] dvs[i*i]
+    (src/element.cpp:1519,1530), reading the derivative matrix at the
+    linear indices j*j and i*i rather than its diagonal (j*np+j, i*np+i).
+    Expected: the diagonal of the independently assembled dense element
+    matrices (true cross term 2 Q3 D[j,j] D[i,i]) plus, per element node,
+    2 Q3 (D.flat[j*j] D.flat[i*i] - D[j,j] D[i,i]), with Q3 = W (rx sx +
+    ry sy) computed here from the nodal geometry.  A dropped cross term,
+    the true diagonal in its place, a missing factor 2 or a transposed
+    derivative matrix each fail this."""
+    from semtex_amd.mesh import gll
+    m = rect_mesh(3, 2, 7, 0.0, 2.0, -1.0, 0.5, bc=("D", "D", "D", "D"))
+    x0, y0 = m.x.copy(), m.y.copy()
+    m.x = x0 + 0.06 * np.sin(np.pi * y0) * np.sin(0.5 * np.pi * x0)
+    m.y = y0 + 0.05 * np.sin(np.pi * x0)
+    S = OracleMsys(OracleMesh(m), lam2, bk2)
+    pc = S.pcg_diag()
+    np_ = m.np
+    _, w, D = gll(np_)
+    I = np.eye(np_)
+    Dr, Ds = np.kron(I, D), np.kron(D, I)
+    em, _ = edgemaps(np_)
+    nb, ni = m.next, m.nint
+    want = global_diag(m, lam2 + bk2)
+    jj = np.array([D.flat[j * j] for j in range(np_)])
+    corr = np.zeros(m.npnp)
+    for i in range(np_):
+        for j in range(np_):
+            corr[i * np_ + j] = jj[j] * jj[i] - D[j, j] * D[i, i]
+    q3_seen = 0.0
+    for e in range(m.nel):
+        X, Y = m.x[e], m.y[e]
+        xr, xs, yr, ys = Dr @ X, Ds @ X, Dr @ Y, Ds @ Y
+        J = xr * ys - xs * yr
+        rx, ry, sx, sy = ys / J, -xs / J, -yr / J, xr / J
+        Q3 = np.outer(w, w).ravel() * J * (rx * sx + ry * sy)
+        q3_seen = max(q3_seen, float(np.max(np.abs(Q3))))
+        g = np.empty(m.npnp, dtype=np.int64)
+        g[em[:nb]] = m.btog[e]
+        g[em[nb:]] = m.nglobal + e * ni + np.arange(ni)
+        np.add.at(want, g, 2.0 * Q3 * corr)
+    assert q3_seen > 1e-3                       # the elements are genuinely curved
+    assert np.max(np.abs(pc - 1.0 / want) * np.abs(want)) < 1e-12
+    # and the reference's indexing is not the true diagonal here
+    assert np.max(np.abs(pc * global_diag(m, lam2 + bk2) - 1.0)) > 1e-6
