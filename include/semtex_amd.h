@@ -131,6 +131,21 @@ int sx_field_solve_mixed (sx_msys_t s, int nz, int plane_stride, double* d_f,
                           double refine_tol, int max_iter, int* iters,
                           void* stream);
 
+/* sx_field_solve_mixed without device allocation.  No reference
+ * counterpart: for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, same
+ * kernels and the same results; the workspace (5*nz*plane_stride + 1024
+ * doubles) lives in a fixed device array of SX_NOALLOC_DOUBLES doubles
+ * (80 MiB) that each GPU holds until the process ends, instead of one the
+ * system allocates.  Larger nz*plane_stride is rejected with SX_ERR_LIMIT.
+ * All calls on one GPU share that array: calls that overlap on one GPU
+ * (two streams or two host threads) race. */
+#define SX_NOALLOC_DOUBLES (10 * 1024 * 1024)
+int sx_field_solve_mixed_noalloc (sx_msys_t s, int nz, int plane_stride, double* d_f,
+                                  double* d_u, const double* d_lines,
+                                  double refine_tol, int max_iter, int* iters,
+                                  void* stream);
+
 /* Field::solve, JACPCG branch (src/field.cpp:711-805), every solved plane
  * at once, each with its own stopping test: d_u holds the initial guess on
  * entry (the field's current values, as in the reference) and the solution

@@ -15,6 +15,7 @@ SX_OK = 0
 SX_ERR_ARG, SX_ERR_HIP, SX_ERR_GEOM, SX_ERR_FACTOR, SX_ERR_LIMIT, SX_ERR_SYNC = -1, -2, -3, -4, -5, -6
 SX_FP64, SX_MIXED, SX_JACPCG = 0, 1, 2
 SX_SKEW, SX_ALTSKEW = 0, 1
+SX_NOALLOC_DOUBLES = 10 * 1024 * 1024   # fixed workspace of sx_field_solve_mixed_noalloc
 
 # Every symbol include/semtex_amd.h declares, with (restype, argtypes).
 _vp, _i, _d, _ip, _dp = C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p
@@ -34,6 +35,8 @@ SIGNATURES = {
     "sx_field_solve": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
     "sx_field_solve_mixed": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i,
                                   C.POINTER(_i), _vp]),
+    "sx_field_solve_mixed_noalloc": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i,
+                                          C.POINTER(_i), _vp]),
     "sx_field_solve_host": (_i, [_vp, _i, _i, _dp, _dp, _dp]),
     "sx_helmholtz_op": (_i, [_vp, _i, _i, _dp, _dp, _vp]),
     "sx_fourier_transform": (_i, [_i, _i, _dp, _dp, _i, _vp]),

@@ -73,6 +73,16 @@ int    fourier_z_fused (int nz, int S, const double* in, const double* mul, doub
 // synchronisation (sx_mixed.hip).
 int    field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
                                  const double* d_lines, int passes, hipStream_t st);
+// The tolerance-driven mixed solve (sx_field_solve_mixed) on a workspace of
+// at least mixed_work_doubles() doubles that the caller provides; the
+// argument check shared by its entry points.
+constexpr int MIXED_NBLK = 256;     // workgroups of the correction-norm reduction
+size_t mixed_work_doubles (const sx_msys* s, int nz, int stride);
+int    field_solve_mixed_ws (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                             const double* d_lines, double refine_tol, int max_iter,
+                             int* iters, double* work, hipStream_t st);
+int    check_mixed_solve_args (sx_msys* s, int nz, int stride, const void* f, const void* u,
+                               int max_iter, const char* who);
 // The mixed solve's FP32 element passes on MFMA tiles (systems with
 // d_Mrec32): X (weak, row-major) -> Fp [condensed F_b | F_i] in emap order;
 // Fp + band solution (d_rhs, 2 per mode) -> V row-major.

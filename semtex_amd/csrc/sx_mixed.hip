@@ -307,7 +307,7 @@ int field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, doubl
   const sx_mesh* m = s->m;
   const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
   const size_t nplane = (size_t) nz * stride;
-  const size_t need = 5 * nplane + 2 * 512;
+  const size_t need = mixed_work_doubles (s, nz, stride);
   if (s->work_doubles < need) {
     if (s->d_work) (void) hipFree (s->d_work);
     s->d_work = nullptr;
@@ -347,35 +347,27 @@ int field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, doubl
 
 }  // namespace sx
 
-extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_f,
-                                     double* d_u, const double* d_lines,
-                                     double refine_tol, int max_iter, int* iters,
-                                     void* stream)
+namespace sx {
+
+size_t mixed_work_doubles (const sx_msys* s, int nz, int stride)
 {
-  if (!s || !d_f || !d_u || nz < 1 || stride < s->m->nel * s->m->npnp ||
-      (nz > 2 * s->nmodes && nz != 1) || max_iter < 0) {
-    set_error ("sx_field_solve_mixed: bad arguments");
-    return SX_ERR_ARG;
-  }
-  if (s->precision != SX_MIXED || !s->d_Bm32) {
-    set_error ("sx_field_solve_mixed: system was not created with SX_MIXED");
-    return SX_ERR_ARG;
-  }
-  hipStream_t st = (hipStream_t) stream;
+  (void) s;
+  return 5 * (size_t) nz * stride + 4 * MIXED_NBLK;
+}
+
+// The mixed solve of sx_field_solve_mixed on a caller-provided workspace of
+// at least mixed_work_doubles() doubles: W, X, Fp, V, dV and the norm
+// partials.  Every kernel writes the entries of the solved planes it reads
+// later, so the workspace needs no clearing.
+int field_solve_mixed_ws (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                          const double* d_lines, double refine_tol, int max_iter, int* iters,
+                          double* work, hipStream_t st)
+{
   const sx_mesh* m = s->m;
   const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
   const size_t nplane = (size_t) nz * stride;
   const double tol = refine_tol > 0.0 ? refine_tol : 1e-12;
-  // workspace: W, X, Fp, V, dV (+ norm partials)
-  const size_t need = 5 * nplane + 2 * 512;
-  if (s->work_doubles < need) {
-    if (s->d_work) (void) hipFree (s->d_work);
-    s->d_work = nullptr;
-    s->work_doubles = 0;
-    SX_HIP (hipMalloc (&s->d_work, sizeof (double) * need));
-    s->work_doubles = need;
-  }
-  double *W = s->d_work, *X = W + nplane, *Fp = X + nplane, *V = Fp + nplane,
+  double *W = work, *X = W + nplane, *Fp = X + nplane, *V = Fp + nplane,
          *dV = V + nplane, *part = dV + nplane;
   const int lstride = m->nbound * m->np;
   const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
@@ -388,8 +380,7 @@ extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_
   int rc = inner_solve (s, nz, stride, W, Fp, V, lines, st);
   if (rc) return rc;
   const int n_plane = m->nel * m->npnp;
-  const int nblk = 256;
-  std::vector<double> hp (2 * nblk);
+  double hp[2 * MIXED_NBLK];
   int it = 0;
   for (; it < max_iter; it++) {
     hipLaunchKernelGGL (k_resid, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->cyl,
@@ -397,15 +388,15 @@ extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_
                         s->base_mode);
     SX_HIP (hipGetLastError ());
     if ((rc = inner_solve (s, nz, stride, X, Fp, dV, lines, st))) return rc;
-    hipLaunchKernelGGL (k_norms, dim3 (nblk), dim3 (256), 0, st, V, dV, npl, nz, s->base_mode,
-                        n_plane, stride, part);
+    hipLaunchKernelGGL (k_norms, dim3 (MIXED_NBLK), dim3 (256), 0, st, V, dV, npl, nz,
+                        s->base_mode, n_plane, stride, part);
     hipLaunchKernelGGL (k_axpy_planes, dim3 ((unsigned) (((size_t) npl * n_plane + 255) / 256)),
                         dim3 (256), 0, st, V, dV, npl, nz, s->base_mode, n_plane, stride);
     SX_HIP (hipGetLastError ());
-    SX_HIP (hipMemcpyAsync (hp.data (), part, sizeof (double) * 2 * nblk, hipMemcpyDeviceToHost, st));
+    SX_HIP (hipMemcpyAsync (hp, part, sizeof (double) * 2 * MIXED_NBLK, hipMemcpyDeviceToHost, st));
     SX_HIP (hipStreamSynchronize (st));
     double nv = 0.0, nd = 0.0;
-    for (int b = 0; b < nblk; b++) { nv += hp[2 * b]; nd += hp[2 * b + 1]; }
+    for (int b = 0; b < MIXED_NBLK; b++) { nv += hp[2 * b]; nd += hp[2 * b + 1]; }
     if (nd <= tol * tol * std::max (nv, 1e-300)) { it++; break; }
   }
   hipLaunchKernelGGL (k_finish, dim3 (m->nel, npl), dim3 (64), 0, st, m->np, m->d_emap,
@@ -414,4 +405,41 @@ extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_
   SX_HIP (hipGetLastError ());
   if (iters) *iters = it;
   return SX_OK;
+}
+
+int check_mixed_solve_args (sx_msys* s, int nz, int stride, const void* f, const void* u,
+                            int max_iter, const char* who)
+{
+  if (!s || !f || !u || nz < 1 || stride < s->m->nel * s->m->npnp ||
+      (nz > 2 * s->nmodes && nz != 1) || max_iter < 0) {
+    set_error (std::string (who) + ": bad arguments");
+    return SX_ERR_ARG;
+  }
+  if (s->precision != SX_MIXED || !s->d_Bm32) {
+    set_error (std::string (who) + ": system was not created with SX_MIXED");
+    return SX_ERR_ARG;
+  }
+  return SX_OK;
+}
+
+}  // namespace sx
+
+extern "C" int sx_field_solve_mixed (sx_msys_t s, int nz, int stride, double* d_f,
+                                     double* d_u, const double* d_lines,
+                                     double refine_tol, int max_iter, int* iters,
+                                     void* stream)
+{
+  if (int rc = check_mixed_solve_args (s, nz, stride, d_f, d_u, max_iter, "sx_field_solve_mixed"))
+    return rc;
+  // workspace: W, X, Fp, V, dV (+ norm partials), kept with the system
+  const size_t need = mixed_work_doubles (s, nz, stride);
+  if (s->work_doubles < need) {
+    if (s->d_work) (void) hipFree (s->d_work);
+    s->d_work = nullptr;
+    s->work_doubles = 0;
+    SX_HIP (hipMalloc (&s->d_work, sizeof (double) * need));
+    s->work_doubles = need;
+  }
+  return field_solve_mixed_ws (s, nz, stride, d_f, d_u, d_lines, refine_tol, max_iter, iters,
+                               s->d_work, (hipStream_t) stream);
 }
