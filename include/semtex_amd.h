@@ -36,6 +36,9 @@ extern "C" {
 
 #define SX_ESSENTIAL 0       /* boundary kinds (src/condition.h)           */
 #define SX_NATURAL   1
+#define SX_AXIS      2       /* cylindrical axis, natural with no flux      */
+                             /* (class Axis, src/condition.cpp): keeps its  */
+                             /* place in the boundary-line layout           */
 
 #define SX_FP64  0           /* solver precisions                          */
 #define SX_MIXED 1           /* FP32 elemental applies + FP64 refinement   */
@@ -88,7 +91,38 @@ int sx_mesh_nclass (sx_mesh_t m);
 int sx_msys_create (sx_mesh_t m, double lambda2, double beta, int base_mode,
                     int num_modes, int precision, void* stream,
                     sx_msys_t* out);
+/* The same for the field `name` ('u', 'v', 'w', 'p' or 'c'), whose Fourier
+ * constant is Field::modeConstant (src/field.cpp:1204-1229): beta*mode,
+ * except on a cylindrical mesh (cyl = 1) for the coupled radial and
+ * azimuthal velocities v~ = v + i w and w~ = v - i w, which solve with
+ * beta*mode + 1 ('v') and beta*mode - 1 ('w'), and with 1 for mode 0.
+ * sx_msys_create is this with name 'c'.  As in ModalMatrixSys over a
+ * BoundarySys (src/bsys.cpp:42-123), a cylindrical 3D field whose axis
+ * condition changes with the mode (BCmgr::getCondition,
+ * src/bcmgr.cpp:478-530: u, p, c natural on the axis for mode 0 and
+ * essential above; v~ essential; w~ natural for mode 1 only) takes one
+ * mesh (numbering) per mode class and one system per class: base_mode 0,
+ * 1 and 2 with 1, 1 and num_modes-2 modes, solved on the matching planes
+ * ([0,2), [2,4), [4,nz)) of the field. */
+int sx_msys_create_named (sx_mesh_t m, double lambda2, double beta, int base_mode,
+                          int num_modes, char name, int precision, void* stream,
+                          sx_msys_t* out);
 int sx_msys_destroy (sx_msys_t s);
+
+/* AuxField::couple (src/auxfield.cpp:1376-1451) and the FORWARD branch of
+ * Field::coupleBCs (src/field.cpp:1127-1201) on nz planes of n values at
+ * stride plane_stride (fields: n = nel*np*np; boundary lines: n =
+ * plane_stride = nbound*np), modes base_mode+k for planes 2k, 2k+1:
+ *   dir SX_FORWARD (+1):  v~ = v + i w,  w~ = v - i w
+ *   dir SX_INVERSE (-1):  v = (v~ + w~)/2,  w = i (w~ - v~)/2
+ * in place in d_v and d_w; mode 0 (base_mode 0) and nz = 1 are left
+ * unchanged.  (The reference's Field::coupleBCs INVERSE branch, never
+ * called, is not the inverse of its FORWARD branch and is not provided.) */
+#define SX_FORWARD  1
+#define SX_INVERSE -1
+int sx_couple (int nz, int plane_stride, int n, double* d_v, double* d_w, int dir,
+               int base_mode, void* stream);
+
 /* Status of the solves issued on this system since the last query:
  * synchronises `stream`, returns SX_OK, or SX_ERR_SYNC (and clears the
  * status) when a wait of the split band sweeps gave up.  The split sweeps

@@ -262,3 +262,61 @@ def solve_modes(mesh, lambda2, beta, planes, lines=None):
         line = None if lines is None else lines[k]
         out[k] = systems[mode].solve_plane(planes[k], line)
     return out
+
+
+# --------------------------------------------------------------------------
+# Cylindrical 3D: mode constants, coupling, per-mode-class numbering
+def mode_constant(name, cyl, mode, beta):
+    """Field::modeConstant (src/field.cpp:1204-1229): beta*mode, except for
+    the coupled radial / azimuthal velocities of a cylindrical problem:
+    v~ beta*mode + 1, w~ beta*mode - 1, and 1 for both at mode 0."""
+    if not cyl or name not in ("v", "w"):
+        return beta * mode
+    if mode == 0:
+        return 1.0
+    return beta * mode + 1.0 if name == "v" else beta * mode - 1.0
+
+
+def couple(V, W, direction, base_mode=0):
+    """AuxField::couple (src/auxfield.cpp:1376-1451) on [nz][n] Fourier
+    planes, returning new arrays: FORWARD (+1) v~ = v + i w, w~ = v - i w per
+    mode (Re planes 2k, Im 2k+1); INVERSE (-1) v = (v~ + w~)/2,
+    w = i (w~ - v~)/2.  Mode 0 of base mode 0 and nz = 1 stay unchanged."""
+    V, W = np.array(V, np.float64), np.array(W, np.float64)
+    nz = V.shape[0]
+    if nz < 2:
+        return V, W
+    for k in range(1 if base_mode == 0 else 0, nz // 2):
+        vr, vi, wr, wi = V[2 * k].copy(), V[2 * k + 1].copy(), W[2 * k].copy(), W[2 * k + 1].copy()
+        if direction > 0:                       # src/auxfield.cpp:1411-1424
+            V[2 * k], V[2 * k + 1] = vr - wi, vi + wr
+            W[2 * k], W[2 * k + 1] = vr + wi, vi - wr
+        else:                                   # src/auxfield.cpp:1428-1445
+            V[2 * k], V[2 * k + 1] = 0.5 * (vr + wr), 0.5 * (vi + wi)
+            W[2 * k], W[2 * k + 1] = 0.5 * (vi - wi), 0.5 * (wr - vr)
+    return V, W
+
+
+def solve_field_cyl(meshes, name, lambda2, beta, planes, lines=None, base_mode=0):
+    """Field::solve over a BoundarySys whose numbering depends on the mode
+    (src/field.cpp:648-700 with _bsys->Nsys(mode), src/bsys.cpp:42-123):
+    plane k of mode base_mode + k>>1 is solved on meshes[min(mode, 2)] with
+    betak2 = mode_constant(name, cyl, mode, beta)^2; the Nyquist plane 1 of
+    base mode 0 is skipped (left 0)."""
+    nz = planes.shape[0]
+    out = np.zeros_like(planes)
+    oms, systems = {}, {}
+    for k in range(nz):
+        if base_mode == 0 and nz > 1 and k == 1:
+            continue
+        mode = base_mode + (k >> 1)
+        cls = min(mode, 2)
+        m = meshes[cls]
+        if cls not in oms:
+            oms[cls] = OracleMesh(m)
+        if mode not in systems:
+            bk = mode_constant(name, m.cyl, mode, beta)
+            systems[mode] = OracleMsys(oms[cls], lambda2, bk * bk)
+        line = None if lines is None else lines[k]
+        out[k] = systems[mode].solve_plane(planes[k], line)
+    return out

@@ -15,6 +15,7 @@ SX_OK = 0
 SX_ERR_ARG, SX_ERR_HIP, SX_ERR_GEOM, SX_ERR_FACTOR, SX_ERR_LIMIT, SX_ERR_SYNC = -1, -2, -3, -4, -5, -6
 SX_FP64, SX_MIXED, SX_JACPCG = 0, 1, 2
 SX_SKEW, SX_ALTSKEW = 0, 1
+SX_FORWARD, SX_INVERSE = 1, -1
 SX_NOALLOC_DOUBLES = 10 * 1024 * 1024   # fixed workspace of sx_field_solve_mixed_noalloc
 
 # Every symbol include/semtex_amd.h declares, with (restype, argtypes).
@@ -28,7 +29,9 @@ SIGNATURES = {
     "sx_mesh_destroy": (_i, [_vp]),
     "sx_mesh_nclass": (_i, [_vp]),
     "sx_msys_create": (_i, [_vp, _d, _d, _i, _i, _i, _vp, C.POINTER(_vp)]),
+    "sx_msys_create_named": (_i, [_vp, _d, _d, _i, _i, C.c_char, _i, _vp, C.POINTER(_vp)]),
     "sx_msys_destroy": (_i, [_vp]),
+    "sx_couple": (_i, [_i, _i, _i, _dp, _dp, _i, _i, _vp]),
     "sx_msys_nsolve": (_i, [_vp, _i]),
     "sx_msys_layout": (_i, [_vp, C.POINTER(C.c_int), _i]),
     "sx_msys_status": (_i, [_vp, _vp]),

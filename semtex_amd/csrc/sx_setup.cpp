@@ -240,6 +240,11 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
     set_error ("sx_mesh_create: bad arguments (np must be 3..16)");
     return SX_ERR_ARG;
   }
+  for (int b = 0; b < nbound; b++)
+    if (bnd_kind[b] != SX_ESSENTIAL && bnd_kind[b] != SX_NATURAL && bnd_kind[b] != SX_AXIS) {
+      set_error ("sx_mesh_create: boundary kind must be SX_ESSENTIAL, SX_NATURAL or SX_AXIS");
+      return SX_ERR_ARG;
+    }
   sx_mesh* m = new sx_mesh;
   *out = nullptr;
   m->np = np; m->nel = nel; m->cyl = cyl ? 1 : 0;
@@ -358,7 +363,8 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       const int loc = (k < nm) ? s * nm + k : ((s == 3) ? 0 : s * nm + nm);
       const int g = bm[loc];
       if (bnd_kind[b] == SX_ESSENTIAL) ess[g] = b * np + k;
-      else nat[g].push_back ({b * np + k, area[k]});
+      else if (bnd_kind[b] == SX_NATURAL) nat[g].push_back ({b * np + k, area[k]});
+      // SX_AXIS: natural with no flux, no contribution
     }
   }
   std::vector<int> nptr (nglobal + 1, 0), nidx;

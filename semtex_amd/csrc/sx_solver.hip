@@ -1047,12 +1047,33 @@ __global__ void k_to_f32 (const double* a, float* b, size_t n)
 
 namespace { bool tiled_ok (const sx_mesh* m); bool tiled_mfma (const sx_mesh* m); }
 
+// The Fourier constant of a field's mode (Field::modeConstant,
+// src/field.cpp:1204-1229): beta k, except for the radial and azimuthal
+// velocity of a cylindrical mesh, whose coupled components v~ = v + i w and
+// w~ = v - i w solve with beta k + 1 and beta k - 1 (1 for mode 0, where
+// v and w are uncoupled).
+static double mode_constant (int cyl, char name, int mode, double beta)
+{
+  if (!cyl || (name != 'v' && name != 'w')) return beta * mode;
+  if (mode == 0) return 1.0;
+  return name == 'v' ? beta * mode + 1.0 : beta * mode - 1.0;
+}
+
 extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
                                int base_mode, int num_modes, int precision,
                                void* stream, sx_msys_t* out)
 {
+  return sx_msys_create_named (m, lambda2, beta, base_mode, num_modes, 'c', precision, stream,
+                               out);
+}
+
+extern "C" int sx_msys_create_named (sx_mesh_t m, double lambda2, double beta,
+                                     int base_mode, int num_modes, char name, int precision,
+                                     void* stream, sx_msys_t* out)
+{
   if (!m || !out || num_modes < 1 || base_mode < 0 ||
-      (precision != SX_FP64 && precision != SX_MIXED && precision != SX_JACPCG)) {
+      (precision != SX_FP64 && precision != SX_MIXED && precision != SX_JACPCG) ||
+      !(name == 'u' || name == 'v' || name == 'w' || name == 'p' || name == 'c')) {
     set_error ("sx_msys_create: bad arguments");
     return SX_ERR_ARG;
   }
@@ -1064,7 +1085,7 @@ extern "C" int sx_msys_create (sx_mesh_t m, double lambda2, double beta,
   const int next = m->next, nint = m->nint, npnp = m->npnp, ld = m->nband;
   const bool fixed = m->nglobal > m->nsolve;
   for (int md = 0; md < num_modes; md++) {
-    const double bk = beta * (base_mode + md);
+    const double bk = mode_constant (m->cyl, name, base_mode + md, beta);
     const double bk2 = bk * bk;
     s->betak2.push_back (bk2);
     const bool singular = (lambda2 + bk2) < EPSSP && !fixed;

@@ -26,7 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-ESSENTIAL, NATURAL = 0, 1
+ESSENTIAL, NATURAL, AXIS = 0, 1, 2
 
 
 def gll(np_: int):
@@ -227,7 +227,8 @@ def rect_mesh(nx: int, ny: int, np_: int, x0=0.0, x1=1.0, y0=0.0, y1=1.0,
     """nx x ny grid of straight-sided quadrilateral elements.
 
     bc gives the boundary kind on (bottom y=y0, right x=x1, top y=y1,
-    left x=x0): "D" essential, "N" natural, "A" axis (natural, no data).
+    left x=x0): "D" essential, "N" natural, "A" axis (natural with no
+    flux: kind AXIS, a boundary segment with a place in the line data).
     A periodic direction ignores its two entries.  xs/ys optionally give the
     element break points (length nx+1 / ny+1) for graded meshes.
     Elements are numbered with x fastest; every element's vertices are
@@ -293,9 +294,7 @@ def _finish(np_, X, Y, nx, ny, box, bc, periodic_x, periodic_y, cyl, ordering):
                 (ex == 0 and not periodic_x, bc[3]))):
             if not on:
                 continue
-            if kind == "A":
-                continue
-            k = ESSENTIAL if kind == "D" else NATURAL
+            k = {"D": ESSENTIAL, "N": NATURAL, "A": AXIS}[kind]
             bnd_e.append(e)
             bnd_s.append(side)
             bnd_k.append(k)

@@ -110,3 +110,28 @@ def uneven_batch(count=64, seed=SEED):
         out.append(dict(mesh=m, lambda2=lam2, forcing=f.reshape(1, -1), cyl=cyl,
                         np=np_, nx=nx, ny=ny, aspect=a))
     return out
+
+
+# axis condition per field and mode class (BCmgr::getCondition,
+# src/bcmgr.cpp:478-530): "A" natural, "D" essential (zero on the axis)
+AXIS_KIND = {"u": ("A", "D", "D"), "p": ("A", "D", "D"), "c": ("A", "D", "D"),
+             "v": ("D", "D", "D"), "w": ("D", "A", "D")}
+
+
+def cylindrical_meshes(name, nx=4, ny=3, np_=8, length=2.0, radius=1.0, wall="D",
+                       ordering="auto"):
+    """The numbering of field `name` of a cylindrical 3D problem for mode 0,
+    mode 1 and modes >= 2 (a BoundarySys's three NumberSys,
+    src/bsys.cpp:42-123): nx x ny elements on [0, length] x [0, radius],
+    axis on y = 0 with the mode-dependent axis condition, `wall` on
+    y = radius, periodic in x.  Classes that share an axis kind share one
+    mesh object."""
+    made = {}
+    out = []
+    for kind in AXIS_KIND[name]:
+        if kind not in made:
+            made[kind] = rect_mesh(nx, ny, np_, 0.0, length, 0.0, radius,
+                                   bc=(kind, "N", wall, "N"), periodic_x=True, cyl=1,
+                                   ordering=ordering)
+        out.append(made[kind])
+    return out

@@ -67,16 +67,17 @@ class ModalMatrixSys:
     the GPU.  mode k uses betak2 = (beta * (base_mode + k))^2."""
 
     def __init__(self, lambda2, beta, base_mode, num_modes, gmesh: GpuMesh,
-                 precision=SX_FP64):
+                 precision=SX_FP64, name="c"):
         L = _lib.load()
         self.gmesh = gmesh
         self.lambda2, self.beta = float(lambda2), float(beta)
         self.base_mode, self.num_modes = int(base_mode), int(num_modes)
-        self.precision = precision
+        self.precision, self.name = precision, name
         h = C.c_void_p()
-        _lib.check(L.sx_msys_create(gmesh.handle, self.lambda2, self.beta,
-                                    self.base_mode, self.num_modes, precision,
-                                    _stream(), C.byref(h)), "sx_msys_create")
+        _lib.check(L.sx_msys_create_named(gmesh.handle, self.lambda2, self.beta,
+                                          self.base_mode, self.num_modes, name.encode(),
+                                          precision, _stream(), C.byref(h)),
+                   "sx_msys_create_named")
         self.handle = h
 
     def nsolve(self, mode):
@@ -175,6 +176,61 @@ def fourier_transform(x, sign, out=None):
                                                 C.c_void_p(y.data_ptr()), int(sign), _stream()),
                "sx_fourier_transform")
     return y
+
+
+SX_FORWARD, SX_INVERSE = _lib.SX_FORWARD, _lib.SX_INVERSE
+
+
+def couple(v, w, direction, n=None, base_mode=0):
+    """AuxField::couple (src/auxfield.cpp:1376-1451) in place on two
+    [nz][stride] CUDA float64 tensors (Field::coupleBCs FORWARD on boundary
+    lines, n = stride): FORWARD v~ = v + i w, w~ = v - i w per mode; INVERSE
+    back.  Mode 0 of base mode 0 is not coupled."""
+    import torch
+    assert v.dtype == torch.float64 and v.is_cuda and v.is_contiguous() and v.shape == w.shape
+    n = v.shape[1] if n is None else n
+    _lib.check(_lib.load().sx_couple(v.shape[0], v.shape[1], n, C.c_void_p(v.data_ptr()),
+                                     C.c_void_p(w.data_ptr()), int(direction), int(base_mode),
+                                     _stream()), "sx_couple")
+    return v, w
+
+
+class CylModalSys:
+    """ModalMatrixSys of one field of a cylindrical 3D problem over its
+    BoundarySys (src/matrix.cpp:33-105, src/bsys.cpp:42-123): the axis
+    condition changes with the mode (BCmgr::getCondition,
+    src/bcmgr.cpp:478-530), so `meshes` holds the numbering of mode 0,
+    mode 1 and modes >= 2 (the same mesh object may repeat) and one system
+    is built per class with the field's Fourier constant (Field::modeConstant:
+    beta k, or beta k +- 1 for the coupled v~, w~).  solve() hands each class
+    its planes ([0,2), [2,4), [4,nz)) and boundary lines."""
+
+    def __init__(self, name, lambda2, beta, num_modes, meshes, precision=SX_FP64):
+        assert len(meshes) == 3 and num_modes >= 1
+        self.name, self.num_modes = name, int(num_modes)
+        self.parts = []                   # (first plane, planes, system, mesh)
+        for base, count in ((0, 1), (1, 1), (2, num_modes - 2)):
+            if count <= 0 or base >= num_modes:
+                continue
+            gm = meshes[base]
+            M = ModalMatrixSys(lambda2, beta, base, count, gm, precision, name=name)
+            self.parts.append((2 * base, 2 * count, M, gm))
+
+    def solve(self, f, u_data, lines=None):
+        """Field::solve for the whole field: f, u_data [nz][stride] CUDA
+        float64 (f destroyed); lines [nz][nbound*np] or None."""
+        L = _lib.load()
+        nz, S = f.shape
+        for k0, npl, M, gm in self.parts:
+            if k0 >= nz:
+                continue
+            npl = min(npl, nz - k0)
+            lp = (C.c_void_p(lines[k0].data_ptr())
+                  if lines is not None and gm.mesh.nbound else None)
+            _lib.check(L.sx_field_solve(M.handle, npl, S, C.c_void_p(f[k0].data_ptr()),
+                                        C.c_void_p(u_data[k0].data_ptr()), lp, _stream()),
+                       "sx_field_solve")
+        return u_data
 
 
 SX_SKEW, SX_ALTSKEW = _lib.SX_SKEW, _lib.SX_ALTSKEW

@@ -56,7 +56,7 @@ def dense_global_solve(mesh, hcon, f, line=None):
             vals = line[bnd * np_:(bnd + 1) * np_]
             if mesh.bnd_kind[bnd] == 0:
                 uess[g] = vals
-            else:
+            elif mesh.bnd_kind[bnd] == 1:
                 idx = side_node_index(np_, s)
                 xs, ys = mesh.x[e, idx], mesh.y[e, idx]
                 # straight side: |dx/dr| constant = length/2
