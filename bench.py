@@ -286,26 +286,28 @@ def dns_roofline(dn):
 
 
 def dns_parity(dn, torch):
-    """Per-step parity of the GPU DNS with the oracle, per field:
-    (1) steps 1-2 against the oracle run here, every Fourier plane (Nyquist
-    excluded), relative L2; (2) steps 1-100 against the committed oracle
-    fixtures at 256 sampled (plane, node) positions, max |error| over max
-    |value|.  Each is set beside the floor the fixtures record: the same
-    measure between the oracle and an independently ordered FP64 variant
-    of it (the singular mode-0 pressure system, condition ~2.6e5, makes two
-    valid FP64 solves differ by ~2e-11 relative)."""
+    """Per-step parity of the GPU DNS with the oracle, the measures of
+    tests/test_gpu_dns_channel_full.py: (1) steps 1-2 against the oracle
+    run here, every Fourier plane (Nyquist excluded), relative L2 per field,
+    beside the floor the fixture records for that step (the oracle against
+    an independently ordered FP64 solve of the singular mode-0 pressure,
+    condition ~2.6e5); (2) steps 1-100 against the committed oracle fixture
+    at 256 sampled (plane, node) positions: the velocity components'
+    max |error| over the velocity scale (max |u, v, w|), the pressure's over
+    max |p| beside 10x the running floor."""
     from oracle.dns_oracle import DnsOracle
     P, D = dn["P"], dn["D"]
     mu, nz = P["mesh_u"], P["nz"]
     n = mu.nel * mu.npnp
     pl = [k for k in range(nz) if k != 1]
     out = {}
+    F = np.load(FIXTURE) if os.path.exists(FIXTURE) else None
     O = DnsOracle(mu, P["mesh_p"], nz, P["beta"], P["kinvis"], P["dt"],
                   const_force=P["const_force"], advection="skew")
     O.set_initial(P["u0"])
     D.set_state(dn["u0"])
     live = {f: [] for f in FIELDS}
-    for _ in range(2):
+    for s in range(2):
         O.step()
         D.step(1)
         for i, f in enumerate(FIELDS):
@@ -313,39 +315,68 @@ def dns_parity(dn, torch):
             o = (O.u[i] if i < 3 else O.p)[pl]
             live[f].append(float(np.linalg.norm(g - o) / np.linalg.norm(o)))
     out["full_field_rel_l2_steps_1_2"] = live
-    if os.path.exists(FIXTURE):
-        F = np.load(FIXTURE)
+    if F is not None:
+        out["floor_rel_l2_steps_1_2"] = {f: [float(x) for x in F["floors"][:2, i]]
+                                         for i, f in enumerate(FIELDS)}
         steps = int(F["steps"])
-        sp, sn = F["sample_planes"], F["sample_nodes"]
+        sp = torch.from_numpy(F["sample_planes"]).cuda()
+        sn = torch.from_numpy(F["sample_nodes"]).cuda()
         D.set_state(dn["u0"])
-        err = np.zeros((steps, 4))
-        flo = np.zeros((steps, 4))
+        verr = np.zeros(steps)
+        perr = np.zeros(steps)
+        pflo = np.zeros(steps)
         for s in range(steps):
             D.step(1)
-            for i in range(4):
-                g = D.field(i).cpu().numpy()[sp, sn]
-                o = F["samples"][s, i]
-                scale = np.max(np.abs(o))
-                err[s, i] = np.max(np.abs(g - o)) / scale
-                flo[s, i] = np.max(np.abs(F["vsamples"][s, i] - o)) / scale
+            g = [D.field(i)[sp, sn].cpu().numpy() for i in range(4)]
+            o = F["samples"][s]
+            vs = max(np.max(np.abs(o[i])) for i in range(3))
+            verr[s] = max(np.max(np.abs(g[i] - o[i])) for i in range(3)) / vs
+            ps = np.max(np.abs(o[3]))
+            perr[s] = np.max(np.abs(g[3] - o[3])) / ps
+            pflo[s] = np.max(np.abs(F["vsamples"][s, 3] - o[3])) / ps
+        run_floor = np.maximum.accumulate(pflo)
         out["sampled_steps"] = steps
-        out["sampled_max_err"] = {f: float(err[:, i].max()) for i, f in enumerate(FIELDS)}
-        out["sampled_floor_max"] = {f: float(flo[:, i].max()) for i, f in enumerate(FIELDS)}
-        out["fixture_floor_l2_max"] = {f: float(F["floors"][:, i].max())
-                                       for i, f in enumerate(FIELDS)}
+        out["velocity_err_over_velocity_scale"] = {"step_1": float(verr[0]),
+                                                   "step_10": float(verr[9]),
+                                                   "step_100": float(verr[-1]),
+                                                   "max": float(verr.max())}
+        out["pressure_err"] = {"step_1": float(perr[0]), "step_100": float(perr[-1]),
+                               "max": float(perr.max())}
+        out["pressure_floor_running_max_step_100"] = float(run_floor[-1])
+        out["pressure_err_over_10x_floor_max"] = float(np.max(perr / (10 * run_floor)))
     return out
 
 
 def batch_leg(a, torch):
     """Uneven batch: 64 independent 2D problems (N_P 4..12, 4x4..20x20
     elements, aspect 1..4, lambda2 1e-2..1e2, half cylindrical), each solved
-    once per run by one captured graph; FP64 and mixed (3 passes)."""
+    once per run by one captured graph of fused launches (every kernel
+    instance once per phase for all problems), the sweep choice per
+    band-system class set by the library's autotuner (sx_batch_tune); FP64
+    and mixed (one FP64 refinement pass), both against the oracle; the
+    graph of per-problem launches on 4 lane streams timed beside it."""
+    from oracle import oracle as O
     from semtex_amd.problems import uneven_batch
     from semtex_amd.solver import SX_FP64, SX_MIXED, Batch, Field, GpuMesh, ModalMatrixSys
     probs = uneven_batch()
     t0 = time.perf_counter()
     res = {}
-    sols = {}
+    refs = []
+    for p in probs:
+        m = p["mesh"]
+        refs.append(O.solve_modes(m, p["lambda2"], 0.0, p["forcing"][:, :m.nel * m.npnp])[0])
+
+    def timed(B):
+        B.run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(a.batch_runs):
+            B.run()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / a.batch_runs
+
     for name, prec in (("fp64", SX_FP64), ("mixed", SX_MIXED)):
         items = []
         for p in probs:
@@ -356,29 +387,31 @@ def batch_leg(a, torch):
             src = torch.zeros((1, m.plane_size), dtype=torch.float64, device="cuda")
             src[0, :m.nel * m.npnp] = torch.from_numpy(p["forcing"][0])
             items.append((M, u, src, gm))
-        B = Batch([(M, u, src) for (M, u, src, _) in items], refine_passes=2)
-        torch.cuda.synchronize()
-        B.run()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(a.batch_runs):
-            B.run()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / a.batch_runs
-        sols[name] = [u.data[0, :p["mesh"].nel * p["mesh"].npnp].clone()
-                      for (p, (_, u, _, _)) in zip(probs, items)]
-        res[name] = {"ms_per_run": round(ms, 4), "problems_per_s": round(len(probs) / (ms / 1e3), 1)}
-        del B, items
-    err = max(float(torch.linalg.norm(x - y) / torch.linalg.norm(y))
-              for x, y in zip(sols["mixed"], sols["fp64"]))
-    res["mixed"]["max_rel_err_vs_fp64"] = err
+        entries = [(M, u, src) for (M, u, src, _) in items]
+        B = Batch(entries, refine_passes=2)
+        ms_untuned = timed(B)
+        tuned = B.tune(reps=5)
+        ms = timed(B)
+        err = max(float(np.linalg.norm(u.data[0, :p["mesh"].nel * p["mesh"].npnp].cpu().numpy()
+                                       - r) / np.linalg.norm(r))
+                  for (p, (_, u, _, _), r) in zip(probs, items, refs))
+        res[name] = {"ms_per_run": round(ms, 4), "problems_per_s": round(len(probs) / (ms / 1e3), 1),
+                     "ms_untuned": round(ms_untuned, 4), "max_rel_err_vs_oracle": err,
+                     "plan": tuned}
+        del B
+        if name == "fp64":
+            os.environ["SX_BATCH_FUSED"] = "0"
+            B0 = Batch(entries, refine_passes=2)
+            del os.environ["SX_BATCH_FUSED"]
+            res["fp64_lane_graph_ms"] = round(timed(B0), 4)
+            del B0
+        del items, entries
     res["setup_s"] = round(time.perf_counter() - t0, 2)
     res["problems"] = len(probs)
     res["config"] = ("64 independent 2D Helmholtz problems, N_P 4..12, 4x4..20x20 elements, "
                      "aspect 1..4, lambda2 log-uniform 1e-2..1e2, N(0,1) forcing, half Cartesian "
-                     "(Dirichlet), half cylindrical axisymmetric (axis y=0); one HIP graph per run")
+                     "(Dirichlet), half cylindrical axisymmetric (axis y=0); one HIP graph per run "
+                     "(fused launches, autotuned sweep choice); mixed: one FP64 refinement pass")
     return res
 
 
@@ -590,7 +623,8 @@ def main():
     if not a.no_batch:
         out["batch"] = batch_leg(a, torch)
         print(f"# batch: {out['batch']['fp64']['problems_per_s']} problems/s FP64, "
-              f"{out['batch']['mixed']['problems_per_s']} mixed", file=sys.stderr, flush=True)
+              f"{out['batch']['mixed']['problems_per_s']} mixed; lane graph "
+              f"{out['batch']['fp64_lane_graph_ms']} ms", file=sys.stderr, flush=True)
 
     if not a.no_adjoint:
         out["adjoint"] = adjoint_leg(torch, P, gm)

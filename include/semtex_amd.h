@@ -192,7 +192,12 @@ int sx_field_solve_pcg (sx_msys_t s, int nz, int plane_stride, double* d_f, doub
 
 /* A batch of independent problems (each system with its own mesh, order,
  * lambda2, Cartesian or cylindrical geometry) solved by one captured HIP
- * graph.  For problem i the graph copies d_f_src[i] into d_f_work[i] (the
+ * graph.  By default the graph is fused: every kernel instance runs once
+ * per phase for all the problems that use it (a few dozen launches for
+ * the whole batch); SX_BATCH_FUSED=0 in the environment, or a system the
+ * fused plans do not cover (offset-major band sweeps for band offsets past
+ * 511, untiled element kernels for N_P > 12), records each problem's own
+ * launches instead.  For problem i the graph copies d_f_src[i] into d_f_work[i] (the
  * solve destroys its forcing) and runs the solve into d_u[i], with
  * d_lines[i] (d_lines or an entry may be NULL).  SX_MIXED systems run
  * refine_passes passes (the first solve plus refine_passes-1 FP64
@@ -212,6 +217,16 @@ int sx_batch_create (int n, const sx_msys_t* systems, const int* nz, const int* 
                      const double* const* d_lines, int refine_passes, sx_batch_t* out);
 int sx_batch_run (sx_batch_t b, void* stream);
 int sx_batch_destroy (sx_batch_t b);
+/* Autotuner of a fused batch: for every class of band system in it (off-
+ * diagonal blocks D, unit-diagonal panels, twisted layout) tries 1 and 2
+ * right-hand sides per band-sweep workgroup, times `reps` replays of the
+ * whole batch on `stream` for each, keeps what is fastest and re-records
+ * the graph with it.  No-op for the graph-of-launches form. */
+int sx_batch_tune (sx_batch_t b, int reps, void* stream);
+/* The batch's form and choices as text: "fused=1 launches=<merged launches
+ * in the graph>; D=.. unit=.. twisted=.. problems=.. nr=..; ..." (nr 0:
+ * the default choice). */
+int sx_batch_info (sx_batch_t b, char* buf, int len);
 
 /* Adjoint building blocks.  The DIRECT system is symmetric, so the adjoint
  * of Field::solve is a solve with a weak-form right-hand side:

@@ -50,6 +50,8 @@ SIGNATURES = {
     "sx_batch_create": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, C.POINTER(_vp)]),
     "sx_batch_run": (_i, [_vp, _vp]),
     "sx_batch_destroy": (_i, [_vp]),
+    "sx_batch_tune": (_i, [_vp, _i, _vp]),
+    "sx_batch_info": (_i, [_vp, C.c_char_p, _i]),
     "sx_mass_inner": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
     "sx_mass_weight": (_i, [_vp, _i, _i, _dp, _dp, _d, _vp]),
     "sx_field_solve_pcg": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i, C.POINTER(_i), _vp]),

@@ -495,7 +495,7 @@ __device__ __forceinline__ bool sweep_seg (SweepSeg& g, int NR, int D,
 }
 
 template <int NR, int D>
-__global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
+__device__ __forceinline__ void k_panel_sweeps_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
@@ -710,10 +710,22 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
         for (int r = 0; r < NR; r++) bg[(size_t) r * rs + gidx (K * PB + lane)] = r0[r] + r1[r];
       }
     }
-    // the next pass reads what this one wrote to global memory
-    __threadfence ();
+    // the next pass reads what this one wrote to global memory: written and
+    // read by waves of this workgroup only (one CU, one L1), so a
+    // workgroup-scope fence orders it; a device-scope fence would write back
+    // and invalidate the XCD's L2 under the other workgroups' panel streams
+    __threadfence_block ();
     __syncthreads ();
   }
+}
+
+template <int NR, int D>
+__global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
+{
+  k_panel_sweeps_body<NR, D> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, kd);
 }
 
 // Moves between the right-hand side (rows in global order) and the bottom
@@ -723,10 +735,10 @@ __global__ __launch_bounds__(panel_nw (D) * 64) void k_panel_sweeps (
 //   1  combine  b[m+a] += rb[nbot-1-a]    (S: the two pieces' residuals)
 //   2  x_S      rb[nbot-1-a] = b[m+a]
 //   3  scatter  b[n-1-(q-p)] = rb[q]      for B rows
-__global__ void k_twist_move (double* rhs, int nmax, double* rhsb, int nmaxb, int nr,
-                              const int* __restrict__ tw, int S, int what)
+__device__ __forceinline__ void k_twist_move_body (double* rhs, int nmax, double* rhsb,
+                                                   int nmaxb, int nr, const int* __restrict__ tw,
+                                                   int S, int what, int mode, int col)
 {
-  const int mode = blockIdx.z, col = blockIdx.y;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const int* t = tw + TW * mode;
   const int p = t[TW_PAD], m = t[TW_M], nbot = t[TW_NBOT], n = t[TW_N];
@@ -747,6 +759,12 @@ __global__ void k_twist_move (double* rhs, int nmax, double* rhsb, int nmaxb, in
     if (q >= p && q < p + nbr) b[n - 1 - (q - p)] = rb[q];
     break;
   }
+}
+
+__global__ void k_twist_move (double* rhs, int nmax, double* rhsb, int nmaxb, int nr,
+                              const int* __restrict__ tw, int S, int what)
+{
+  k_twist_move_body (rhs, nmax, rhsb, nmaxb, nr, tw, S, what, blockIdx.z, blockIdx.y);
 }
 
 // The bottom piece of a twisted solve: rows [m, n) of the band reversed,
@@ -858,7 +876,7 @@ constexpr int UW = 8;                               // wavefronts per workgroup
 constexpr int UPW = 4;                              // row pairs per wave and block
 
 template <int NR, int D>
-__global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
+__device__ __forceinline__ void k_unit_sweeps_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
@@ -1042,10 +1060,22 @@ __global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
         }
       }
     }
-    // the next pass reads what this one wrote to global memory
-    __threadfence ();
+    // the next pass reads what this one wrote to global memory: written and
+    // read by waves of this workgroup only (one CU, one L1), so a
+    // workgroup-scope fence orders it; a device-scope fence would write back
+    // and invalidate the XCD's L2 under the other workgroups' panel streams
+    __threadfence_block ();
     __syncthreads ();
   }
+}
+
+template <int NR, int D>
+__global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
+    const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
+    const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
+    const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
+{
+  k_unit_sweeps_body<NR, D> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, kd);
 }
 
 // ---------------------------------------------------------------------------
@@ -1117,20 +1147,47 @@ __device__ __forceinline__ void msg_wait (__amdgpu_buffer_rsrc_t rs, int off0, i
   for (int r = 0; r < NR; r++) v[r] = __builtin_nan ("");
 }
 
-template <int NR, int D>
+// Roles of a split chain (R workgroups): the near one (role 0) takes T_J
+// and W_{J,J+1..J+H}, H = ceil((D+1)/R) - 1 (at least 1); far role
+// r = 1..R-1 takes a contiguous range [a_r, b_r] of the remaining blocks
+// H+1..D, as even as they divide (the first ones one block more).
+template <int D, int R> __host__ __device__ constexpr int sp_h ()
+{
+  return (D + R) / R - 1 < 1 ? 1 : (D + R) / R - 1;
+}
+template <int D, int R> __host__ __device__ constexpr int sp_fbmax ()
+{
+  return (D - sp_h<D, R> () + R - 2) / (R - 1);
+}
+// first block and block count of far role r (1..R-1)
+template <int D, int R> __host__ __device__ constexpr int sp_a (int r)
+{
+  const int F = D - sp_h<D, R> (), base = F / (R - 1), extra = F % (R - 1);
+  int a = sp_h<D, R> () + 1;
+  for (int q = 1; q < r; q++) a += base + (q - 1 < extra ? 1 : 0);
+  return a;
+}
+template <int D, int R> __host__ __device__ constexpr int sp_n (int r)
+{
+  const int F = D - sp_h<D, R> (), base = F / (R - 1), extra = F % (R - 1);
+  return base + (r - 1 < extra ? 1 : 0);
+}
+
+template <int NR, int D, int R>
 __device__ __forceinline__ void split_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
     char* xch, unsigned ep, int spin, bool& dead)
 {
-  constexpr int H = D / 2, FB = D - H;              // near blocks 1..H, far H+1..D
-  constexpr int NA = UPW * (H + 1), NB = UPW * FB;  // pairs per wave
+  constexpr int H = sp_h<D, R> (), FBM = sp_fbmax<D, R> ();
+  constexpr int NA = UPW * (H + 1), NB = UPW * FBM;  // pairs per wave
   constexpr int NP = NA > NB ? NA : NB;
+  constexpr int NF = R - 1;                         // far roles
   constexpr int MS = PB * 16;                       // bytes per record row (64 lanes)
   extern __shared__ double sm[];
   SweepSeg g;
-  if (!sweep_seg (g, NR, D, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, 2))
+  if (!sweep_seg (g, NR, D, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, R))
     return;
   const int n = g.n, rs = g.rs, role = g.role;
   double* bg = g.bg;
@@ -1153,9 +1210,9 @@ __device__ __forceinline__ void split_body (
   const int w = __builtin_amdgcn_readfirstlane (threadIdx.x >> 6);
   const size_t PS = (size_t) (D + 1) * PB * PB;
   // this chain's rings: [dir][XR][NR] rows of 64 records; dir 0: z (near ->
-  // far), dir 1: p (far -> near)
+  // far roles), dir r: p of far role r (-> near)
   const __amdgpu_buffer_rsrc_t xrs = __builtin_amdgcn_make_buffer_rsrc (
-      xch + (size_t) g.chain * 2 * XR * NR * MS, (short) 0, 2 * XR * NR * MS, 0x00020000);
+      xch + (size_t) g.chain * R * XR * NR * MS, (short) 0, R * XR * NR * MS, 0x00020000);
   auto xoff = [&] (int dir, int K) __attribute__ ((always_inline)) {
     return ((dir * XR + (K & (XR - 1))) * NR) * MS + 16 * lane;
   };
@@ -1171,27 +1228,30 @@ __device__ __forceinline__ void split_body (
   // loads retire in order, so a wait for a load issued after them would
   // not).  Taken when their tags match, else the step polls.
   typedef int v4i __attribute__ ((ext_vector_type (4)));
-  v4i spec[NR];
+  v4i spec[NF][NR];
   int spec_k = -1;                                   // step whose records spec holds
-  auto spec_read = [&] (int dir, int K) __attribute__ ((always_inline)) {
+  auto spec_read = [&] (int dir0, int K) __attribute__ ((always_inline)) {
 #pragma unroll
-    for (int r = 0; r < NR; r++)
-      spec[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, xoff (dir, K) + r * MS, 0, 17);
+    for (int f = 0; f < NF; f++)
+#pragma unroll
+      for (int r = 0; r < NR; r++)
+        spec[f][r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, xoff (dir0 + f, K) + r * MS, 0, 17);
     spec_k = K;
   };
-  // this step's records: the speculative ones when all lanes carry the tag
-  auto take = [&] (int dir, int K, unsigned sq, double (&v)[NR]) __attribute__ ((always_inline)) {
+  // this step's records of direction dir0 + f: the speculative ones when
+  // all lanes carry the tag
+  auto take = [&] (int dir0, int f, int K, unsigned sq, double (&v)[NR]) __attribute__ ((always_inline)) {
     bool ok = spec_k == K && !dead;
     if (ok) {
       bool lane_ok = true;
 #pragma unroll
       for (int r = 0; r < NR; r++) {
-        v[r] = __longlong_as_double (((long long) spec[r].y << 32) | (unsigned) spec[r].x);
-        lane_ok = lane_ok && (unsigned) spec[r].z == ep && (unsigned) spec[r].w == sq;
+        v[r] = __longlong_as_double (((long long) spec[f][r].y << 32) | (unsigned) spec[f][r].x);
+        lane_ok = lane_ok && (unsigned) spec[f][r].z == ep && (unsigned) spec[f][r].w == sq;
       }
       ok = __builtin_amdgcn_ballot_w64 (!lane_ok) == 0;
     }
-    if (!ok) msg_wait<NR> (xrs, xoff (dir, K), MS, ep, sq, v, dead, spin);
+    if (!ok) msg_wait<NR> (xrs, xoff (dir0 + f, K), MS, ep, sq, v, dead, spin);
   };
 
   for (int pass = 0; pass < 2; pass++) {
@@ -1284,7 +1344,15 @@ __device__ __forceinline__ void split_body (
 
       auto step = [&] (int J, double2 (&u)[NP], double (&bq)[NR]) __attribute__ ((always_inline)) {
         double pv[NR], zs[NR][2 * UPW];
-        take (1, J, seq (pass, J), pv);
+#pragma unroll
+        for (int r = 0; r < NR; r++) pv[r] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          double pf[NR];
+          take (1, f, J, seq (pass, J), pf);
+#pragma unroll
+          for (int r = 0; r < NR; r++) pv[r] += pf[r];
+        }
         if (J + 1 < J1) spec_read (1, J + 1);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -1345,11 +1413,19 @@ __device__ __forceinline__ void split_body (
       }
       if (J < J1) step (J, ua, ba);
       put_y (J1 - 1);
-      // the far side's last records: p_{J1..J1+D-1}; with them, the window
+      // the far roles' last records: p_{J1..J1+D-1}; with them, the window
       // handed on (blocks that never entered the near ring keep b)
       for (int K = J1; K < J1 + D; K++) {
         double pv[NR];
-        msg_wait<NR> (xrs, xoff (1, K), MS, ep, seq (pass, K), pv, dead, spin);
+#pragma unroll
+        for (int r = 0; r < NR; r++) pv[r] = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; f++) {
+          double pf[NR];
+          msg_wait<NR> (xrs, xoff (1 + f, K), MS, ep, seq (pass, K), pf, dead, spin);
+#pragma unroll
+          for (int r = 0; r < NR; r++) pv[r] += pf[r];
+        }
         if (wout && w == K - J1 && K * PB + lane < n) {
 #pragma unroll
           for (int r = 0; r < NR; r++) {
@@ -1364,25 +1440,33 @@ __device__ __forceinline__ void split_body (
           }
         }
       }
-      // the far side has read every z of this pass: tell it the pass is over
+      // the far roles have read every z of this pass: tell them the pass is over
       if (w == 0)
 #pragma unroll
         for (int r = 0; r < NR; r++) msg_put (xrs, xoff (0, J1) + r * MS, 0.0, ep, seq (pass, J1));
     } else {
-      // ---------------- far side: W_{J,J+H+1..J+D} ----------------
-      constexpr int RE = FB + 1;                     // ring entries: blocks J+H..J+D
+      // ---------------- far role: W_{J,J+a..J+a+FB-1} ----------------
+      // a, FB from the role; register arrays and loads for FBM blocks (the
+      // loads past FB read nothing: out-of-range offsets), so every role
+      // issues the same instruction stream
+      const int a = R == 2 || role == 1 ? sp_a<D, R> (1) : role == 2 ? sp_a<D, R> (2)
+                                                                   : sp_a<D, R> (3);
+      const int FB = R == 2 || role == 1 ? sp_n<D, R> (1) : role == 2 ? sp_n<D, R> (2)
+                                                                    : sp_n<D, R> (3);
+      const int bl = a + FB - 1;                     // last block
+      const int RE = FB + 1;                         // ring entries: blocks J+a-1..J+bl
       double* ring = sm;                             // [RE][UW][64][NR]
       auto ring_at = [&] (int K, int s) __attribute__ ((always_inline)) {
         return ring + ((size_t) ((K % RE) * UW + s) * PB + lane) * NR;
       };
-      // p_K of a completed block, published by wave 0
+      // p_K of this role, published by wave 0
       auto put_p = [&] (int K) __attribute__ ((always_inline)) {
 #pragma unroll
         for (int r = 0; r < NR; r++) {
           double v = ring_at (K, 0)[r];
 #pragma unroll
           for (int s = 1; s < UW; s++) v += ring_at (K, s)[r];
-          msg_put (xrs, xoff (1, K) + r * MS, v, ep, seq (pass, K));
+          msg_put (xrs, xoff (role, K) + r * MS, v, ep, seq (pass, K));
         }
       };
       // the near side must be done with the previous pass's records
@@ -1391,70 +1475,76 @@ __device__ __forceinline__ void split_body (
         const int Jp = pass ? g.f1 : 0;
         msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (pass - 1, Jp), dv, dead, spin);
       }
-      // blocks J0+H+1..J0+D-1 start empty here (their earlier far terms are
-      // in the window the near side reads from b); p_K = 0 for K <= J0+H
+      // blocks J0+a..J0+bl-1 start empty here (their earlier terms are in
+      // the window the near side reads from b); p_K = 0 for K <= J0+a-1
       if (w < FB - 1) {
-        const int K = J0 + H + 1 + w;
+        const int K = J0 + a + w;
 #pragma unroll
         for (int s = 0; s < UW; s++)
 #pragma unroll
           for (int r = 0; r < NR; r++) ring_at (K, s)[r] = 0.0;
       }
       if (w == 0)
-        for (int K = J0; K <= J0 + H; K++)
+        for (int K = J0; K <= J0 + a - 1; K++)
 #pragma unroll
-          for (int r = 0; r < NR; r++) msg_put (xrs, xoff (1, K) + r * MS, 0.0, ep, seq (pass, K));
+          for (int r = 0; r < NR; r++) msg_put (xrs, xoff (role, K) + r * MS, 0.0, ep, seq (pass, K));
+      auto fvoff = [&] (int vb, int d, int k) __attribute__ ((always_inline)) {
+        return d < FB ? pvoff (vb, a + d, k) : OOB;
+      };
       {
         const int v0 = vbase (J0), v1 = vbase (J0 + 1);
 #pragma unroll
         for (int k = 0; k < UPW; k++)
 #pragma unroll
-          for (int d = 0; d < FB; d++)
-            ua[k * FB + d] = pld (pvoff (v0, H + 1 + d, k), 32768 * (H + 1 + d) + 1024 * k);
+          for (int d = 0; d < FBM; d++)
+            ua[k * FBM + d] = pld (fvoff (v0, d, k), 32768 * (a + d) + 1024 * k);
 #pragma unroll
         for (int k = 0; k < UPW; k++)
 #pragma unroll
-          for (int d = 0; d < FB; d++)
-            ub[k * FB + d] = pld (pvoff (v1, H + 1 + d, k), 32768 * (H + 1 + d) + 1024 * k);
+          for (int d = 0; d < FBM; d++)
+            ub[k * FBM + d] = pld (fvoff (v1, d, k), 32768 * (a + d) + 1024 * k);
       }
       __builtin_amdgcn_s_waitcnt (0);
       __syncthreads ();
 
       auto step = [&] (int J, double2 (&u)[NP]) __attribute__ ((always_inline)) {
-        if (J > J0 && w == 0) put_p (J + H);
+        if (J > J0 && w == 0) put_p (J + a - 1);
         double zv[NR], zs[NR][2 * UPW];
-        take (0, J, seq (pass, J), zv);
+        take (0, 0, J, seq (pass, J), zv);
         if (J + 1 < J1) spec_read (0, J + 1);
 #pragma unroll
         for (int r = 0; r < NR; r++)
 #pragma unroll
           for (int j = 0; j < 2 * UPW; j++) zs[r][j] = lane_bcast (zv[r], 8 * w + j);
-        double acc[FB][NR];
+        double acc[FBM][NR];
 #pragma unroll
-        for (int d = 0; d < FB; d++)
+        for (int d = 0; d < FBM; d++)
 #pragma unroll
           for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
         const int vb = vbase (J + 2);
 #pragma unroll
         for (int k = 0; k < UPW; k++) {
 #pragma unroll
-          for (int d = 0; d < FB; d++) {
-            const double2 v = u[k * FB + d];
+          for (int d = 0; d < FBM; d++) {
+            const double2 v = u[k * FBM + d];
 #pragma unroll
             for (int r = 0; r < NR; r++)
               acc[d][r] = fma (v.y, zs[r][2 * k + 1], fma (v.x, zs[r][2 * k], acc[d][r]));
-            int so = 32768 * (H + 1 + d) + 1024 * k;
+            int so = 32768 * (a + d) + 1024 * k;
 #pragma unroll
             for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(acc[d][r]));
-            u[k * FB + d] = pld (pvoff (vb, H + 1 + d, k), so);
+            u[k * FBM + d] = pld (fvoff (vb, d, k), so);
           }
         }
 #pragma unroll
-        for (int d = 0; d < FB - 1; d++)
+        for (int d = 0; d < FBM; d++)
+          if (d < FB - 1) {
 #pragma unroll
-          for (int r = 0; r < NR; r++) ring_at (J + H + 1 + d, w)[r] -= acc[d][r];
+            for (int r = 0; r < NR; r++) ring_at (J + a + d, w)[r] -= acc[d][r];
+          } else if (d == FB - 1) {
 #pragma unroll
-        for (int r = 0; r < NR; r++) ring_at (J + D, w)[r] = -acc[FB - 1][r];
+            for (int r = 0; r < NR; r++) ring_at (J + bl, w)[r] = -acc[d][r];
+          }
         lds_sync ();
       };
       int J = J0;
@@ -1464,12 +1554,18 @@ __device__ __forceinline__ void split_body (
       }
       if (J < J1) step (J, ua);
       // the block completed by the last step and the partial sums still
-      // pending: p_{J1+H..J1+D-1}
-      if (w == 0)
-        for (int K = J1 + H; K < J1 + D; K++) put_p (K);
+      // pending: p_{J1+a-1..J1+bl-1}; past its last block this role adds
+      // nothing: zero records up to J1+D-1
+      if (w == 0) {
+        for (int K = J1 + a - 1; K < J1 + bl; K++) put_p (K);
+        for (int K = J1 + bl; K < J1 + D; K++)
+#pragma unroll
+          for (int r = 0; r < NR; r++) msg_put (xrs, xoff (role, K) + r * MS, 0.0, ep, seq (pass, K));
+      }
       ran = true;
     }
-    __threadfence ();
+    // the next pass reads what this workgroup wrote (workgroup scope, as above)
+    __threadfence_block ();
     __syncthreads ();
   }
 }
@@ -1481,26 +1577,30 @@ __device__ __forceinline__ void split_body (
 // always use one epoch (also when one of them starts only after its partner
 // has given up and finished), and a record written under an earlier launch
 // of any layout never carries the current epoch.
-template <int NR, int D>
+template <int NR, int D, int R>
 __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd,
     char* xch, unsigned* sync, int spin)
 {
-  const unsigned ep = __hip_atomic_load (sync, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  // relaxed throughout: successive launches are ordered by the kernel
+  // boundary, and within a launch every workgroup has used its epoch long
+  // before it counts itself finished (an agent-scope acquire or release here
+  // would make every workgroup invalidate or write back its L2 at its end)
+  const unsigned ep = __hip_atomic_load (sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   bool dead = false;
-  split_body<NR, D> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
+  split_body<NR, D, R> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
                      nmaxb, kd, xch, ep, spin, dead);
   const int any_dead = __syncthreads_or (dead ? 1 : 0);
   if (threadIdx.x == 0) {
     if (any_dead)
       __hip_atomic_fetch_or (sync + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned done = __hip_atomic_fetch_add (sync + 1, 1u, __ATOMIC_ACQ_REL,
+    const unsigned done = __hip_atomic_fetch_add (sync + 1, 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
     if (done == gridDim.x - 1) {
       __hip_atomic_store (sync + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store (sync, ep + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store (sync, ep + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }
@@ -1570,26 +1670,31 @@ int split_spin ()
   return e ? std::max (-1, std::atoi (e)) : XSPIN;
 }
 
-template <int NR, int D>
+template <int NR, int D, int R>
 size_t split_lds ()
 {
-  constexpr int H = D / 2, FB = D - H;
-  return sizeof (double) * (size_t) std::max (H + 3, FB + 1) * UW * PB * NR;
+  constexpr int H = sp_h<D, R> (), FBM = sp_fbmax<D, R> ();
+  return sizeof (double) * (size_t) std::max (H + 3, FBM + 1) * UW * PB * NR;
 }
 
-// workgroups of k_split_sweeps<NR, D> the device holds at once
-template <int NR, int D>
+// workgroups of k_split_sweeps<NR, D> the device holds at once; 0 when
+// the kernel uses scratch (private memory): the occupancy query does not
+// count the runtime's limit on waves holding scratch at once, so a grid it
+// reports as fitting could leave a partner workgroup waiting for scratch
+template <int NR, int D, int R>
 int split_capacity ()
 {
   static int cap = -1;
   if (cap < 0) {
     int dev = 0, cus = 0, per = 0;
-    auto kern = k_split_sweeps<NR, D>;
-    const int lds = (int) split_lds<NR, D> ();
+    auto kern = k_split_sweeps<NR, D, R>;
+    const int lds = (int) split_lds<NR, D, R> ();
+    hipFuncAttributes fa;
     if (hipGetDevice (&dev) != hipSuccess ||
         hipDeviceGetAttribute (&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         hipFuncSetAttribute ((const void*) kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                              lds) != hipSuccess ||
+        hipFuncGetAttributes (&fa, (const void*) kern) != hipSuccess || fa.localSizeBytes != 0 ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor (&per, (const void*) kern, UW * 64,
                                                       (size_t) lds) != hipSuccess)
       cap = 0;
@@ -1599,14 +1704,14 @@ int split_capacity ()
   return cap;
 }
 
-template <int NR, int D>
+template <int NR, int D, int R>
 int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool probe)
 {
   const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
-  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * 2 * pieces;
-  if (probe) return nblocks <= split_capacity<NR, D> () ? SX_OK : SX_ERR_LIMIT;
-  const size_t lds = split_lds<NR, D> ();
-  auto kern = k_split_sweeps<NR, D>;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * R * pieces;
+  if (probe) return nblocks <= split_capacity<NR, D, R> () ? SX_OK : SX_ERR_LIMIT;
+  const size_t lds = split_lds<NR, D, R> ();
+  auto kern = k_split_sweeps<NR, D, R>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
@@ -1617,14 +1722,20 @@ int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool pro
   return SX_OK;
 }
 
+// workgroups per chain of the split sweeps (near and far).  split_body
+// takes the role count as a parameter; three roles (the far blocks on two
+// workgroups) let waits run out on the 32-mode Helmholtz system and are not
+// instantiated.
+constexpr int SPLIT_ROLES = 2;
+
 // probe: SX_OK iff every workgroup of the launch fits on the device at once
 template <int NR>
 int launch_split (const sx_msys* s, int G, int stage, hipStream_t st, bool probe = false)
 {
   switch (s->sw_D) {
-  case 3: return launch_split_d<NR, 3> (s, G, stage, st, probe);
-  case 4: return launch_split_d<NR, 4> (s, G, stage, st, probe);
-  case 5: return launch_split_d<NR, 5> (s, G, stage, st, probe);
+  case 3: return launch_split_d<NR, 3, SPLIT_ROLES> (s, G, stage, st, probe);
+  case 4: return launch_split_d<NR, 4, SPLIT_ROLES> (s, G, stage, st, probe);
+  case 5: return launch_split_d<NR, 5, SPLIT_ROLES> (s, G, stage, st, probe);
   default: break;
   }
   set_error ("split panel sweep: unsupported block count");
@@ -1638,7 +1749,7 @@ constexpr int XMAXR = 6;
 int split_alloc (sx_msys* s, hipStream_t st)
 {
   const size_t rows = (size_t) 2 * s->nmodes * XMAXR;
-  s->xch_bytes = rows * 2 * XR * PB * 16;
+  s->xch_bytes = rows * SPLIT_ROLES * XR * PB * 16;  // one direction per role
   SX_HIP (hipMalloc (&s->d_xch, s->xch_bytes));
   SX_HIP (hipMalloc (&s->d_xep, sizeof (unsigned) * 4));
   SX_HIP (hipMemsetAsync (s->d_xch, 0, s->xch_bytes, st));
@@ -1959,7 +2070,7 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
           const int fits = c == 1 ? launch_split<1> (s, nr, stg, st, true)
                          : c == 2 ? launch_split<2> (s, nr / 2, stg, st, true)
                                   : launch_split<3> (s, nr / 3, stg, st, true);
-          const int dist = std::abs (slots1 * (nr / c) * 2 - 128);
+          const int dist = std::abs (slots1 * (nr / c) * SPLIT_ROLES - 128);
           if (fits == SX_OK && dist <= best) { best = dist; sper = c; }
         }
       }
@@ -2035,6 +2146,119 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
   set_error ("band solver: band width " + std::to_string (s->m->nband) +
              " outside the instantiated range (nband <= 768)");
   return SX_ERR_LIMIT;
+}
+
+// ---------------------------------------------------------------------------
+// Fused-batch plan of band_solve without the split sweeps (the batch's
+// solves run side by side, so no launch may need its whole grid resident)
+// ---------------------------------------------------------------------------
+namespace {
+template <int NR, int D> struct FnUnit {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_unit_sweeps_body<NR, D> (a...); }
+};
+template <int NR, int D> struct FnPanel {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_panel_sweeps_body<NR, D> (a...); }
+};
+// blockIdx.y = mode * nr + column (k_twist_move has them in z and y)
+struct FnTwistMove {
+  __device__ __forceinline__ void operator() (double* rhs, int nmax, double* rhsb, int nmaxb,
+                                              int nr, const int* tw, int S, int what) const
+  { k_twist_move_body (rhs, nmax, rhsb, nmaxb, nr, tw, S, what, blockIdx.y / nr,
+                       blockIdx.y % nr); }
+};
+
+template <int NR, int D>
+void plan_unit_d (BatchPlan& P, const sx_msys* s, int G, int stage)
+{
+  const size_t lds = sizeof (double) * ((size_t) (D + 3) * UW * PB * NR);
+  const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
+  batch_rec<UW * 64, FnUnit<NR, D>> (P, dim3 (nblocks), UW * 64, lds,
+                                     (const double*) s->d_swF, (const double*) s->d_swR,
+                                     s->sw_stride, (const int*) s->d_nsolve, s->d_rhs, s->nmax,
+                                     s->nmodes, G, (const int*) s->d_tw, stage, s->tw_boff,
+                                     s->d_rhsb, s->nmaxb, s->m->nband - 1);
+}
+
+template <int NR, int D>
+void plan_panel_d (BatchPlan& P, const sx_msys* s, int G, int stage)
+{
+  constexpr int NW = PanelSweep<NR, D>::NW;
+  const size_t lds = sizeof (double) * ((size_t) PB * NR * (2 * D + NW));
+  const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
+  const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
+  batch_rec<panel_nw (D) * 64, FnPanel<NR, D>> (P, dim3 (nblocks), NW * 64, lds,
+                                               (const double*) s->d_swF,
+                                               (const double*) s->d_swR, s->sw_stride,
+                                               (const int*) s->d_nsolve, s->d_rhs, s->nmax,
+                                               s->nmodes, G, (const int*) s->d_tw, stage,
+                                               s->tw_boff, s->d_rhsb, s->nmaxb,
+                                               s->m->nband - 1);
+}
+
+template <int NR>
+int plan_sweep (BatchPlan& P, const sx_msys* s, int G, int stage)
+{
+  if (s->sw_unit) {
+    switch (s->sw_D) {
+    case 3: plan_unit_d<NR, 3> (P, s, G, stage); return SX_OK;
+    case 4: plan_unit_d<NR, 4> (P, s, G, stage); return SX_OK;
+    case 5: plan_unit_d<NR, 5> (P, s, G, stage); return SX_OK;
+    default: return SX_ERR_LIMIT;
+    }
+  }
+  switch (s->sw_D) {
+  case 1: plan_panel_d<NR, 1> (P, s, G, stage); return SX_OK;
+  case 2: plan_panel_d<NR, 2> (P, s, G, stage); return SX_OK;
+  case 3: plan_panel_d<NR, 3> (P, s, G, stage); return SX_OK;
+  case 4: plan_panel_d<NR, 4> (P, s, G, stage); return SX_OK;
+  case 5: plan_panel_d<NR, 5> (P, s, G, stage); return SX_OK;
+  case 6: plan_panel_d<NR, 6> (P, s, G, stage); return SX_OK;
+  case 7: plan_panel_d<NR, 7> (P, s, G, stage); return SX_OK;
+  case 8: plan_panel_d<NR, 8> (P, s, G, stage); return SX_OK;
+  default: return SX_ERR_LIMIT;
+  }
+}
+}  // namespace
+
+// The right-hand sides per sweep workgroup: P.tune when it divides nr and
+// the kernel takes it, else band_solve's default (the fewest that keep one
+// workgroup per CU, at most 2 for the unit-diagonal sweeps).
+int plan_band_solve (BatchPlan& P, const sx_msys* s, int nr)
+{
+  if (!s->sw_panel || nr != 2) return SX_ERR_LIMIT;
+  const int pieces = s->tw ? 2 : 1, slots = (s->nmodes + 7) / 8 * 8 * pieces;
+  int per = nr;
+  for (int c : {1, 2})
+    if (nr % c == 0 && slots * (nr / c) <= 256) { per = c; break; }
+  if (P.tune == 1 || P.tune == 2) per = P.tune;
+  const int G = nr / per;
+  auto sweep = [&] (int stage) {
+    return per == 1 ? plan_sweep<1> (P, s, G, stage) : plan_sweep<2> (P, s, G, stage);
+  };
+  auto move = [&] (int what) {
+    const int rows = (what == 1 || what == 2) ? s->tw_S : s->nmaxb;
+    batch_rec<256, FnTwistMove> (P, dim3 ((rows + 255) / 256, nr * s->nmodes), 256, 0,
+                                 s->d_rhs, s->nmax, s->d_rhsb, s->nmaxb, nr,
+                                 (const int*) s->d_tw, s->tw_S, what);
+    P.phase++;
+  };
+  int rc;
+  if (!s->tw) {
+    if ((rc = sweep (0))) return rc;
+    P.phase++;
+    return SX_OK;
+  }
+  move (0);
+  for (int stage = 1; stage <= 3; stage++) {
+    if ((rc = sweep (stage))) return rc;
+    P.phase++;
+    if (stage < 3) move (stage);
+  }
+  move (3);
+  return SX_OK;
 }
 
 // status word of the split sweeps: SX_OK, or SX_ERR_SYNC (and the word

@@ -189,3 +189,136 @@ struct sx_msys {
   double* d_pc = nullptr;      // SX_JACPCG: [nmodes][npts] inverse assembled diagonal
   size_t work_doubles = 0;
 };
+
+// ---------------------------------------------------------------------------
+// Fused batches (sx_batch.hip).  A batch of independent solves is planned
+// instead of launched: every launch of every solve is recorded with its
+// phase (its place in that solve's dependency order) and merged with the
+// launches of the other solves that run the same kernel in the same phase.
+// A merged launch runs k_batch: blockIdx.z picks the solve, whose recorded
+// arguments (a Pack) and grid it reads; workgroups past that solve's own
+// grid return at once, and the kernel body sees its usual blockIdx.x/.y.
+// ---------------------------------------------------------------------------
+#include <cstring>
+#include <utility>
+
+namespace sx {
+
+template <typename... A> struct Pack;
+template <> struct Pack<> {};
+template <typename T, typename... R> struct Pack<T, R...> {
+  T h;
+  Pack<R...> t;
+};
+
+// The host writes Packs that the device reads: both compilation passes
+// must lay them out alike (checked here for the member kinds used).
+static_assert (sizeof (Pack<int, const int*, double>) == 32 &&
+               sizeof (Pack<double*, size_t, int, int>) == 32 &&
+               sizeof (Pack<int, double, const double*>) == 32 &&
+               sizeof (Pack<int, int>) == 12 && alignof (Pack<int, double*>) == 8,
+               "argument packs must have one layout on host and device");
+
+inline Pack<> make_pack () { return {}; }
+template <typename T, typename... R> Pack<T, R...> make_pack (T h, R... r)
+{
+  return Pack<T, R...> {h, make_pack (r...)};
+}
+
+template <int I, typename T, typename... R>
+__host__ __device__ __forceinline__ const auto& pack_get (const Pack<T, R...>& p)
+{
+  if constexpr (I == 0) return p.h;
+  else return pack_get<I - 1> (p.t);
+}
+
+template <typename Fn, typename... A, int... I>
+__device__ __forceinline__ void pack_call (const Pack<A...>& p, std::integer_sequence<int, I...>)
+{
+  Fn {} (pack_get<I> (p)...);
+}
+
+template <int LB, typename Fn, typename... A>
+__global__ __launch_bounds__(LB) void k_batch (const Pack<A...>* __restrict__ args,
+                                               const uint2* __restrict__ grid)
+{
+  const uint2 g = grid[blockIdx.z];
+  if (blockIdx.x >= g.x || blockIdx.y >= g.y) return;
+  pack_call<Fn> (args[blockIdx.z], std::make_integer_sequence<int, sizeof... (A)> {});
+}
+
+typedef int (*BatchLauncher) (const void* d_args, const uint2* d_grid, int n, unsigned gx,
+                              unsigned gy, int block, size_t lds, hipStream_t st);
+
+template <int LB, typename Fn, typename... A>
+int batch_launcher (const void* d_args, const uint2* d_grid, int n, unsigned gx, unsigned gy,
+                    int block, size_t lds, hipStream_t st)
+{
+  auto k = k_batch<LB, Fn, A...>;
+  if (lds > 0)
+    SX_HIP (hipFuncSetAttribute ((const void*) k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int) lds));
+  hipLaunchKernelGGL (k, dim3 (gx, gy, (unsigned) n), dim3 (block), lds, st,
+                      (const Pack<A...>*) d_args, d_grid);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// The launches of a batch, merged by (phase, kernel instance).
+struct BatchPlan {
+  struct Group {
+    int phase = 0;
+    BatchLauncher launch = nullptr;
+    int block = 0;
+    size_t lds = 0, psize = 0;
+    std::vector<char> blob;          // the solves' argument Packs, psize bytes each
+    std::vector<uint2> grid;         // the solves' own grids
+    unsigned gx = 0, gy = 0;         // their maximum
+    void* d_args = nullptr;
+    uint2* d_grid = nullptr;
+  };
+  std::vector<Group> groups;
+  int phase = 0;                     // the phase the next recorded launch belongs to
+  int tune = 0;                      // right-hand sides per sweep workgroup (0: default)
+};
+
+template <int LB, typename Fn, typename... A>
+void batch_rec (BatchPlan& P, dim3 grid, int block, size_t lds, A... a)
+{
+  const BatchLauncher L = &batch_launcher<LB, Fn, A...>;
+  const Pack<A...> pk = make_pack (a...);
+  BatchPlan::Group* g = nullptr;
+  for (auto& x : P.groups)
+    if (x.phase == P.phase && x.launch == L && x.block == block) { g = &x; break; }
+  if (!g) {
+    P.groups.emplace_back ();
+    g = &P.groups.back ();
+    g->phase = P.phase;
+    g->launch = L;
+    g->block = block;
+    g->psize = sizeof (pk);
+  }
+  const size_t o = g->blob.size ();
+  g->blob.resize (o + sizeof (pk));
+  std::memcpy (g->blob.data () + o, &pk, sizeof (pk));
+  g->grid.push_back (make_uint2 (grid.x, grid.y));
+  g->gx = std::max (g->gx, grid.x);
+  g->gy = std::max (g->gy, grid.y);
+  g->lds = std::max (g->lds, lds);
+}
+
+// Plans of the solve paths (each starts at P.phase and leaves P.phase at
+// the phase after its last launch); SX_ERR_LIMIT when a system needs a
+// kernel the fused batch does not merge (the caller then records launches).
+int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride, double* d_f,
+                             double* d_u, const double* d_lines);
+int plan_band_solve (BatchPlan& P, const sx_msys* s, int nr);
+int plan_mixed_passes (BatchPlan& P, sx_msys* s, int nz, int stride, double* d_f,
+                       double* d_u, const double* d_lines, int passes);
+int plan_copy (BatchPlan& P, double* dst, const double* src, size_t n);
+int plan_condense32_tiled (BatchPlan& P, const sx_msys* s, int nz, int stride, const double* X,
+                           double* Fp);
+int plan_recover32_tiled (BatchPlan& P, const sx_msys* s, int nz, int stride, const double* Fp,
+                          double* V);
+
+}  // namespace sx

@@ -68,7 +68,7 @@ __device__ void helm_op_elem (int np, const double* D, const double* geo,
 }
 
 // W = -(M f + H g) (Field::constrain, src/field.cpp:812-852), row-major.
-__global__ __launch_bounds__(64) void k_weak (
+__device__ __forceinline__ void k_weak_body (
     int np, int cyl, double lambda2, const double* __restrict__ betak2,
     const double* __restrict__ geo, const double* __restrict__ Dg,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -97,8 +97,18 @@ __global__ __launch_bounds__(64) void k_weak (
   for (int i = lane; i < npnp; i += 64) W[off + i] = -buf[i];
 }
 
+__global__ __launch_bounds__(64) void k_weak (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ emask, const double* __restrict__ G, int nglobal,
+    const double* __restrict__ F, double* W, int stride, int nz, int base_mode)
+{
+  k_weak_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, emask, G, nglobal, F, W, stride, nz, base_mode);
+}
+
 // R = W - H v (FP64), row-major.
-__global__ __launch_bounds__(64) void k_resid (
+__device__ __forceinline__ void k_resid_body (
     int np, int cyl, double lambda2, const double* __restrict__ betak2,
     const double* __restrict__ geo, const double* __restrict__ Dg,
     const double* __restrict__ W, const double* __restrict__ V, double* R_out,
@@ -115,9 +125,18 @@ __global__ __launch_bounds__(64) void k_resid (
   for (int i = lane; i < npnp; i += 64) R_out[off + i] = W[off + i] - out[i];
 }
 
+__global__ __launch_bounds__(64) void k_resid (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const double* __restrict__ W, const double* __restrict__ V, double* R_out,
+    int stride, int nz, int base_mode)
+{
+  k_resid_body (np, cyl, lambda2, betak2, geo, Dg, W, V, R_out, stride, nz, base_mode);
+}
+
 // Inner solve, part 1: permute to emap order, condense the boundary
 // forcing with the FP32 hbi hii^-1.  Fp gets [F_b condensed | F_i].
-__global__ __launch_bounds__(64) void k_condense32 (
+__device__ __forceinline__ void k_condense32_body (
     int np, int nclass, const int* __restrict__ emap_g, const int* __restrict__ cls,
     const float* __restrict__ Bm, const double* __restrict__ X, double* Fp,
     int stride, int nz, int base_mode)
@@ -140,9 +159,17 @@ __global__ __launch_bounds__(64) void k_condense32 (
   for (int a = next + lane; a < npnp; a += 64) Fp[off + a] = fp[a];
 }
 
+__global__ __launch_bounds__(64) void k_condense32 (
+    int np, int nclass, const int* __restrict__ emap_g, const int* __restrict__ cls,
+    const float* __restrict__ Bm, const double* __restrict__ X, double* Fp,
+    int stride, int nz, int base_mode)
+{
+  k_condense32_body (np, nclass, emap_g, cls, Bm, X, Fp, stride, nz, base_mode);
+}
+
 // Inner solve, part 3: FP32 interior recovery into row-major V (no
 // essential insertion: essential boundary values are zero here).
-__global__ __launch_bounds__(64) void k_recover32 (
+__device__ __forceinline__ void k_recover32_body (
     int np, int nclass, const int* __restrict__ emap_g, const int* __restrict__ cls,
     const int* __restrict__ btog, const float* __restrict__ Bm,
     const float* __restrict__ Hinv, const double* __restrict__ Fp, double* V,
@@ -180,7 +207,17 @@ __global__ __launch_bounds__(64) void k_recover32 (
   for (int i = lane; i < next; i += 64) V[off + emap[i]] = ubd[i];
 }
 
-__global__ void k_axpy_planes (double* V, const double* dV, int npl, int nz,
+__global__ __launch_bounds__(64) void k_recover32 (
+    int np, int nclass, const int* __restrict__ emap_g, const int* __restrict__ cls,
+    const int* __restrict__ btog, const float* __restrict__ Bm,
+    const float* __restrict__ Hinv, const double* __restrict__ Fp, double* V,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs,
+    int nmax, int nz, int base_mode)
+{
+  k_recover32_body (np, nclass, emap_g, cls, btog, Bm, Hinv, Fp, V, stride, nsolve, rhs, nmax, nz, base_mode);
+}
+
+__device__ __forceinline__ void k_axpy_planes_body (double* V, const double* dV, int npl, int nz,
                                int base_mode, int n_plane, int stride)
 {
   const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,6 +225,12 @@ __global__ void k_axpy_planes (double* V, const double* dV, int npl, int nz,
   const int y = (int) (idx / n_plane), j = (int) (idx % n_plane);
   const int k = solved_plane (y, nz, base_mode);
   V[(size_t) k * stride + j] += dV[(size_t) k * stride + j];
+}
+
+__global__ void k_axpy_planes (double* V, const double* dV, int npl, int nz,
+                               int base_mode, int n_plane, int stride)
+{
+  k_axpy_planes_body (V, dV, npl, nz, base_mode, n_plane, stride);
 }
 
 // sum of squares of V and of dV over the solved planes (partials per block)
@@ -216,7 +259,7 @@ __global__ __launch_bounds__(256) void k_norms (const double* V, const double* d
 }
 
 // u = V with the essential values of G inserted (Field::setEssential).
-__global__ __launch_bounds__(64) void k_finish (
+__device__ __forceinline__ void k_finish_body (
     int np, const int* __restrict__ emap_g, const int* __restrict__ btog,
     const int* __restrict__ bmask, const int* __restrict__ emask,
     const double* __restrict__ G, int nglobal, const double* __restrict__ V,
@@ -233,7 +276,16 @@ __global__ __launch_bounds__(64) void k_finish (
       U[off + emap_g[i]] = G[(size_t) k * nglobal + btog[(size_t) e * next + i]];
 }
 
-__global__ void k_essential_m (int nglobal, const int* __restrict__ ess,
+__global__ __launch_bounds__(64) void k_finish (
+    int np, const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ bmask, const int* __restrict__ emask,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ V,
+    double* U, int stride, int nz, int base_mode)
+{
+  k_finish_body (np, emap_g, btog, bmask, emask, G, nglobal, V, U, stride, nz, base_mode);
+}
+
+__device__ __forceinline__ void k_essential_m_body (int nglobal, const int* __restrict__ ess,
                                const double* __restrict__ lines, int lstride,
                                double* G, int nz, int base_mode)
 {
@@ -244,7 +296,14 @@ __global__ void k_essential_m (int nglobal, const int* __restrict__ ess,
   G[(size_t) k * nglobal + g] = (s >= 0 && lines) ? lines[(size_t) k * lstride + s] : 0.0;
 }
 
-__global__ void k_assemble_m (int nglobal, const int* __restrict__ gptr,
+__global__ void k_essential_m (int nglobal, const int* __restrict__ ess,
+                               const double* __restrict__ lines, int lstride,
+                               double* G, int nz, int base_mode)
+{
+  k_essential_m_body (nglobal, ess, lines, lstride, G, nz, base_mode);
+}
+
+__device__ __forceinline__ void k_assemble_m_body (int nglobal, const int* __restrict__ gptr,
                               const int* __restrict__ gsrc, const int* __restrict__ nptr,
                               const int* __restrict__ nidx, const double* __restrict__ nw,
                               const double* __restrict__ F, int stride,
@@ -262,6 +321,17 @@ __global__ void k_assemble_m (int nglobal, const int* __restrict__ gptr,
   if (lines)
     for (int c = nptr[g]; c < nptr[g + 1]; c++) s += lines[(size_t) k * lstride + nidx[c]] * nw[c];
   rhs[((size_t) mode * 2 + rr) * nmax + g] = s;
+}
+
+__global__ void k_assemble_m (int nglobal, const int* __restrict__ gptr,
+                              const int* __restrict__ gsrc, const int* __restrict__ nptr,
+                              const int* __restrict__ nidx, const double* __restrict__ nw,
+                              const double* __restrict__ F, int stride,
+                              const double* __restrict__ lines, int lstride,
+                              const int* __restrict__ nsolve, double* rhs, int nmax,
+                              int nz, int base_mode)
+{
+  k_assemble_m_body (nglobal, gptr, gsrc, nptr, nidx, nw, F, stride, lines, lstride, nsolve, rhs, nmax, nz, base_mode);
 }
 
 int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
@@ -297,6 +367,113 @@ int inner_solve (sx_msys* s, int nz, int stride, const double* X, double* Fp,
 }  // namespace
 
 namespace sx {
+
+}  // namespace sx
+
+namespace {
+#define SX_BODY_FN(Name, body)                                                   \
+  struct Name {                                                                 \
+    template <typename... A> __device__ __forceinline__ void operator() (A... a) const \
+    { body (a...); }                                                            \
+  };
+SX_BODY_FN (FnEssentialM, k_essential_m_body)
+SX_BODY_FN (FnWeak, k_weak_body)
+SX_BODY_FN (FnAssembleM, k_assemble_m_body)
+SX_BODY_FN (FnResid, k_resid_body)
+SX_BODY_FN (FnAxpyPlanes, k_axpy_planes_body)
+SX_BODY_FN (FnFinish, k_finish_body)
+SX_BODY_FN (FnCondense32, k_condense32_body)
+SX_BODY_FN (FnRecover32, k_recover32_body)
+#undef SX_BODY_FN
+
+// the inner solve of the mixed passes, recorded (FP32 MFMA tiles only)
+int plan_inner (BatchPlan& P, sx_msys* s, int nz, int stride, const double* X, double* Fp,
+                double* V, const double* lines)
+{
+  const sx_mesh* m = s->m;
+  const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
+  const int lstride = m->nbound * m->np;
+  const bool tiled = s->d_Mcon32 && s->d_Mrec32;
+  if (tiled) {
+    if (int rc = plan_condense32_tiled (P, s, nz, stride, X, Fp)) return rc;
+  } else {
+    batch_rec<64, FnCondense32> (P, dim3 (m->nel, npl), 64, 0, m->np, m->nclass,
+                                 (const int*) m->d_emap, (const int*) m->d_cls,
+                                 (const float*) s->d_Bm32, X, Fp, stride, nz, s->base_mode);
+    P.phase++;
+  }
+  batch_rec<256, FnAssembleM> (P, dim3 ((m->nglobal + 255) / 256, npl), 256, 0, m->nglobal,
+                               (const int*) m->d_gptr, (const int*) m->d_gsrc,
+                               (const int*) m->d_nptr, (const int*) m->d_nidx,
+                               (const double*) m->d_nw, (const double*) Fp, stride, lines,
+                               lstride, (const int*) s->d_nsolve, s->d_rhs, s->nmax, nz,
+                               s->base_mode);
+  P.phase++;
+  if (int rc = plan_band_solve (P, s, 2)) return rc;
+  if (tiled) return plan_recover32_tiled (P, s, nz, stride, Fp, V);
+  batch_rec<64, FnRecover32> (P, dim3 (m->nel, npl), 64, 0, m->np, m->nclass,
+                              (const int*) m->d_emap, (const int*) m->d_cls,
+                              (const int*) m->d_btog, (const float*) s->d_Bm32,
+                              (const float*) s->d_Hinv32, (const double*) Fp, V, stride,
+                              (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz,
+                              s->base_mode);
+  P.phase++;
+  return SX_OK;
+}
+}  // namespace
+
+namespace sx {
+
+// field_solve_mixed_passes, recorded for a fused batch; the system's
+// workspace is allocated here (planning is set-up, not the replayed work)
+int plan_mixed_passes (BatchPlan& P, sx_msys* s, int nz, int stride, double* d_f, double* d_u,
+                       const double* d_lines, int passes)
+{
+  const sx_mesh* m = s->m;
+  const int npl = (s->base_mode == 0 && nz > 1) ? nz - 1 : nz;
+  const size_t nplane = (size_t) nz * stride;
+  const size_t need = mixed_work_doubles (s, nz, stride);
+  if (s->work_doubles < need) {
+    if (s->d_work) (void) hipFree (s->d_work);
+    s->d_work = nullptr;
+    s->work_doubles = 0;
+    SX_HIP (hipMalloc (&s->d_work, sizeof (double) * need));
+    s->work_doubles = need;
+  }
+  double *W = s->d_work, *X = W + nplane, *Fp = X + nplane, *V = Fp + nplane,
+         *dV = V + nplane;
+  const int lstride = m->nbound * m->np;
+  const double* lines = (d_lines && m->nbound) ? d_lines : (const double*) nullptr;
+  batch_rec<256, FnEssentialM> (P, dim3 ((m->nglobal + 255) / 256, npl), 256, 0, m->nglobal,
+                                (const int*) m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+  P.phase++;
+  batch_rec<64, FnWeak> (P, dim3 (m->nel, npl), 64, 0, m->np, m->cyl, s->lambda2,
+                         (const double*) s->d_betak2, (const double*) m->d_geo,
+                         (const double*) m->d_D, (const int*) m->d_emap, (const int*) m->d_btog,
+                         (const int*) m->d_emask, (const double*) s->d_G, m->nglobal,
+                         (const double*) d_f, W, stride, nz, s->base_mode);
+  P.phase++;
+  if (int rc = plan_inner (P, s, nz, stride, W, Fp, V, lines)) return rc;
+  const int n_plane = m->nel * m->npnp;
+  for (int it = 1; it < passes; it++) {
+    batch_rec<64, FnResid> (P, dim3 (m->nel, npl), 64, 0, m->np, m->cyl, s->lambda2,
+                            (const double*) s->d_betak2, (const double*) m->d_geo,
+                            (const double*) m->d_D, (const double*) W, (const double*) V, X,
+                            stride, nz, s->base_mode);
+    P.phase++;
+    if (int rc = plan_inner (P, s, nz, stride, X, Fp, dV, lines)) return rc;
+    batch_rec<256, FnAxpyPlanes> (P, dim3 ((unsigned) (((size_t) npl * n_plane + 255) / 256)),
+                                  256, 0, V, (const double*) dV, npl, nz, s->base_mode, n_plane,
+                                  stride);
+    P.phase++;
+  }
+  batch_rec<64, FnFinish> (P, dim3 (m->nel, npl), 64, 0, m->np, (const int*) m->d_emap,
+                           (const int*) m->d_btog, (const int*) m->d_bmask,
+                           (const int*) m->d_emask, (const double*) s->d_G, m->nglobal,
+                           (const double*) V, d_u, stride, nz, s->base_mode);
+  P.phase++;
+  return SX_OK;
+}
 
 // The mixed solve with a fixed number of refinement passes and no host
 // synchronisation (graph capture): the first pass is the FP32-condensed

@@ -206,7 +206,7 @@ __global__ void k_band_assemble (int npairs, const long long* __restrict__ ppos,
 // ---------------------------------------------------------------------------
 // Solve path
 // ---------------------------------------------------------------------------
-__global__ void k_essential (int nglobal, const int* __restrict__ ess,
+__device__ __forceinline__ void k_essential_body (int nglobal, const int* __restrict__ ess,
                              const double* __restrict__ lines, int lstride,
                              double* G, int nz, int base_mode)
 {
@@ -215,6 +215,13 @@ __global__ void k_essential (int nglobal, const int* __restrict__ ess,
   if (g >= nglobal) return;
   const int s = ess[g];
   G[(size_t) k * nglobal + g] = (s >= 0 && lines) ? lines[(size_t) k * lstride + s] : 0.0;
+}
+
+__global__ void k_essential (int nglobal, const int* __restrict__ ess,
+                             const double* __restrict__ lines, int lstride,
+                             double* G, int nz, int base_mode)
+{
+  k_essential_body (nglobal, ess, lines, lstride, G, nz, base_mode);
 }
 
 // Elemental Helmholtz operator out = H src on one element by one
@@ -498,7 +505,7 @@ __device__ __forceinline__ int mode_planes (int m, int nz, int base_mode, int* k
 
 // Constrain on the elements with essential nodes (Field::constrain,
 // src/field.cpp:812-852): f <- -(M f + H g), row-major, in place.
-__global__ __launch_bounds__(64) void k_weak_ess (
+__device__ __forceinline__ void k_weak_ess_body (
     int np, int cyl, double lambda2, const double* __restrict__ betak2,
     const double* __restrict__ geo, const double* __restrict__ Dg,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -521,10 +528,20 @@ __global__ __launch_bounds__(64) void k_weak_ess (
   for (int i = lane; i < npnp; i += 64) f[i] = -(f[i] * ge[G_Q8 * npnp + i] + Hg[i]);
 }
 
+__global__ __launch_bounds__(64) void k_weak_ess (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ ess_elem, const double* __restrict__ G, int nglobal,
+    double* F, int stride, int nz, int base_mode)
+{
+  k_weak_ess_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, ess_elem, G, nglobal, F, stride, nz, base_mode);
+}
+
 // f (row-major; already weak on essential elements, else -M f is formed
 // here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i.
 template <bool MF>
-__global__ __launch_bounds__(CT_THREADS) void k_condense_t (
+__device__ __forceinline__ void k_condense_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
     const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
@@ -599,12 +616,22 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
   }
 }
 
+template <bool MF>
+__global__ __launch_bounds__(CT_THREADS) void k_condense_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ emask,
+    const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
+    int nz, int base_mode, int weak)
+{
+  k_condense_t_body<MF> (np, nclass, tiles, elist, emap_g, emask, geo, Mcon, F, stride, nz, base_mode, weak);
+}
+
 // u_i = Hinv^T F_i - Bm^T u_b with u_b from the band solution, then the
 // essential values (Element::global2localSC src/element.cpp:368-407,
 // Field::setEssential src/field.cpp:1105-1124).  One operand
 // M[k][a] = (k < next ? -Bm[k][a] : Hinv[k-next][a]) against [u_b | F_i].
 template <bool MF>
-__global__ __launch_bounds__(CT_THREADS) void k_recover_t (
+__device__ __forceinline__ void k_recover_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
     const int* __restrict__ bmask, const int* __restrict__ emask,
@@ -720,12 +747,25 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
   }
 }
 
+template <bool MF>
+__global__ __launch_bounds__(CT_THREADS) void k_recover_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ bmask, const int* __restrict__ emask,
+    const double* __restrict__ Mrec,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
+    int nz, int base_mode, int rnr, int roff)
+{
+  k_recover_t_body<MF> (np, nclass, tiles, elist, emap_g, btog, bmask, emask, Mrec, G, nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+}
+
 // FP32 forms of the tiled condensation and recovery for the mixed solve's
 // inner passes (FP32 operands and MFMA tiles, FP32 accumulation; the
 // boundary values stay FP64).
 // Condensation: X (row-major, already weak) -> Fp = [w_b - Bm w_i | w_i] in
 // emap order.
-__global__ __launch_bounds__(CT_THREADS) void k_condense32_t (
+__device__ __forceinline__ void k_condense32_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const float* __restrict__ Mcon,
     const double* __restrict__ X, double* Fp, int stride, int nz, int base_mode)
@@ -779,10 +819,18 @@ __global__ __launch_bounds__(CT_THREADS) void k_condense32_t (
   }
 }
 
+__global__ __launch_bounds__(CT_THREADS) void k_condense32_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const float* __restrict__ Mcon,
+    const double* __restrict__ X, double* Fp, int stride, int nz, int base_mode)
+{
+  k_condense32_t_body (np, nclass, tiles, elist, emap_g, Mcon, X, Fp, stride, nz, base_mode);
+}
+
 // Recovery: V (row-major) = [u_b | Hinv^T F_i - Bm^T u_b] with u_b from the
 // band solution (no essential insertion: the correction's essential values
 // are zero).
-__global__ __launch_bounds__(CT_THREADS) void k_recover32_t (
+__device__ __forceinline__ void k_recover32_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
     const float* __restrict__ Mrec, const double* __restrict__ Fp, double* V, int stride,
@@ -851,6 +899,16 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover32_t (
   }
 }
 
+__global__ __launch_bounds__(CT_THREADS) void k_recover32_t (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const float* __restrict__ Mrec, const double* __restrict__ Fp, double* V, int stride,
+    const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax, int nz,
+    int base_mode)
+{
+  k_recover32_t_body (np, nclass, tiles, elist, emap_g, btog, Mrec, Fp, V, stride, nsolve, rhs, nmax, nz, base_mode);
+}
+
 // FP32 copies of the MFMA-layout operands (mixed systems)
 __global__ void k_ops32 (const double* __restrict__ a, float* b, size_t n)
 {
@@ -859,7 +917,7 @@ __global__ void k_ops32 (const double* __restrict__ a, float* b, size_t n)
     b[i] = (float) a[i];
 }
 
-__global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
+__device__ __forceinline__ void k_assemble_body (int nglobal, const int* __restrict__ gptr,
                             const int* __restrict__ gsrc,
                             const int* __restrict__ nptr,
                             const int* __restrict__ nidx,
@@ -880,6 +938,19 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
     for (int c = nptr[g]; c < nptr[g + 1]; c++)
       s += lines[(size_t) k * lstride + nidx[c]] * nw[c];
   rhs[((size_t) mode * rnr + rr) * nmax + g] = s;
+}
+
+__global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
+                            const int* __restrict__ gsrc,
+                            const int* __restrict__ nptr,
+                            const int* __restrict__ nidx,
+                            const double* __restrict__ nw,
+                            const double* __restrict__ F, int stride,
+                            const double* __restrict__ lines, int lstride,
+                            const int* __restrict__ nsolve, double* rhs,
+                            int nmax, int nz, int base_mode, int rnr, int roff)
+{
+  k_assemble_body (nglobal, gptr, gsrc, nptr, nidx, nw, F, stride, lines, lstride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
 }
 
 // One wavefront per (element, solved plane): interior recovery and
@@ -1434,6 +1505,144 @@ int recover32_tiled (const sx_msys* s, int nz, int stride, const double* Fp, dou
                       s->d_Mrec32, Fp, V, stride, s->d_nsolve, s->d_rhs, s->nmax, nz,
                       s->base_mode);
   SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fused-batch plans of the element phases (the launches of
+// field_solve_direct, condense32_tiled and recover32_tiled, recorded)
+// ---------------------------------------------------------------------------
+}  // namespace sx
+
+namespace {
+struct FnEssential {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_essential_body (a...); }
+};
+struct FnWeakEss {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_weak_ess_body (a...); }
+};
+struct FnAssemble {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_assemble_body (a...); }
+};
+template <bool MF> struct FnCondenseT {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_condense_t_body<MF> (a...); }
+};
+template <bool MF> struct FnRecoverT {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_recover_t_body<MF> (a...); }
+};
+struct FnCondense32T {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_condense32_t_body (a...); }
+};
+struct FnRecover32T {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_recover32_t_body (a...); }
+};
+}  // namespace
+
+namespace sx {
+
+int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride, double* d_f,
+                             double* d_u, const double* d_lines)
+{
+  const sx_mesh* m = s->m;
+  if (!tiled_ok (m) || !s->d_Mcon || !s->d_Mrec || s->precision != SX_FP64) return SX_ERR_LIMIT;
+  const int npl = n_solved_planes (nz, s->base_mode);
+  const int lstride = m->nbound * m->np;
+  const double* lines = (d_lines && m->nbound) ? d_lines : (const double*) nullptr;
+  const unsigned gb = (unsigned) ((m->nglobal + 255) / 256);
+  batch_rec<256, FnEssential> (P, dim3 (gb, npl), 256, 0, m->nglobal, (const int*) m->d_ess,
+                               lines, lstride, s->d_G, nz, s->base_mode);
+  P.phase++;
+  if (m->ness) {
+    batch_rec<64, FnWeakEss> (P, dim3 (m->ness, npl), 64, 0, m->np, m->cyl, s->lambda2,
+                              (const double*) s->d_betak2, (const double*) m->d_geo,
+                              (const double*) m->d_D, (const int*) m->d_emap,
+                              (const int*) m->d_btog, (const int*) m->d_ess_elem,
+                              (const double*) s->d_G, m->nglobal, d_f, stride, nz,
+                              s->base_mode);
+    P.phase++;
+  }
+  const dim3 tg (m->ntile, n_mode_blocks (nz));
+  const int weak = 0;
+  if (s->tile_mf)
+    batch_rec<CT_THREADS, FnCondenseT<true>> (
+        P, tg, CT_THREADS, tiled_lds (m, false, true), m->np, m->nclass, (const int*) m->d_tile,
+        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_emask,
+        (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride, nz, s->base_mode, weak);
+  else
+    batch_rec<CT_THREADS, FnCondenseT<false>> (
+        P, tg, CT_THREADS, tiled_lds (m, false, false), m->np, m->nclass,
+        (const int*) m->d_tile, (const int*) m->d_elist, (const int*) m->d_emap,
+        (const int*) m->d_emask, (const double*) m->d_geo, (const double*) s->d_Mcon, d_f,
+        stride, nz, s->base_mode, weak);
+  P.phase++;
+  const int rnr = 2, roff = 0;
+  batch_rec<256, FnAssemble> (P, dim3 (gb, npl), 256, 0, m->nglobal, (const int*) m->d_gptr,
+                              (const int*) m->d_gsrc, (const int*) m->d_nptr,
+                              (const int*) m->d_nidx, (const double*) m->d_nw,
+                              (const double*) d_f, stride, lines, lstride,
+                              (const int*) s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode,
+                              rnr, roff);
+  P.phase++;
+  if (int rc = plan_band_solve (P, s, 2)) return rc;
+  if (s->tile_mf)
+    batch_rec<CT_THREADS, FnRecoverT<true>> (
+        P, tg, CT_THREADS, tiled_lds (m, true, true), m->np, m->nclass, (const int*) m->d_tile,
+        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_btog,
+        (const int*) m->d_bmask, (const int*) m->d_emask, (const double*) s->d_Mrec,
+        (const double*) s->d_G, m->nglobal, (const double*) d_f, d_u, stride,
+        (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
+        roff);
+  else
+    batch_rec<CT_THREADS, FnRecoverT<false>> (
+        P, tg, CT_THREADS, tiled_lds (m, true, false), m->np, m->nclass,
+        (const int*) m->d_tile, (const int*) m->d_elist, (const int*) m->d_emap,
+        (const int*) m->d_btog, (const int*) m->d_bmask, (const int*) m->d_emask,
+        (const double*) s->d_Mrec, (const double*) s->d_G, m->nglobal, (const double*) d_f,
+        d_u, stride, (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz,
+        s->base_mode, rnr, roff);
+  P.phase++;
+  return SX_OK;
+}
+
+int plan_condense32_tiled (BatchPlan& P, const sx_msys* s, int nz, int stride, const double* X,
+                           double* Fp)
+{
+  const sx_mesh* m = s->m;
+  const TileShape Cs = con_shape (m->np, true);
+  const size_t lds = sizeof (double) * m->next * CG_EP +
+                     sizeof (float) * ((size_t) Cs.KR * Cs.NA + (size_t) Cs.KR * CG_EP) +
+                     sizeof (int) * m->npnp;
+  batch_rec<CT_THREADS, FnCondense32T> (P, dim3 (m->ntile, n_mode_blocks (nz)), CT_THREADS, lds,
+                                        m->np, m->nclass, (const int*) m->d_tile,
+                                        (const int*) m->d_elist, (const int*) m->d_emap,
+                                        (const float*) s->d_Mcon32, X, Fp, stride, nz,
+                                        s->base_mode);
+  P.phase++;
+  return SX_OK;
+}
+
+int plan_recover32_tiled (BatchPlan& P, const sx_msys* s, int nz, int stride, const double* Fp,
+                          double* V)
+{
+  const sx_mesh* m = s->m;
+  const TileShape R = rec_shape (m->np, true);
+  const size_t lds = sizeof (double) * m->next * CG_EP +
+                     sizeof (float) * ((size_t) R.KR * R.NA + (size_t) R.KR * CG_EP) +
+                     sizeof (int) * m->npnp;
+  batch_rec<CT_THREADS, FnRecover32T> (P, dim3 (m->ntile, n_mode_blocks (nz)), CT_THREADS, lds,
+                                       m->np, m->nclass, (const int*) m->d_tile,
+                                       (const int*) m->d_elist, (const int*) m->d_emap,
+                                       (const int*) m->d_btog, (const float*) s->d_Mrec32, Fp,
+                                       V, stride, (const int*) s->d_nsolve,
+                                       (const double*) s->d_rhs, s->nmax, nz, s->base_mode);
+  P.phase++;
   return SX_OK;
 }
 

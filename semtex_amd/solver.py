@@ -471,6 +471,17 @@ class Batch:
     def run(self):
         _lib.check(_lib.load().sx_batch_run(self.handle, _stream()), "sx_batch_run")
 
+    def tune(self, reps=10):
+        """sx_batch_tune: time the sweep choices per band-system class and
+        keep the fastest; returns info()."""
+        _lib.check(_lib.load().sx_batch_tune(self.handle, int(reps), _stream()), "sx_batch_tune")
+        return self.info()
+
+    def info(self):
+        buf = C.create_string_buffer(4096)
+        _lib.check(_lib.load().sx_batch_info(self.handle, buf, 4096), "sx_batch_info")
+        return buf.value.decode()
+
     def __del__(self):
         if getattr(self, "handle", None):
             _lib.load().sx_batch_destroy(self.handle)

@@ -130,3 +130,67 @@ def test_batch_problems_sharing_a_system(torch_cuda, batch_problems, monkeypatch
         torch.cuda.synchronize()
         for (_, u, _), ref in zip(entries, refs):
             assert torch.equal(u.data, ref), rep
+
+
+def test_fused_batch_all_64_problems_vs_oracle(torch_cuda, oracle, monkeypatch):
+    """The whole uneven batch (64 problems) as one fused graph: FP64 against
+    the oracle to 1e-12 for every problem, before and after the autotuner
+    (sx_batch_tune), equal to the graph of per-problem launches
+    (SX_BATCH_FUSED=0) to round-off; the mixed fused graph (one FP64
+    refinement) within 1e-10 of the oracle."""
+    import re
+
+    import torch
+    from semtex_amd.problems import uneven_batch
+    from semtex_amd.solver import SX_FP64, SX_MIXED, Batch
+    probs = uneven_batch()
+    print("case: fused batch of", len(probs), flush=True)
+    refs = []
+    for p in probs:
+        m = p["mesh"]
+        n = m.nel * m.npnp
+        refs.append(oracle.solve_modes(m, p["lambda2"], 0.0, p["forcing"][:, :n])[0])
+
+    def check(built, tol, what):
+        worst = 0.0
+        for p, (_, u, _, _), uo in zip(probs, built, refs):
+            n = p["mesh"].nel * p["mesh"].npnp
+            got = u.data[0, :n].cpu().numpy()
+            worst = max(worst, float(np.linalg.norm(got - uo) / np.linalg.norm(uo)))
+        print(f"  {what}: worst rel L2 vs oracle {worst:.2e}", flush=True)
+        assert worst < tol, (what, worst)
+
+    built = _build(torch, probs, SX_FP64)
+    B = Batch([(M, u, src) for (M, u, src, _) in built])
+    info = B.info()
+    print("  ", info, flush=True)
+    assert info.startswith("fused=1")
+    assert int(re.search(r"launches=(\d+)", info).group(1)) < 200
+    for (_, u, _, _) in built:
+        u.data.zero_()
+    B.run()
+    torch.cuda.synchronize()
+    check(built, 1e-12, "fused FP64")
+    fused = [u.data.clone() for (_, u, _, _) in built]
+    print("   tuned:", B.tune(reps=3), flush=True)
+    for (_, u, _, _) in built:
+        u.data.zero_()
+    B.run()
+    torch.cuda.synchronize()
+    check(built, 1e-12, "tuned fused FP64")
+    # the graph of per-problem launches
+    monkeypatch.setenv("SX_BATCH_FUSED", "0")
+    B0 = Batch([(M, u, src) for (M, u, src, _) in built])
+    monkeypatch.delenv("SX_BATCH_FUSED")
+    assert B0.info().startswith("fused=0")
+    B0.run()
+    torch.cuda.synchronize()
+    for (_, u, _, _), f in zip(built, fused):
+        assert torch.allclose(u.data, f, rtol=0, atol=1e-13 * float(f.abs().max())), "fused vs lanes"
+    del B, B0
+    builtm = _build(torch, probs, SX_MIXED)
+    Bm = Batch([(M, u, src) for (M, u, src, _) in builtm], refine_passes=2)
+    assert Bm.info().startswith("fused=1")
+    Bm.run()
+    torch.cuda.synchronize()
+    check(builtm, 1e-10, "fused mixed, one refinement")
