@@ -245,6 +245,23 @@ int sx_mass_inner (sx_mesh_t m, int nz, int plane_stride, const double* d_a,
 int sx_mass_weight (sx_mesh_t m, int nz, int plane_stride, const double* d_in, double* d_out,
                     double scale, void* stream);
 
+/* Element-local operators of the DNS step and their transposes, for the
+ * hand-written adjoint of the step (no reference counterpart).  Out of
+ * place on nz planes [nz][plane_stride]; transpose = 1 applies the
+ * transpose (adjoint) of the operator.
+ *   sx_grad_planes:   d/dx (dir 0) or d/dy (dir 1) per element
+ *                     (Element::grad, src/element.cpp:521-565);
+ *   sx_smooth_planes: Field::smooth (src/field.cpp:282-327), the
+ *                     mass-weighted average on element-boundary nodes;
+ *   sx_dz_planes:     d/dz of Fourier planes (i beta k; planes 0, 1 zero;
+ *                     AuxField::gradient direction 2), n values per plane. */
+int sx_grad_planes (sx_mesh_t m, int nz, int plane_stride, const double* d_in, double* d_out,
+                    int dir, int transpose, void* stream);
+int sx_smooth_planes (sx_mesh_t m, int nz, int plane_stride, const double* d_in, double* d_out,
+                      int transpose, void* stream);
+int sx_dz_planes (int nz, int plane_stride, int n, double beta, const double* d_in,
+                  double* d_out, int transpose, void* stream);
+
 /* Host-pointer convenience wrapper for drop-in callers whose Field data
  * lives in host memory (Domain::u, src/domain.cpp:75-79): copies in,
  * solves, copies u back (f is not copied back). */

@@ -54,6 +54,9 @@ SIGNATURES = {
     "sx_batch_info": (_i, [_vp, C.c_char_p, _i]),
     "sx_mass_inner": (_i, [_vp, _i, _i, _dp, _dp, _dp, _vp]),
     "sx_mass_weight": (_i, [_vp, _i, _i, _dp, _dp, _d, _vp]),
+    "sx_grad_planes": (_i, [_vp, _i, _i, _dp, _dp, _i, _i, _vp]),
+    "sx_smooth_planes": (_i, [_vp, _i, _i, _dp, _dp, _i, _vp]),
+    "sx_dz_planes": (_i, [_i, _i, _i, _d, _dp, _dp, _i, _vp]),
     "sx_field_solve_pcg": (_i, [_vp, _i, _i, _dp, _dp, _dp, _d, _i, C.POINTER(_i), _vp]),
     "sx_dns_create": (_i, [_vp, _vp, _i, _d, _d, _d, _dp, _i, _i, _vp, C.POINTER(_vp)]),
     "sx_dns_destroy": (_i, [_vp]),

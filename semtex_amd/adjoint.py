@@ -16,10 +16,16 @@ hand-written adjoint:
   again transforms: F^T g = INV(W g) / N and INV^T h = N W' FWD(h), with
   W = diag(1, 1, 1/2, 1/2, ...) and W' = diag(1, 1, 2, 2, ...).
 
-The element-local algebra (tensor-product gradients, Element::grad
-src/element.cpp:521-565; products; Field::smooth src/field.cpp:282-327; the
-computed pressure boundary condition, src/bcmgr.cpp:1037-1335 and
-src/edge.cpp:98-220) is torch arithmetic whose adjoint torch derives.
+* ``grad_xy``, ``grad_z``, ``smooth``: Element::grad (src/element.cpp:521-565),
+  the z derivative of AuxField::gradient (src/auxfield.cpp:640-668) and
+  Field::smooth (src/field.cpp:282-327) by the library's plane kernels
+  (sx_grad_planes, sx_dz_planes, sx_smooth_planes), each backward the
+  transpose kernel of the same entry point.
+
+The products of the nonlinear term and the linear combinations of the time
+levels and of the computed pressure boundary condition (src/bcmgr.cpp:1037-1335,
+src/edge.cpp:98-220; boundary values gathered from the planes) are pointwise
+torch arithmetic between these operators, whose adjoint torch forms.
 Recomputation is checkpointed every ``checkpoint`` steps
 (torch.utils.checkpoint): the backward pass of a segment re-runs its
 forward from the stored state.
@@ -140,7 +146,90 @@ def _make_functions():
                 dlines[1] = 0.0
             return df, dlines, dkeep, None, None, None
 
-    return Transform, Solve
+    def _planes(x):
+        import torch
+        assert x.dtype == torch.float64 and x.is_cuda
+        return x.contiguous()
+
+    class GradXY(torch.autograd.Function):
+        """Element::grad of [nz, n] planes, direction d (0: x, 1: y), by the
+        library's kernel; backward: its transpose kernel."""
+
+        @staticmethod
+        def forward(ctx, x, d, gm):
+            x = _planes(x)
+            ctx.d, ctx.gm = d, gm
+            out = torch.empty_like(x)
+            _lib_check(_L().sx_grad_planes(gm.handle, x.shape[0], x.shape[1], _p(x), _p(out),
+                                           d, 0, S._stream()), "sx_grad_planes")
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            g = _planes(g)
+            out = torch.empty_like(g)
+            _lib_check(_L().sx_grad_planes(ctx.gm.handle, g.shape[0], g.shape[1], _p(g), _p(out),
+                                           ctx.d, 1, S._stream()), "sx_grad_planes")
+            return out, None, None
+
+    class Smooth(torch.autograd.Function):
+        """Field::smooth of [nz, n] planes by the library's kernel;
+        backward: its transpose kernel."""
+
+        @staticmethod
+        def forward(ctx, x, gm):
+            x = _planes(x)
+            ctx.gm = gm
+            out = torch.empty_like(x)
+            _lib_check(_L().sx_smooth_planes(gm.handle, x.shape[0], x.shape[1], _p(x), _p(out),
+                                             0, S._stream()), "sx_smooth_planes")
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            g = _planes(g)
+            out = torch.empty_like(g)
+            _lib_check(_L().sx_smooth_planes(ctx.gm.handle, g.shape[0], g.shape[1], _p(g),
+                                             _p(out), 1, S._stream()), "sx_smooth_planes")
+            return out, None
+
+    class GradZ(torch.autograd.Function):
+        """d/dz of [nz, n] Fourier planes (i beta k) by the library's kernel;
+        backward: the transpose kernel (-d/dz)."""
+
+        @staticmethod
+        def forward(ctx, x, beta):
+            x = _planes(x)
+            ctx.beta = beta
+            out = torch.empty_like(x)
+            _lib_check(_L().sx_dz_planes(x.shape[0], x.shape[1], x.shape[1], float(beta), _p(x),
+                                         _p(out), 0, S._stream()), "sx_dz_planes")
+            return out
+
+        @staticmethod
+        def backward(ctx, g):
+            g = _planes(g)
+            out = torch.empty_like(g)
+            _lib_check(_L().sx_dz_planes(g.shape[0], g.shape[1], g.shape[1], float(ctx.beta),
+                                         _p(g), _p(out), 1, S._stream()), "sx_dz_planes")
+            return out, None
+
+    return Transform, Solve, GradXY, Smooth, GradZ
+
+
+def _L():
+    from . import _lib
+    return _lib.load()
+
+
+def _lib_check(rc, what):
+    from . import _lib
+    _lib.check(rc, what)
+
+
+def _p(t):
+    import ctypes as C
+    return C.c_void_p(t.data_ptr())
 
 
 _FN = None
@@ -195,7 +284,7 @@ class DnsTape:
         self.Mp = S.ModalMatrixSys(0.0, self.beta, 0, nz // 2, self.gp)
         self.Mu = {je: S.ModalMatrixSys(SS[je][0] / (self.dt * self.nu), self.beta, 0, nz // 2,
                                         self.gu) for je in (1, 2)}
-        self.Transform, self.Solve = _fns()
+        self.Transform, self.Solve, self.GradXY, self.Smooth, self.GradZ = _fns()
 
     # ------------------------------------------------------------ operators
     def fwd(self, x):
@@ -205,21 +294,36 @@ class DnsTape:
         return self.Transform.apply(x.contiguous(), -1)
 
     def grad_xy(self, f, d):
+        """Element::grad (src/element.cpp:521-565) of [nz, n] planes in
+        direction d: the library's kernel and its transpose."""
+        return self.GradXY.apply(f, d, self.gu)
+
+    def grad_z(self, F):
+        """Re_k <- -beta k Im_k, Im_k <- beta k Re_k; planes 0, 1 zero."""
+        return self.GradZ.apply(F, self.beta)
+
+    def smooth(self, F):
+        """Field::smooth (src/field.cpp:282-327): the library's kernel and
+        its transpose."""
+        return self.Smooth.apply(F, self.gu)
+
+    # the same three operators as torch arithmetic (Element::grad,
+    # AuxField::gradient in z, Field::smooth), kept to check the kernels
+    def grad_xy_torch(self, f, d):
         u = f.reshape(-1, self.nel, self.np_, self.np_)
         ur = u @ self.D.T
         us = self.D @ u
         out = ur * self.rx + us * self.sx if d == 0 else ur * self.ry + us * self.sy
         return out.reshape(f.shape)
 
-    def grad_z(self, F):
-        """Re_k <- -beta k Im_k, Im_k <- beta k Re_k; planes 0, 1 zero."""
+    def grad_z_torch(self, F):
         import torch
         out = torch.zeros_like(F)
         out[2::2] = -self.bk[2::2] * F[3::2]
         out[3::2] = self.bk[3::2] * F[2::2]
         return out
 
-    def smooth(self, F):
+    def smooth_torch(self, F):
         import torch
         nz = F.shape[0]
         Fe = F.reshape(nz, self.nel, -1)

@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libsemtex_amd.so")
 SOURCES = ["sx_setup.cpp", "sx_solver.hip", "sx_band.hip", "sx_dns.hip", "sx_mixed.hip",
            "sx_fourier.hip", "sx_pcg.hip", "sx_batch.hip", "sx_io.cpp", "sx_enumerate.cpp",
-           "sx_noalloc.hip", "sx_cyl.hip"]
+           "sx_noalloc.hip", "sx_cyl.hip", "sx_tape.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

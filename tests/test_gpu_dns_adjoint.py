@@ -3,8 +3,10 @@ respect to the initial perturbation (the channel shrunk to 6x12 elements,
 N_P=6, N_Z=32, Re=2000, dt=1e-3), with the taped step of
 semtex_amd.adjoint checkpointed every 5 steps:
 
-* the transposes of the z transform and of the solve (forcing and natural
-  boundary data) pass the dot-product identity <A x, y> = <x, A^T y>;
+* the transposes of the z transform, of the solve (forcing and natural
+  boundary data) and of the element operators (gradient, z derivative,
+  smoothing kernels) pass the dot-product identity <A x, y> = <x, A^T y>,
+  and the operator kernels equal the operators written as torch arithmetic;
 * the taped step reproduces the library's DNS step (sx_dns_*) to 1e-11 per
   step;
 * the directional derivative along a seeded direction matches central
@@ -48,6 +50,33 @@ def _single_valued(m, a):
         g[m.btog] = f[:, ext]
         f[:, ext] = g[m.btog]
     return a
+
+
+def test_element_operator_kernels_and_transposes(torch_cuda, setup):
+    """sx_grad_planes (x, y), sx_dz_planes and sx_smooth_planes equal the
+    same operators written as torch arithmetic (Element::grad, the z
+    derivative, Field::smooth) to round-off, and their transpose kernels
+    pass the dot-product identity <A x, y> = <x, A^T y>."""
+    torch = torch_cuda
+    _, T = setup
+    print("case: element operator kernels 6x12 np=6 nz=32", flush=True)
+    rng = np.random.default_rng(3)
+    shape = (T.nz, T.n)
+    ops = [("grad_x", lambda f: T.grad_xy(f, 0), lambda f: T.grad_xy_torch(f, 0)),
+           ("grad_y", lambda f: T.grad_xy(f, 1), lambda f: T.grad_xy_torch(f, 1)),
+           ("grad_z", T.grad_z, T.grad_z_torch),
+           ("smooth", T.smooth, T.smooth_torch)]
+    for name, kern, ref in ops:
+        x = torch.from_numpy(rng.standard_normal(shape)).cuda().requires_grad_(True)
+        y = torch.from_numpy(rng.standard_normal(shape)).cuda()
+        ax = kern(x)
+        r = ref(x.detach())
+        err = float((ax.detach() - r).abs().max() / r.abs().max())
+        assert err < 1e-13, (name, err)
+        lhs = float((ax * y).sum())
+        ax.backward(y)
+        rhs = float((x.detach() * x.grad).sum())
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0), (name, lhs, rhs)
 
 
 def test_transform_transposes(torch_cuda, setup):
