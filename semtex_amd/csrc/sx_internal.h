@@ -134,6 +134,13 @@ struct sx_mesh {
   int* d_tile = nullptr;       // [ntile][3] class, first index into d_elist, count
   int* d_elist = nullptr;      // [nel] element numbers, class-sorted
   int* d_ess_elem = nullptr;   // [ness] elements with essential nodes (emask)
+  // the tile tables in class-sorted element order: row i of d_tbtog /
+  // d_tbmask is the btog / bmask row of element d_elist[i], d_temask[i] its
+  // emask (the register-operand tile kernels read them without the element
+  // number)
+  int* d_tbtog = nullptr;      // [nel][next]
+  int* d_tbmask = nullptr;     // [nel][next]
+  int* d_temask = nullptr;     // [nel]
 };
 constexpr int ETILE = 16;
 

@@ -219,7 +219,8 @@ extern "C" int sx_mesh_destroy (sx_mesh_t m)
   void* ptrs[] = {m->d_geo, m->d_cgeo, m->d_D, m->d_cls, m->d_btog, m->d_bmask,
                   m->d_emask, m->d_emap, m->d_gptr, m->d_gsrc, m->d_ess,
                   m->d_nptr, m->d_nidx, m->d_nw, m->d_ppos, m->d_pmn,
-                  m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem};
+                  m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem,
+                  m->d_tbtog, m->d_tbmask, m->d_temask};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete m;
   return SX_OK;
@@ -411,6 +412,15 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   }
   for (int e = 0; e < nel; e++)
     if (m->emask[e]) ess_elem.push_back (e);
+  std::vector<int> tbtog ((size_t) nel * m->next), tbmask ((size_t) nel * m->next), temask (nel);
+  for (int i = 0; i < nel; i++) {
+    const int e = elist[i];
+    std::copy (m->btog.begin () + (size_t) e * m->next, m->btog.begin () + (size_t) (e + 1) * m->next,
+               tbtog.begin () + (size_t) i * m->next);
+    std::copy (m->bmask.begin () + (size_t) e * m->next,
+               m->bmask.begin () + (size_t) (e + 1) * m->next, tbmask.begin () + (size_t) i * m->next);
+    temask[i] = m->emask[e];
+  }
   m->ntile = (int) tile.size () / 3;
   m->ness = (int) ess_elem.size ();
   if (ess_elem.empty ()) ess_elem.push_back (0);
@@ -427,7 +437,8 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       (rc = upload (&m->d_ppos, ppos)) || (rc = upload (&m->d_pmn, pmn)) ||
       (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc)) ||
       (rc = upload (&m->d_tile, tile)) || (rc = upload (&m->d_elist, elist)) ||
-      (rc = upload (&m->d_ess_elem, ess_elem))) {
+      (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_tbtog, tbtog)) ||
+      (rc = upload (&m->d_tbmask, tbmask)) || (rc = upload (&m->d_temask, temask))) {
     sx_mesh_destroy (m);
     return rc;
   }

@@ -760,6 +760,322 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
   k_recover_t_body<MF> (np, nclass, tiles, elist, emap_g, btog, bmask, emask, Mrec, G, nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
 }
 
+// ---------------------------------------------------------------------------
+// Register-operand MFMA condensation and recovery.  Each wavefront reads the
+// (mode, class) operator columns of its output tiles straight into
+// registers -- a 16-column strip, one double per lane and k-step (the A
+// operand of v_mfma_f64_16x16x4f64: lane l holds M[4s + l/16][a0 + l%16] at
+// step s) -- from L2, where every tile of the mode finds it.  The LDS holds
+// only the element-plane operand, and a workgroup's global loads form two
+// dependent rounds: (1) the tile's element numbers and tile tables
+// (class-sorted btog, bmask, emask rows) with the operator strips, (2) the
+// gathers that need them (field values, band solution, essential values).
+// NPM bounds np: KS k-steps of 4 per strip, NV gathered values per thread.
+// ---------------------------------------------------------------------------
+template <int NPM> struct RegShape {
+  static constexpr int NPNP = NPM * NPM, NEXT = 4 * (NPM - 1), NINT = NPNP - NEXT;
+  static constexpr int KS_REC = pad4 (NPNP) / 4, KS_CON = pad4 (NINT) / 4;
+  static constexpr int NV_REC = (2 * ETILE * NINT + CT_THREADS - 1) / CT_THREADS;
+  static constexpr int NV_CON = (2 * ETILE * NPNP + CT_THREADS - 1) / CT_THREADS;
+  static constexpr int NB_REC = (2 * ETILE * NEXT + CT_THREADS - 1) / CT_THREADS;
+};
+// the smallest instantiated bound for np
+inline int reg_npm (int np) { return np <= 8 ? 8 : (np <= 10 ? 10 : 12); }
+
+// wave w's output tiles: a-tiles w and w + 4 with every ep-tile when there
+// are at least 3 a-tiles; otherwise a-tile w % nat with ep-tiles w / nat
+// and w / nat + 4 / nat
+struct WaveTiles {
+  int a0, astep, e0, estep;
+};
+__device__ __forceinline__ WaveTiles wave_tiles (int w, int nat)
+{
+  if (nat >= 3) return {w, 4, 0, 1};
+  return {w % nat, 1 << 20, w / nat, 4 / nat};
+}
+
+template <int KS>
+__device__ __forceinline__ void load_strip (double (&a)[KS], const double* __restrict__ M, int NA,
+                                            int ks, int a0, int lane)
+{
+  const double* p = M + (size_t) (lane >> 4) * NA + a0 + (lane & 15);
+#pragma unroll
+  for (int s = 0; s < KS; s++) a[s] = s < ks ? p[(size_t) 4 * s * NA] : 0.0;
+}
+
+// D = sum over k-steps s < ks of A_s B_s with A from the strip and B from
+// the LDS operand X [k][CG_EP] (column block at X): the same chunks of MF_U
+// steps on two accumulators and the same tail as mfma_tile, so the sums
+// are formed in the same order
+template <int KS>
+__device__ __forceinline__ v4d mfma_strip (const double (&a)[KS], int ks,
+                                           const double* __restrict__ X, int lane)
+{
+  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+  const double* B = X + (lane >> 4) * CG_EP + (lane & 15);
+  const int kc = ks / MF_U * MF_U;
+#pragma unroll
+  for (int c = 0; c < KS / MF_U; c++) {
+    if (c * MF_U < kc) {
+      double b[MF_U];
+#pragma unroll
+      for (int u = 0; u < MF_U; u++) b[u] = B[(size_t) 4 * (c * MF_U + u) * CG_EP];
+#pragma unroll
+      for (int u = 0; u < MF_U; u += 2) {
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[c * MF_U + u], b[u], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[c * MF_U + u + 1], b[u + 1], acc1, 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < KS; s++)
+    if (s >= kc && s < ks)
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[s], B[(size_t) 4 * s * CG_EP], acc0, 0, 0, 0);
+  return acc0 + acc1;
+}
+
+// f (row-major; already weak on essential elements, else -M f is formed
+// here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i, as
+// k_condense_t; temask is the class-sorted emask
+template <int NPM>
+__device__ __forceinline__ void k_condense_r_body (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ temask,
+    const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
+    int nz, int base_mode, int weak)
+{
+  typedef RegShape<NPM> RS;
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1);
+  const TileShape Cs = con_shape (np, true);
+  const int NE = Cs.NA, ks = Cs.KR / 4, WR = next + Cs.KR;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Ws = sm;                                   // [WR][CG_EP]: w in emap order
+  int* el = (int*) (Ws + (size_t) WR * CG_EP);       // [ETILE] element numbers
+  int* em = el + ETILE;                              // [ETILE] their emask
+  int* emap = em + ETILE;                            // [npnp]
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nat = NE / 16, nept = (nep + 15) / 16;
+  const WaveTiles wt = wave_tiles (wave, nat);
+  const double* Mg = Mcon + ((size_t) mode * nclass + c) * Cs.KR * NE;
+  // round 1
+  if (tid < cnt) {
+    el[tid] = elist[t0 + tid];
+    em[tid] = temask[t0 + tid];
+  }
+  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  double am[RS::KS_CON];
+  if (wt.a0 < nat) load_strip<RS::KS_CON> (am, Mg, NE, ks, 16 * wt.a0, lane);
+  for (int i = npnp * CG_EP + tid; i < WR * CG_EP; i += CT_THREADS) Ws[i] = 0.0;
+  __syncthreads ();
+  // round 2: w = -Q8 f in emap order (f as given on essential elements /
+  // weak input)
+  {
+    double fv[RS::NV_CON], sc[RS::NV_CON];
+#pragma unroll
+    for (int u = 0; u < RS::NV_CON; u++) {
+      const int i = tid + u * CT_THREADS;
+      fv[u] = 0.0;
+      sc[u] = 1.0;
+      if (i < nep * npnp) {
+        const int ep = i / npnp, a = i % npnp, j = ep / npm, e = el[j], r = emap[a];
+        fv[u] = F[(size_t) (k0 + ep % npm) * stride + (size_t) e * npnp + r];
+        if (!weak && !em[j]) sc[u] = -geo[((size_t) e * NGEO + G_Q8) * npnp + r];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RS::NV_CON; u++) {
+      const int i = tid + u * CT_THREADS;
+      if (i < nep * npnp) Ws[(i % npnp) * CG_EP + i / npnp] = fv[u] * sc[u];
+    }
+  }
+  __syncthreads ();
+  // F_b = w_b - Bm w_i over this wave's 16x16 tiles, written over w_b (each
+  // entry by the lane that reads it; the products read rows next.. only)
+#pragma unroll
+  for (int si = 0; si < 2; si++) {
+    const int ta = wt.a0 + si * wt.astep;
+    if (ta >= nat) break;
+    if (si) load_strip<RS::KS_CON> (am, Mg, NE, ks, 16 * ta, lane);
+#pragma unroll
+    for (int ei = 0; ei < 2; ei++) {
+      const int te = wt.e0 + ei * wt.estep;
+      if (te >= nept) continue;
+      const v4d acc = mfma_strip<RS::KS_CON> (am, ks, Ws + (size_t) next * CG_EP + 16 * te, lane);
+      const int ep = 16 * te + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int a = 16 * ta + 4 * r + (lane >> 4);
+        if (a < next && ep < nep) Ws[a * CG_EP + ep] -= acc[r];
+      }
+    }
+  }
+  __syncthreads ();
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, a = i % npnp, e = el[ep / npm], k = k0 + ep % npm;
+    F[(size_t) k * stride + (size_t) e * npnp + a] = Ws[a * CG_EP + ep];
+  }
+}
+
+template <int NPM>
+__global__ __launch_bounds__(CT_THREADS) void k_condense_r (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ temask,
+    const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
+    int nz, int base_mode, int weak)
+{
+  k_condense_r_body<NPM> (np, nclass, tiles, elist, emap_g, temask, geo, Mcon, F, stride, nz,
+                          base_mode, weak);
+}
+
+// u_i = Hinv^T F_i - Bm^T u_b and the essential values, as k_recover_t;
+// tbtog, tbmask and temask are the class-sorted tile tables
+template <int NPM>
+__device__ __forceinline__ void k_recover_r_body (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ tbtog,
+    const int* __restrict__ tbmask, const int* __restrict__ temask,
+    const double* __restrict__ Mrec,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
+    int nz, int base_mode, int rnr, int roff)
+{
+  typedef RegShape<NPM> RS;
+  extern __shared__ double sm[];
+  const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
+  const TileShape R = rec_shape (np, true);
+  const int NI = R.NA, ks = R.KR / 4;
+  const int mode = blockIdx.y;
+  int k0;
+  const int npm = mode_planes (mode, nz, base_mode, &k0);
+  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
+            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  double* Xs = sm;                                   // [KR][CG_EP]: u_b | F_i, then u
+  int* el = (int*) (Xs + (size_t) R.KR * CG_EP);     // [ETILE] element numbers
+  int* pm = el + ETILE;                              // row-major position -> emap index
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ns = nsolve[mode];
+  const int nat = NI / 16, nept = (nep + 15) / 16;
+  const WaveTiles wt = wave_tiles (wave, nat);
+  const double* Mg = Mrec + ((size_t) mode * nclass + c) * R.KR * NI;
+  // round 1
+  if (tid < cnt) el[tid] = elist[t0 + tid];
+  for (int i = tid; i < npnp; i += CT_THREADS) pm[emap_g[i]] = i;
+  int gg[RS::NB_REC];
+  bool ess[RS::NB_REC];
+#pragma unroll
+  for (int u = 0; u < RS::NB_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    gg[u] = nmax;
+    ess[u] = false;
+    if (i < nep * next) {
+      const int row = t0 + i / next / npm, a = i % next;
+      gg[u] = tbtog[(size_t) row * next + a];
+      ess[u] = (temask[row] != 0) & (tbmask[(size_t) row * next + a] != 0);
+    }
+  }
+  double am[RS::KS_REC];
+  if (wt.a0 < nat) load_strip<RS::KS_REC> (am, Mg, NI, ks, 16 * wt.a0, lane);
+  for (int i = npnp * CG_EP + tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0;
+  __syncthreads ();
+  // round 2: u_b from the band solution (zero on essential nodes, whose
+  // values entered the right-hand side; the output takes G there) and F_i
+  double ub[RS::NB_REC], bout[RS::NB_REC], fv[RS::NV_REC];
+#pragma unroll
+  for (int u = 0; u < RS::NB_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    ub[u] = 0.0;
+    bout[u] = 0.0;
+    if (i < nep * next) {
+      const int k = k0 + (i / next) % npm;
+      if (gg[u] < ns) ub[u] = rhs[((size_t) mode * rnr + roff + (k & 1)) * nmax + gg[u]];
+      if (ess[u]) bout[u] = G[(size_t) k * nglobal + gg[u]];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RS::NV_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    fv[u] = 0.0;
+    if (i < nep * nint) {
+      const int ep = i / nint, q = i % nint;
+      fv[u] = F[(size_t) (k0 + ep % npm) * stride + (size_t) el[ep / npm] * npnp + next + q];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RS::NB_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    if (i < nep * next) {
+      Xs[(i % next) * CG_EP + i / next] = ub[u];
+      if (!ess[u]) bout[u] = ub[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RS::NV_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    if (i < nep * nint) Xs[(next + i % nint) * CG_EP + i / nint] = fv[u];
+  }
+  __syncthreads ();
+  // u_i over this wave's 16x16 tiles, held until every wave has read Xs
+  v4d acc[4];
+#pragma unroll
+  for (int si = 0; si < 2; si++) {
+    const int ta = wt.a0 + si * wt.astep;
+#pragma unroll
+    for (int ei = 0; ei < 2; ei++) acc[2 * si + ei] = v4d {0.0, 0.0, 0.0, 0.0};
+    if (ta >= nat) continue;
+    if (si) load_strip<RS::KS_REC> (am, Mg, NI, ks, 16 * ta, lane);
+#pragma unroll
+    for (int ei = 0; ei < 2; ei++) {
+      const int te = wt.e0 + ei * wt.estep;
+      if (te < nept) acc[2 * si + ei] = mfma_strip<RS::KS_REC> (am, ks, Xs + 16 * te, lane);
+    }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int si = 0; si < 2; si++) {
+    const int ta = wt.a0 + si * wt.astep;
+    if (ta >= nat) continue;
+#pragma unroll
+    for (int ei = 0; ei < 2; ei++) {
+      const int te = wt.e0 + ei * wt.estep, ep = 16 * te + (lane & 15);
+      if (te >= nept || ep >= nep) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int a = 16 * ta + 4 * r + (lane >> 4);
+        if (a < nint) Xs[(next + a) * CG_EP + ep] = acc[2 * si + ei][r];
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < RS::NB_REC; u++) {
+    const int i = tid + u * CT_THREADS;
+    if (i < nep * next) Xs[(i % next) * CG_EP + i / next] = bout[u];
+  }
+  __syncthreads ();
+  // whole element-planes out, row-major
+  for (int i = tid; i < nep * npnp; i += CT_THREADS) {
+    const int ep = i / npnp, r = i % npnp, e = el[ep / npm], k = k0 + ep % npm;
+    U[(size_t) k * stride + (size_t) e * npnp + r] = Xs[pm[r] * CG_EP + ep];
+  }
+}
+
+template <int NPM>
+__global__ __launch_bounds__(CT_THREADS) void k_recover_r (
+    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
+    const int* __restrict__ emap_g, const int* __restrict__ tbtog,
+    const int* __restrict__ tbmask, const int* __restrict__ temask,
+    const double* __restrict__ Mrec,
+    const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
+    int nz, int base_mode, int rnr, int roff)
+{
+  k_recover_r_body<NPM> (np, nclass, tiles, elist, emap_g, tbtog, tbmask, temask, Mrec, G,
+                         nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+}
+
 // FP32 forms of the tiled condensation and recovery for the mixed solve's
 // inner passes (FP32 operands and MFMA tiles, FP32 accumulation; the
 // boundary values stay FP64).
@@ -1323,6 +1639,55 @@ bool tiled_mfma (const sx_mesh* m)
 int n_mode_blocks (int nz) { return nz == 1 ? 1 : nz / 2; }
 
 // constrain + condense of one field (F: forcing in, condensed out)
+// LDS of the register-operand tile kernels
+size_t reg_lds (const sx_mesh* m, bool recover)
+{
+  if (recover) {
+    const TileShape R = rec_shape (m->np, true);
+    return sizeof (double) * (size_t) R.KR * CG_EP + sizeof (int) * (ETILE + m->npnp);
+  }
+  const TileShape Cs = con_shape (m->np, true);
+  return sizeof (double) * (size_t) (m->next + Cs.KR) * CG_EP +
+         sizeof (int) * (2 * ETILE + m->npnp);
+}
+bool elem_lds ()
+{
+  static const int v = [] { const char* e = std::getenv ("SX_ELEM_LDS"); return e ? std::atoi (e) : 0; } ();
+  return v != 0;
+}
+
+template <int NPM>
+int launch_condense_r (const sx_msys* s, int nz, int stride, double* F, int weak, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const size_t lds = reg_lds (m, false);
+  auto kern = k_condense_r<NPM>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds, st,
+                      m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_temask, m->d_geo,
+                      s->d_Mcon, F, stride, nz, s->base_mode, weak);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
+template <int NPM>
+int launch_recover_r (const sx_msys* s, int nz, int stride, const double* F, double* U,
+                      const double* G, int rnr, int roff, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const size_t lds = reg_lds (m, true);
+  auto kern = k_recover_r<NPM>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds, st,
+                      m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_tbtog,
+                      m->d_tbmask, m->d_temask, s->d_Mrec, G, m->nglobal, F, U, stride,
+                      s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
 int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st,
               int weak = 0)
 {
@@ -1345,6 +1710,13 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
                         s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
                         m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
   const bool mf = s->tile_mf;
+  if (mf && !elem_lds ()) {
+    switch (reg_npm (m->np)) {
+    case 8: return launch_condense_r<8> (s, nz, stride, F, weak, st);
+    case 10: return launch_condense_r<10> (s, nz, stride, F, weak, st);
+    default: return launch_condense_r<12> (s, nz, stride, F, weak, st);
+    }
+  }
   const size_t lds = tiled_lds (m, false, mf);
   auto kern = mf ? k_condense_t<true> : k_condense_t<false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
@@ -1370,6 +1742,13 @@ int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, c
     return SX_OK;
   }
   const bool mf = s->tile_mf;
+  if (mf && !elem_lds ()) {
+    switch (reg_npm (m->np)) {
+    case 8: return launch_recover_r<8> (s, nz, stride, F, U, G, rnr, roff, st);
+    case 10: return launch_recover_r<10> (s, nz, stride, F, U, G, rnr, roff, st);
+    default: return launch_recover_r<12> (s, nz, stride, F, U, G, rnr, roff, st);
+    }
+  }
   const size_t lds = tiled_lds (m, true, mf);
   auto kern = mf ? k_recover_t<true> : k_recover_t<false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
@@ -1535,6 +1914,14 @@ template <bool MF> struct FnRecoverT {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
   { k_recover_t_body<MF> (a...); }
 };
+template <int NPM> struct FnCondenseR {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_condense_r_body<NPM> (a...); }
+};
+template <int NPM> struct FnRecoverR {
+  template <typename... A> __device__ __forceinline__ void operator() (A... a) const
+  { k_recover_r_body<NPM> (a...); }
+};
 struct FnCondense32T {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
   { k_condense32_t_body (a...); }
@@ -1570,7 +1957,18 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
   }
   const dim3 tg (m->ntile, n_mode_blocks (nz));
   const int weak = 0;
-  if (s->tile_mf)
+  const int npmr = reg_npm (m->np);
+  auto con_r = [&] (auto fn) {
+    batch_rec<CT_THREADS, decltype (fn)> (
+        P, tg, CT_THREADS, reg_lds (m, false), m->np, m->nclass, (const int*) m->d_tile,
+        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_temask,
+        (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride, nz, s->base_mode, weak);
+  };
+  if (s->tile_mf && !elem_lds ()) {
+    if (npmr == 8) con_r (FnCondenseR<8> {});
+    else if (npmr == 10) con_r (FnCondenseR<10> {});
+    else con_r (FnCondenseR<12> {});
+  } else if (s->tile_mf)
     batch_rec<CT_THREADS, FnCondenseT<true>> (
         P, tg, CT_THREADS, tiled_lds (m, false, true), m->np, m->nclass, (const int*) m->d_tile,
         (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_emask,
@@ -1591,7 +1989,20 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
                               rnr, roff);
   P.phase++;
   if (int rc = plan_band_solve (P, s, 2)) return rc;
-  if (s->tile_mf)
+  auto rec_r = [&] (auto fn) {
+    batch_rec<CT_THREADS, decltype (fn)> (
+        P, tg, CT_THREADS, reg_lds (m, true), m->np, m->nclass, (const int*) m->d_tile,
+        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_tbtog,
+        (const int*) m->d_tbmask, (const int*) m->d_temask, (const double*) s->d_Mrec,
+        (const double*) s->d_G, m->nglobal, (const double*) d_f, d_u, stride,
+        (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
+        roff);
+  };
+  if (s->tile_mf && !elem_lds ()) {
+    if (npmr == 8) rec_r (FnRecoverR<8> {});
+    else if (npmr == 10) rec_r (FnRecoverR<10> {});
+    else rec_r (FnRecoverR<12> {});
+  } else if (s->tile_mf)
     batch_rec<CT_THREADS, FnRecoverT<true>> (
         P, tg, CT_THREADS, tiled_lds (m, true, true), m->np, m->nclass, (const int*) m->d_tile,
         (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_btog,
