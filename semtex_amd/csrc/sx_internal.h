@@ -134,15 +134,15 @@ struct sx_mesh {
   int* d_tile = nullptr;       // [ntile][3] class, first index into d_elist, count
   int* d_elist = nullptr;      // [nel] element numbers, class-sorted
   int* d_ess_elem = nullptr;   // [ness] elements with essential nodes (emask)
-  // the tile tables in class-sorted element order: row i of d_tbtog /
-  // d_tbmask is the btog / bmask row of element d_elist[i], d_temask[i] its
-  // emask (the register-operand tile kernels read them without the element
-  // number)
-  int* d_tbtog = nullptr;      // [nel][next]
-  int* d_tbmask = nullptr;     // [nel][next]
-  int* d_temask = nullptr;     // [nel]
+  // per-tile tables of the register-operand tile kernels, read by block
+  // index with no prior load: [ntile][tile_ints (next)], per tile the class,
+  // the element count, 2 unused, the ETILE element numbers (the last one
+  // repeated past the count), their emask, then their btog and bmask rows
+  int* d_ttab = nullptr;
 };
 constexpr int ETILE = 16;
+constexpr int TT_EL = 4, TT_EM = TT_EL + ETILE, TT_BT = TT_EM + ETILE;
+__host__ __device__ constexpr int tile_ints (int next) { return TT_BT + 2 * ETILE * next; }
 
 struct sx_msys {
   sx_mesh* m = nullptr;

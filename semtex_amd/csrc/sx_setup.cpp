@@ -220,7 +220,7 @@ extern "C" int sx_mesh_destroy (sx_mesh_t m)
                   m->d_emask, m->d_emap, m->d_gptr, m->d_gsrc, m->d_ess,
                   m->d_nptr, m->d_nidx, m->d_nw, m->d_ppos, m->d_pmn,
                   m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem,
-                  m->d_tbtog, m->d_tbmask, m->d_temask};
+                  m->d_ttab};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete m;
   return SX_OK;
@@ -412,16 +412,24 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   }
   for (int e = 0; e < nel; e++)
     if (m->emask[e]) ess_elem.push_back (e);
-  std::vector<int> tbtog ((size_t) nel * m->next), tbmask ((size_t) nel * m->next), temask (nel);
-  for (int i = 0; i < nel; i++) {
-    const int e = elist[i];
-    std::copy (m->btog.begin () + (size_t) e * m->next, m->btog.begin () + (size_t) (e + 1) * m->next,
-               tbtog.begin () + (size_t) i * m->next);
-    std::copy (m->bmask.begin () + (size_t) e * m->next,
-               m->bmask.begin () + (size_t) (e + 1) * m->next, tbmask.begin () + (size_t) i * m->next);
-    temask[i] = m->emask[e];
-  }
   m->ntile = (int) tile.size () / 3;
+  const int ti = tile_ints (m->next);
+  std::vector<int> ttab ((size_t) m->ntile * ti, 0);
+  for (int t = 0; t < m->ntile; t++) {
+    int* T = ttab.data () + (size_t) t * ti;
+    const int t0 = tile[3 * t + 1], cnt = tile[3 * t + 2];
+    T[0] = tile[3 * t];
+    T[1] = cnt;
+    for (int j = 0; j < ETILE; j++) {
+      const int e = elist[t0 + std::min (j, cnt - 1)];
+      T[TT_EL + j] = e;
+      T[TT_EM + j] = m->emask[e];
+      for (int a = 0; a < m->next; a++) {
+        T[TT_BT + j * m->next + a] = m->btog[(size_t) e * m->next + a];
+        T[TT_BT + (ETILE + j) * m->next + a] = m->bmask[(size_t) e * m->next + a];
+      }
+    }
+  }
   m->ness = (int) ess_elem.size ();
   if (ess_elem.empty ()) ess_elem.push_back (0);
 
@@ -437,8 +445,7 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       (rc = upload (&m->d_ppos, ppos)) || (rc = upload (&m->d_pmn, pmn)) ||
       (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc)) ||
       (rc = upload (&m->d_tile, tile)) || (rc = upload (&m->d_elist, elist)) ||
-      (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_tbtog, tbtog)) ||
-      (rc = upload (&m->d_tbmask, tbmask)) || (rc = upload (&m->d_temask, temask))) {
+      (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_ttab, ttab))) {
     sx_mesh_destroy (m);
     return rc;
   }

@@ -836,11 +836,12 @@ __device__ __forceinline__ v4d mfma_strip (const double (&a)[KS], int ks,
 
 // f (row-major; already weak on essential elements, else -M f is formed
 // here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i, as
-// k_condense_t; temask is the class-sorted emask
+// k_condense_t.  Round 1 reads the tile table (class, count, element
+// numbers, emask) and the emap entries each thread needs; round 2 the
+// field values, the mass weights and the operator strip.
 template <int NPM>
 __device__ __forceinline__ void k_condense_r_body (
-    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
-    const int* __restrict__ emap_g, const int* __restrict__ temask,
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
     const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
     int nz, int base_mode, int weak)
 {
@@ -852,28 +853,31 @@ __device__ __forceinline__ void k_condense_r_body (
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
-  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
-            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  const int* T = ttab + (size_t) blockIdx.x * tile_ints (next);
   double* Ws = sm;                                   // [WR][CG_EP]: w in emap order
   int* el = (int*) (Ws + (size_t) WR * CG_EP);       // [ETILE] element numbers
-  int* em = el + ETILE;                              // [ETILE] their emask
-  int* emap = em + ETILE;                            // [npnp]
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int nat = NE / 16, nept = (nep + 15) / 16;
+  const int nat = NE / 16;
   const WaveTiles wt = wave_tiles (wave, nat);
-  const double* Mg = Mcon + ((size_t) mode * nclass + c) * Cs.KR * NE;
   // round 1
-  if (tid < cnt) {
-    el[tid] = elist[t0 + tid];
-    em[tid] = temask[t0 + tid];
+  const int c = T[0], cnt = T[1], nep = cnt * npm, nept = (nep + 15) / 16;
+  if (tid < ETILE) el[tid] = T[TT_EL + tid];
+  int ev[RS::NV_CON], rv[RS::NV_CON];
+  bool mv[RS::NV_CON];
+#pragma unroll
+  for (int u = 0; u < RS::NV_CON; u++) {
+    const int i = min (tid + u * CT_THREADS, 2 * ETILE * npnp - 1);
+    const int j = min (i / npnp / npm, ETILE - 1);
+    ev[u] = T[TT_EL + j];
+    mv[u] = T[TT_EM + j] != 0;
+    rv[u] = emap_g[i % npnp];
   }
-  for (int i = tid; i < npnp; i += CT_THREADS) emap[i] = emap_g[i];
+  for (int i = npnp * CG_EP + tid; i < WR * CG_EP; i += CT_THREADS) Ws[i] = 0.0;
+  // round 2: w = -Q8 f in emap order (f as given on essential elements /
+  // weak input) and the operator strip
+  const double* Mg = Mcon + ((size_t) mode * nclass + c) * Cs.KR * NE;
   double am[RS::KS_CON];
   if (wt.a0 < nat) load_strip<RS::KS_CON> (am, Mg, NE, ks, 16 * wt.a0, lane);
-  for (int i = npnp * CG_EP + tid; i < WR * CG_EP; i += CT_THREADS) Ws[i] = 0.0;
-  __syncthreads ();
-  // round 2: w = -Q8 f in emap order (f as given on essential elements /
-  // weak input)
   {
     double fv[RS::NV_CON], sc[RS::NV_CON];
 #pragma unroll
@@ -882,9 +886,9 @@ __device__ __forceinline__ void k_condense_r_body (
       fv[u] = 0.0;
       sc[u] = 1.0;
       if (i < nep * npnp) {
-        const int ep = i / npnp, a = i % npnp, j = ep / npm, e = el[j], r = emap[a];
+        const int ep = i / npnp, e = ev[u], r = rv[u];
         fv[u] = F[(size_t) (k0 + ep % npm) * stride + (size_t) e * npnp + r];
-        if (!weak && !em[j]) sc[u] = -geo[((size_t) e * NGEO + G_Q8) * npnp + r];
+        if (!weak && !mv[u]) sc[u] = -geo[((size_t) e * NGEO + G_Q8) * npnp + r];
       }
     }
 #pragma unroll
@@ -923,22 +927,20 @@ __device__ __forceinline__ void k_condense_r_body (
 
 template <int NPM>
 __global__ __launch_bounds__(CT_THREADS) void k_condense_r (
-    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
-    const int* __restrict__ emap_g, const int* __restrict__ temask,
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
     const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
     int nz, int base_mode, int weak)
 {
-  k_condense_r_body<NPM> (np, nclass, tiles, elist, emap_g, temask, geo, Mcon, F, stride, nz,
-                          base_mode, weak);
+  k_condense_r_body<NPM> (np, nclass, ttab, emap_g, geo, Mcon, F, stride, nz, base_mode, weak);
 }
 
-// u_i = Hinv^T F_i - Bm^T u_b and the essential values, as k_recover_t;
-// tbtog, tbmask and temask are the class-sorted tile tables
+// u_i = Hinv^T F_i - Bm^T u_b and the essential values, as k_recover_t.
+// Round 1 reads the tile table (class, count, element numbers, the btog and
+// mask entries of each thread's boundary values) and the emap; round 2 the
+// band solution, the essential values, F_i and the operator strip.
 template <int NPM>
 __device__ __forceinline__ void k_recover_r_body (
-    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
-    const int* __restrict__ emap_g, const int* __restrict__ tbtog,
-    const int* __restrict__ tbmask, const int* __restrict__ temask,
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
     const double* __restrict__ Mrec,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
@@ -952,37 +954,38 @@ __device__ __forceinline__ void k_recover_r_body (
   const int mode = blockIdx.y;
   int k0;
   const int npm = mode_planes (mode, nz, base_mode, &k0);
-  const int c = tiles[3 * blockIdx.x], t0 = tiles[3 * blockIdx.x + 1],
-            cnt = tiles[3 * blockIdx.x + 2], nep = cnt * npm;
+  const int* T = ttab + (size_t) blockIdx.x * tile_ints (next);
   double* Xs = sm;                                   // [KR][CG_EP]: u_b | F_i, then u
   int* el = (int*) (Xs + (size_t) R.KR * CG_EP);     // [ETILE] element numbers
   int* pm = el + ETILE;                              // row-major position -> emap index
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, ns = nsolve[mode];
-  const int nat = NI / 16, nept = (nep + 15) / 16;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int nat = NI / 16;
   const WaveTiles wt = wave_tiles (wave, nat);
-  const double* Mg = Mrec + ((size_t) mode * nclass + c) * R.KR * NI;
   // round 1
-  if (tid < cnt) el[tid] = elist[t0 + tid];
+  const int c = T[0], cnt = T[1], nep = cnt * npm, nept = (nep + 15) / 16, ns = nsolve[mode];
+  if (tid < ETILE) el[tid] = T[TT_EL + tid];
   for (int i = tid; i < npnp; i += CT_THREADS) pm[emap_g[i]] = i;
-  int gg[RS::NB_REC];
+  int gg[RS::NB_REC], ev[RS::NV_REC];
   bool ess[RS::NB_REC];
 #pragma unroll
   for (int u = 0; u < RS::NB_REC; u++) {
-    const int i = tid + u * CT_THREADS;
-    gg[u] = nmax;
-    ess[u] = false;
-    if (i < nep * next) {
-      const int row = t0 + i / next / npm, a = i % next;
-      gg[u] = tbtog[(size_t) row * next + a];
-      ess[u] = (temask[row] != 0) & (tbmask[(size_t) row * next + a] != 0);
-    }
+    const int i = min (tid + u * CT_THREADS, 2 * ETILE * next - 1);
+    const int j = min (i / next / npm, ETILE - 1), a = i % next;
+    gg[u] = T[TT_BT + j * next + a];
+    ess[u] = (T[TT_EM + j] != 0) & (T[TT_BT + (ETILE + j) * next + a] != 0);
   }
+#pragma unroll
+  for (int u = 0; u < RS::NV_REC; u++) {
+    const int i = min (tid + u * CT_THREADS, 2 * ETILE * nint - 1);
+    ev[u] = T[TT_EL + min (i / nint / npm, ETILE - 1)];
+  }
+  for (int i = npnp * CG_EP + tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0;
+  // round 2: u_b from the band solution (zero on essential nodes, whose
+  // values entered the right-hand side; the output takes G there), F_i and
+  // the operator strip
+  const double* Mg = Mrec + ((size_t) mode * nclass + c) * R.KR * NI;
   double am[RS::KS_REC];
   if (wt.a0 < nat) load_strip<RS::KS_REC> (am, Mg, NI, ks, 16 * wt.a0, lane);
-  for (int i = npnp * CG_EP + tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0;
-  __syncthreads ();
-  // round 2: u_b from the band solution (zero on essential nodes, whose
-  // values entered the right-hand side; the output takes G there) and F_i
   double ub[RS::NB_REC], bout[RS::NB_REC], fv[RS::NV_REC];
 #pragma unroll
   for (int u = 0; u < RS::NB_REC; u++) {
@@ -1001,7 +1004,7 @@ __device__ __forceinline__ void k_recover_r_body (
     fv[u] = 0.0;
     if (i < nep * nint) {
       const int ep = i / nint, q = i % nint;
-      fv[u] = F[(size_t) (k0 + ep % npm) * stride + (size_t) el[ep / npm] * npnp + next + q];
+      fv[u] = F[(size_t) (k0 + ep % npm) * stride + (size_t) ev[u] * npnp + next + q];
     }
   }
 #pragma unroll
@@ -1064,16 +1067,14 @@ __device__ __forceinline__ void k_recover_r_body (
 
 template <int NPM>
 __global__ __launch_bounds__(CT_THREADS) void k_recover_r (
-    int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
-    const int* __restrict__ emap_g, const int* __restrict__ tbtog,
-    const int* __restrict__ tbmask, const int* __restrict__ temask,
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
     const double* __restrict__ Mrec,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
     int nz, int base_mode, int rnr, int roff)
 {
-  k_recover_r_body<NPM> (np, nclass, tiles, elist, emap_g, tbtog, tbmask, temask, Mrec, G,
-                         nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+  k_recover_r_body<NPM> (np, nclass, ttab, emap_g, Mrec, G, nglobal, F, U, stride, nsolve, rhs,
+                         nmax, nz, base_mode, rnr, roff);
 }
 
 // FP32 forms of the tiled condensation and recovery for the mixed solve's
@@ -1647,8 +1648,7 @@ size_t reg_lds (const sx_mesh* m, bool recover)
     return sizeof (double) * (size_t) R.KR * CG_EP + sizeof (int) * (ETILE + m->npnp);
   }
   const TileShape Cs = con_shape (m->np, true);
-  return sizeof (double) * (size_t) (m->next + Cs.KR) * CG_EP +
-         sizeof (int) * (2 * ETILE + m->npnp);
+  return sizeof (double) * (size_t) (m->next + Cs.KR) * CG_EP + sizeof (int) * ETILE;
 }
 bool elem_lds ()
 {
@@ -1665,8 +1665,8 @@ int launch_condense_r (const sx_msys* s, int nz, int stride, double* F, int weak
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds, st,
-                      m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_temask, m->d_geo,
-                      s->d_Mcon, F, stride, nz, s->base_mode, weak);
+                      m->np, m->nclass, m->d_ttab, m->d_emap, m->d_geo, s->d_Mcon, F, stride, nz,
+                      s->base_mode, weak);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -1681,9 +1681,8 @@ int launch_recover_r (const sx_msys* s, int nz, int stride, const double* F, dou
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds, st,
-                      m->np, m->nclass, m->d_tile, m->d_elist, m->d_emap, m->d_tbtog,
-                      m->d_tbmask, m->d_temask, s->d_Mrec, G, m->nglobal, F, U, stride,
-                      s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
+                      m->np, m->nclass, m->d_ttab, m->d_emap, s->d_Mrec, G, m->nglobal, F, U,
+                      stride, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -1960,9 +1959,9 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
   const int npmr = reg_npm (m->np);
   auto con_r = [&] (auto fn) {
     batch_rec<CT_THREADS, decltype (fn)> (
-        P, tg, CT_THREADS, reg_lds (m, false), m->np, m->nclass, (const int*) m->d_tile,
-        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_temask,
-        (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride, nz, s->base_mode, weak);
+        P, tg, CT_THREADS, reg_lds (m, false), m->np, m->nclass, (const int*) m->d_ttab,
+        (const int*) m->d_emap, (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride,
+        nz, s->base_mode, weak);
   };
   if (s->tile_mf && !elem_lds ()) {
     if (npmr == 8) con_r (FnCondenseR<8> {});
@@ -1991,9 +1990,8 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
   if (int rc = plan_band_solve (P, s, 2)) return rc;
   auto rec_r = [&] (auto fn) {
     batch_rec<CT_THREADS, decltype (fn)> (
-        P, tg, CT_THREADS, reg_lds (m, true), m->np, m->nclass, (const int*) m->d_tile,
-        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_tbtog,
-        (const int*) m->d_tbmask, (const int*) m->d_temask, (const double*) s->d_Mrec,
+        P, tg, CT_THREADS, reg_lds (m, true), m->np, m->nclass, (const int*) m->d_ttab,
+        (const int*) m->d_emap, (const double*) s->d_Mrec,
         (const double*) s->d_G, m->nglobal, (const double*) d_f, d_u, stride,
         (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
         roff);
