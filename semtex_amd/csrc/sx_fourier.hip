@@ -17,6 +17,7 @@
 // transform is HBM-bound (16 bytes per point per pass) and may run in place.
 // Other nz use the packed DFT matrix, one thread per column.
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -188,7 +189,9 @@ struct Fused {
   const double* amul = nullptr;
 };
 
-template <int N, int R1, int TH>
+// V16: column pairs moved as 16-byte accesses (S even, rows 16-byte
+// aligned).
+template <int N, int R1, int TH, bool V16>
 __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, int S,
                                                         int sign, const double2* __restrict__ tw,
                                                         Fused fu)
@@ -198,50 +201,65 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
   extern __shared__ double2 lds[];
   double2* T = lds;                   // [N][NQ] exchange, then the spectrum
   double2* W = T + N * NQ;            // exp(-2 pi i j / N), j < N/2
-  const int s0 = blockIdx.x * TS, tid = threadIdx.x;
+  const int tid = threadIdx.x, ntiles = (S + TS - 1) / TS;
   const bool inv = sign < 0;
   for (int i = tid; i < N / 2; i += TH) W[i] = tw[i];
-  // step A: thread (q, n2), columns c0 and c0 + 1 as one complex sequence
+  // step A is thread (qa, n2): columns c0 and c0 + 1 as one complex
+  // sequence, points k = R2 n1 + n2.  Raw operands: forward (x, mul) column
+  // pairs, inverse the (Re, Im) rows of the packed spectra.
+  const int qa = tid % NQ, n2 = tid / NQ;
+  double2 p0[R1], p1[R1];
+  auto pair = [&] (const double* row, int c0) __attribute__ ((always_inline)) {
+    if (V16) return c0 < S ? *(const double2*) (row + c0) : make_double2 (0.0, 0.0);
+    return make_double2 (c0 < S ? row[c0] : 0.0, c0 + 1 < S ? row[c0 + 1] : 0.0);
+  };
+  auto fetch = [&] (int t) __attribute__ ((always_inline)) {
+    const int c0 = t * TS + 2 * qa;
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int k = R2 * n1 + n2;
+      if (!inv) {
+        p0[n1] = pair (in + (size_t) k * S, c0);
+        if (fu.mul) p1[n1] = pair (fu.mul + (size_t) k * S, c0);
+      } else {
+        const int kk = k <= N / 2 ? k : N - k;
+        const int pr = kk == 0 ? 0 : (kk == N / 2 ? 1 : 2 * kk);
+        p0[n1] = pair (in + (size_t) pr * S, c0);
+        p1[n1] = kk != 0 && kk != N / 2 ? pair (in + (size_t) (2 * kk + 1) * S, c0)
+                                        : make_double2 (0.0, 0.0);
+      }
+    }
+  };
+  const int t = blockIdx.x, s0 = t * TS;
+  if (t < ntiles) fetch (t);
+  __syncthreads ();                   // twiddles in LDS
+  if (t >= ntiles) return;
   {
-    const int q = tid % NQ, n2 = tid / NQ, c0 = s0 + 2 * q;
-    const bool in0 = c0 < S, in1 = c0 + 1 < S;
     double2 v[R1];
 #pragma unroll
     for (int n1 = 0; n1 < R1; n1++) {
       const int k = R2 * n1 + n2;
       if (!inv) {
-        const double* row = in + (size_t) k * S;
-        v[n1] = make_double2 (in0 ? row[c0] : 0.0, in1 ? row[c0 + 1] : 0.0);
-        if (fu.mul) {
-          const double* mr = fu.mul + (size_t) k * S;
-          v[n1].x *= in0 ? mr[c0] : 0.0;
-          v[n1].y *= in1 ? mr[c0 + 1] : 0.0;
-        }
+        v[n1] = p0[n1];
+        if (fu.mul) { v[n1].x *= p1[n1].x; v[n1].y *= p1[n1].y; }
       } else {
         // Z_k = Xa_k + i Xb_k from the packed spectra (X_{N-k} = conj X_k)
-        const int kk = k <= N / 2 ? k : N - k;
-        const int pr = kk == 0 ? 0 : (kk == N / 2 ? 1 : 2 * kk);
-        const double* re = in + (size_t) pr * S;
-        const double* im = in + (size_t) (2 * kk + 1) * S;
-        const bool cplx = kk != 0 && kk != N / 2;
-        const double ar = in0 ? re[c0] : 0.0, br = in1 ? re[c0 + 1] : 0.0;
-        double ai = cplx && in0 ? im[c0] : 0.0, bi = cplx && in1 ? im[c0 + 1] : 0.0;
+        double ai = p1[n1].x, bi = p1[n1].y;
         if (k > N / 2) { ai = -ai; bi = -bi; }
-        v[n1] = make_double2 (ar - bi, ai + br);
-        if (fu.dz != 0.0) {           // times i beta k_s, k_s = k - N above N/2
+        v[n1] = make_double2 (p0[n1].x - bi, ai + p0[n1].y);
+        if (fu.dz != 0.0) {         // times i beta k_s, k_s = k - N above N/2
           const double a = fu.dz * (k < N / 2 ? k : (k == N / 2 ? 0 : k - N));
           v[n1] = make_double2 (-a * v[n1].y, a * v[n1].x);
         }
       }
     }
-    __syncthreads ();                 // twiddles in LDS
     dft_regs<R1> (v, W, N / R1, inv);
 #pragma unroll
     for (int k1 = 0; k1 < R1; k1++) {
-      const int m = n2 * k1;          // < N
+      const int m = n2 * k1;        // < N
       double2 w = m < N / 2 ? W[m] : make_double2 (-W[m - N / 2].x, -W[m - N / 2].y);
       if (inv) w.y = -w.y;
-      T[(k1 * R2 + n2) * NQ + q] = cmul (v[k1], w);
+      T[(k1 * R2 + n2) * NQ + qa] = cmul (v[k1], w);
     }
   }
   __syncthreads ();
@@ -251,7 +269,7 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
   double2 y[R2];
   if (act) {
 #pragma unroll
-    for (int n2 = 0; n2 < R2; n2++) y[n2] = T[(k1 * R2 + n2) * NQ + q];
+    for (int j = 0; j < R2; j++) y[j] = T[(k1 * R2 + j) * NQ + q];
     dft_regs<R2> (y, W, N / R2, inv);
   }
   if (inv) {
@@ -260,6 +278,21 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
       for (int k2 = 0; k2 < R2; k2++) {
         const size_t ro = (size_t) (k1 + R1 * k2) * S;
         double a = y[k2].x, b = y[k2].y;
+        if (V16) {
+          if (c0 >= S) continue;
+          if (fu.acc) {
+            if (fu.amul) {
+              const double2 m = *(const double2*) (fu.amul + ro + c0);
+              a *= m.x;
+              b *= m.y;
+            }
+            const double2 q2 = *(const double2*) (fu.acc + ro + c0);
+            a = q2.x - a;
+            b = q2.y - b;
+          }
+          *(double2*) (out + ro + c0) = make_double2 (a, b);
+          continue;
+        }
         if (fu.acc) {
           if (fu.amul) {
             a *= c0 < S ? fu.amul[ro + c0] : 0.0;
@@ -273,25 +306,42 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
       }
     return;
   }
-  __syncthreads ();                   // every step-B read of T is done
+  __syncthreads ();                 // every step-B read of T is done
   if (act)
 #pragma unroll
     for (int k2 = 0; k2 < R2; k2++) T[(k1 + R1 * k2) * NQ + q] = y[k2];
   __syncthreads ();
   const double scale = fu.scale / N;
-  for (int i = tid; i < N * TS; i += TH) {
-    const int p = i / TS, c = i % TS, s = s0 + c;
-    if (s >= S) continue;
-    const int qq = c >> 1, k = p == 0 ? 0 : (p == 1 ? N / 2 : p >> 1);
-    const double2 zk = T[k * NQ + qq], zr = T[((N - k) & (N - 1)) * NQ + qq];
-    // Xa = (Z_k + conj Z_{N-k}) / 2,  Xb = (Z_k - conj Z_{N-k}) / 2i
-    double re, im;
-    if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
-    else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
-    out[(size_t) p * S + s] = ((p < 2 || (p & 1) == 0) ? re : im) * scale +
-                              (p == 0 ? fu.add0 : 0.0);
+  if (V16) {
+    // one thread per plane and column pair: both columns from Z_k, Z_{N-k}
+    for (int i = tid; i < N * NQ; i += TH) {
+      const int pp = i / NQ, qq = i % NQ, s = s0 + 2 * qq;
+      if (s >= S) continue;
+      const int k = pp == 0 ? 0 : (pp == 1 ? N / 2 : pp >> 1);
+      const double2 zk = T[k * NQ + qq], zr = T[((N - k) & (N - 1)) * NQ + qq];
+      const bool re = pp < 2 || (pp & 1) == 0;
+      const double ve = re ? 0.5 * (zk.x + zr.x) : 0.5 * (zk.y - zr.y);
+      const double vo = re ? 0.5 * (zk.y + zr.y) : -0.5 * (zk.x - zr.x);
+      const double ad = pp == 0 ? fu.add0 : 0.0;
+      *(double2*) (out + (size_t) pp * S + s) = make_double2 (ve * scale + ad, vo * scale + ad);
+    }
+  } else {
+    for (int i = tid; i < N * TS; i += TH) {
+      const int pp = i / TS, c = i % TS, s = s0 + c;
+      if (s >= S) continue;
+      const int qq = c >> 1, k = pp == 0 ? 0 : (pp == 1 ? N / 2 : pp >> 1);
+      const double2 zk = T[k * NQ + qq], zr = T[((N - k) & (N - 1)) * NQ + qq];
+      // Xa = (Z_k + conj Z_{N-k}) / 2,  Xb = (Z_k - conj Z_{N-k}) / 2i
+      double re, im;
+      if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
+      else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
+      out[(size_t) pp * S + s] = ((pp < 2 || (pp & 1) == 0) ? re : im) * scale +
+                                 (pp == 0 ? fu.add0 : 0.0);
+    }
   }
 }
+
+
 
 // Packed DFT matrix product out[k][s] = sum_n M[k][n] in[n][s] (any nz);
 // one thread per column, KG output planes per pass (M rows in LDS).
@@ -378,18 +428,21 @@ int launch_fft2_t (int S, const double* in, double* out, int sign, const double2
                    hipStream_t st, Fused fu)
 {
   const size_t lds = Fft2<N, R1, TH>::lds ();
-  auto kern = k_fft2_z<N, R1, TH>;
+  const int TS = Fft2<N, R1, TH>::TS, ntiles = (S + TS - 1) / TS;
+  // column pairs as 16-byte accesses when every row starts 16-byte aligned
+  auto al = [] (const double* p) { return ((uintptr_t) p & 15) == 0; };
+  const bool v16 = S % 2 == 0 && al (in) && al (out) && al (fu.mul) &&
+                   al (fu.acc) && al (fu.amul);
+  auto kern = v16 ? k_fft2_z<N, R1, TH, true> : k_fft2_z<N, R1, TH, false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  const int TS = Fft2<N, R1, TH>::TS;
-  hipLaunchKernelGGL (kern, dim3 ((S + TS - 1) / TS), dim3 (TH), lds, st, in, out, S,
-                      sign, tw, fu);
+  hipLaunchKernelGGL (kern, dim3 (ntiles), dim3 (TH), lds, st, in, out, S, sign, tw, fu);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
 
 // Threads per transform tile: 128 (16 real columns of 128-byte rows per
-// tile, more and smaller workgroups than 256: 15.2 against 15.6 us for 128
+// tile, more and smaller workgroups than 256: 9.9 against 10.2 us for 128
 // planes of 18,432 columns); SX_FFT_THREADS=256 selects the larger tile.
 int fft_threads ()
 {
