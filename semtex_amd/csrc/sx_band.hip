@@ -355,6 +355,43 @@ __device__ __forceinline__ double lane_bcast (double v, int l)
   return __longlong_as_double (((long long) hi << 32) | (unsigned) lo);
 }
 
+// the sums over the 64 lanes of p[0..7]: lane l returns the sum of p[l >> 3]
+// (a halving exchange: across lane halves, quarters and eighths each lane
+// keeps half of its values and adds its partner's, then the eight lanes
+// holding one value sum it)
+__device__ __forceinline__ double lane_reduce8 (const double (&p)[8], int lane)
+{
+  const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
+  double a[4], b[2];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+    a[j] = (h32 ? p[j + 4] : p[j]) + __shfl_xor (h32 ? p[j] : p[j + 4], 32);
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+    b[j] = (h16 ? a[j + 2] : a[j]) + __shfl_xor (h16 ? a[j] : a[j + 2], 16);
+  double c = (h8 ? b[1] : b[0]) + __shfl_xor (h8 ? b[0] : b[1], 8);
+  c += __shfl_xor (c, 4);
+  c += __shfl_xor (c, 2);
+  c += __shfl_xor (c, 1);
+  return c;
+}
+
+// the same for p[0..3]: lane l returns the sum of p[l >> 4]
+__device__ __forceinline__ double lane_reduce4 (const double (&p)[4], int lane)
+{
+  const bool h32 = lane & 32, h16 = lane & 16;
+  double a[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++)
+    a[j] = (h32 ? p[j + 2] : p[j]) + __shfl_xor (h32 ? p[j] : p[j + 2], 32);
+  double c = (h16 ? a[1] : a[0]) + __shfl_xor (h16 ? a[0] : a[1], 16);
+  c += __shfl_xor (c, 8);
+  c += __shfl_xor (c, 4);
+  c += __shfl_xor (c, 2);
+  c += __shfl_xor (c, 1);
+  return c;
+}
+
 __device__ __forceinline__ size_t pidx (int t, int c)
 {
   return ((size_t) (t >> 1) * PB + c) * 2 + (t & 1);
@@ -444,6 +481,7 @@ enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
 // congruent mod 8, i.e. the same XCD and L2.  False: nothing to do.
 struct SweepSeg {
   int mode, n, f0, f1, b0, b1, rs, role, chain, grp;
+  int l0, l1;                        // left-looking backward: blocks [l0, l1), descending
   bool wf, wb;
   size_t poff;
   double* bg;
@@ -466,29 +504,29 @@ __device__ __forceinline__ bool sweep_seg (SweepSeg& g, int NR, int D,
   g.grp = grp;
   g.chain = (piece * nmodes + mode) * G + grp;
   g.mode = mode;
-  g.f0 = g.f1 = g.b0 = g.b1 = 0;
+  g.f0 = g.f1 = g.b0 = g.b1 = g.l0 = g.l1 = 0;
   g.wf = g.wb = false;
   g.poff = 0;
   g.bg = rhs + ((size_t) mode * G + grp) * NR * nmax;
   g.rs = nmax;
   if (stage == 0) {
     g.n = nsolve[mode];
-    g.f1 = g.b1 = (g.n + PB - 1) / PB;
+    g.f1 = g.b1 = g.l1 = (g.n + PB - 1) / PB;
   } else {
     const int* t = tw + TW * mode;
     const int nbT = t[TW_NBT], nbB = t[TW_NBB];
     if (piece == 0) {
       g.n = t[TW_NTOP];
       if (stage == 1) { g.f1 = nbT; g.wf = true; }
-      if (stage == 2) { g.f0 = nbT; g.f1 = nbT + D; g.b1 = D; g.wb = true; }
-      if (stage == 3) { g.b0 = D; g.b1 = nbT + D; }
+      if (stage == 2) { g.f0 = nbT; g.f1 = nbT + D; g.b1 = D; g.wb = true; g.l0 = nbT; g.l1 = nbT + D; }
+      if (stage == 3) { g.b0 = D; g.b1 = nbT + D; g.l1 = nbT; }
     } else {
       g.n = t[TW_NBOT];
       g.poff = boff;
       g.bg = rhsb + ((size_t) mode * G + grp) * NR * nmaxb;
       g.rs = nmaxb;
       if (stage == 1) { g.f1 = nbB; g.wf = true; }
-      if (stage == 3) g.b1 = nbB + D;
+      if (stage == 3) { g.b1 = nbB + D; g.l1 = nbB; }
     }
   }
   return g.n > 0;
@@ -875,7 +913,146 @@ int launch_panel (const sx_msys* s, int G, int stage, hipStream_t st)
 constexpr int UW = 8;                               // wavefronts per workgroup
 constexpr int UPW = 4;                              // row pairs per wave and block
 
+// Left-looking backward sweep on the forward panels.  With the forward
+// sweep's y_J = T_J^T z_J in b, V x = (T_J y_J)_J is solved block by block
+// from the last:
+//   x_J = T_J y_J - sum_{d=1..D} W_{J,J+d} x_{J+d},   J = l1-1 .. l0,
+// reading the same panels as the forward sweep (block 0 of panel J is T_J,
+// block d is W_{J,J+d}), in reverse order, so a backward sweep that follows
+// its forward sweep finds the recently read panels in the memory-side cache.
+// Wave w, lane c forms for its rows t = 8w..8w+7 the terms of column c,
+// T_J(t,c) y_J[c] - sum_d W_{J,J+d}(t,c) x_{J+d}[c], and the lanes sum them
+// (lane_reduce8); x_J goes to b and to a ring of D+1 blocks in LDS, from
+// which every lane reads x_K[c] of the blocks it multiplies.  Blocks at or
+// past l1 enter from b (x already known there: the separator of a twisted
+// solve), and past the piece's end they are zero.  Panels two steps ahead
+// stay in registers as in the forward sweep.
 template <int NR, int D>
+__device__ __forceinline__ void unit_backward_ll (
+    const double* P, int n, int nb, double* bg, int rs, int l0, int l1, int lane, int w, int kd,
+    double* hist, double2 (&ua)[UPW * (D + 1)], double2 (&ub)[UPW * (D + 1)], double (&ya)[NR],
+    double (&yb)[NR])
+{
+  if (l0 >= l1) return;
+  const size_t PS = (size_t) (D + 1) * PB * PB;
+  const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc (
+      (void*) P, (short) 0, (int) (sizeof (double) * (size_t) (nb + 1) * PS), 0x00020000);
+  constexpr int OOB = 0x7f000000;
+  auto pld = [&] (int voff, int soff) __attribute__ ((always_inline)) {
+    typedef int v4i __attribute__ ((ext_vector_type (4)));
+    const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (prs, voff, soff, 0);
+    return __builtin_bit_cast (double2, t);
+  };
+  auto vbase = [&] (int Jn) __attribute__ ((always_inline)) {
+    return Jn >= l0 && Jn <= nb
+               ? (int) (sizeof (double) * (size_t) Jn * PS) + 1024 * UPW * w + 16 * lane
+               : OOB;
+  };
+  const bool dneed = lane <= kd - PB * D + PB - 1;
+  auto pvoff = [&] (int vb, int d, int k) __attribute__ ((always_inline)) {
+    if (d == 0) return lane >= 8 * w + 2 * k ? vb : OOB;
+    if (d == D) return dneed ? vb : OOB;
+    return vb;
+  };
+  auto hist_at = [&] (int K, int row) __attribute__ ((always_inline)) {
+    return hist + ((size_t) (K % (D + 1)) * PB + row) * NR;
+  };
+  auto yload = [&] (int J, double (&yq)[NR]) __attribute__ ((always_inline)) {
+    const int j = J * PB + lane;
+#pragma unroll
+    for (int r = 0; r < NR; r++) yq[r] = J >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
+  };
+  // history: blocks l1 .. l1+D-1 (zero past n), one wave per block
+  if (w < D) {
+    const int K = l1 + w, j = K * PB + lane;
+#pragma unroll
+    for (int r = 0; r < NR; r++) hist_at (K, lane)[r] = j < n ? bg[(size_t) r * rs + j] : 0.0;
+  }
+  {
+    const int v0 = vbase (l1 - 1), v1 = vbase (l1 - 2);
+#pragma unroll
+    for (int k = 0; k < UPW; k++)
+#pragma unroll
+      for (int d = 0; d <= D; d++) ua[k * (D + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+    yload (l1 - 1, ya);
+#pragma unroll
+    for (int k = 0; k < UPW; k++)
+#pragma unroll
+      for (int d = 0; d <= D; d++) ub[k * (D + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
+    yload (l1 - 2, yb);
+  }
+  __builtin_amdgcn_s_waitcnt (0);
+  __syncthreads ();
+
+  auto step = [&] (int J, double2 (&u)[UPW * (D + 1)], double (&yq)[NR]) __attribute__ ((always_inline)) {
+    // two halves of the wave's rows (pairs 2h, 2h+1), block by block: the
+    // terms of T_J (d = 0) and of W_{J,J+d} x_{J+d}, then the lane sums
+    const int vb = vbase (J - 2);
+    double xo[2][NR];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      double p[NR][4];
+#pragma unroll
+      for (int d = 0; d <= D; d++) {
+        double xd[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) xd[r] = d == 0 ? yq[r] : -hist_at (J + d, lane)[r];
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+          const int k = 2 * h + kk;
+          const double2 v = u[k * (D + 1) + d];
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            if (d == 0) {
+              p[r][2 * kk] = v.x * xd[r];
+              p[r][2 * kk + 1] = v.y * xd[r];
+            } else {
+              p[r][2 * kk] = fma (v.x, xd[r], p[r][2 * kk]);
+              p[r][2 * kk + 1] = fma (v.y, xd[r], p[r][2 * kk + 1]);
+            }
+          }
+          int so = 32768 * d + 1024 * k;
+#pragma unroll
+          for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(p[r][2 * kk + 1]));
+          u[k * (D + 1) + d] = pld (pvoff (vb, d, k), so);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NR; r++) xo[h][r] = lane_reduce4 (p[r], lane);
+    }
+    // y of the block two steps on, loaded after this one's use
+    {
+      int off = 0;
+#pragma unroll
+      for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(xo[1][r]), "v"(yq[r]));
+      const int j = (J - 2) * PB + lane + off;
+#pragma unroll
+      for (int r = 0; r < NR; r++) yq[r] = J - 2 >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
+    }
+    // row 8w + 4h + lane/16 of x_J, from the first lane of its 16
+    if ((lane & 15) == 0) {
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int row = 8 * w + 4 * h + (lane >> 4), j = J * PB + row;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          hist_at (J, row)[r] = xo[h][r];
+          if (j < n) bg[(size_t) r * rs + j] = xo[h][r];
+        }
+      }
+    }
+    lds_sync ();
+  };
+
+  int J = l1 - 1;
+  for (; J - 1 >= l0; J -= 2) {
+    step (J, ua, ya);
+    step (J - 1, ub, yb);
+  }
+  if (J >= l0) step (J, ua, ya);
+}
+
+template <int NR, int D, bool LL = false>
 __device__ __forceinline__ void k_unit_sweeps_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
@@ -898,6 +1075,11 @@ __device__ __forceinline__ void k_unit_sweeps_body (
   double ba[NR], bb[NR];
 
   for (int pass = 0; pass < 2; pass++) {
+    if (LL && pass == 1) {
+      unit_backward_ll<NR, D> (PF + (size_t) g.mode * pstride + g.poff, n, nb, bg, rs, g.l0,
+                               g.l1, lane, w, kd, ring, ua, ub, ba, bb);
+      continue;
+    }
     const int J0 = pass ? g.b0 : g.f0, J1 = pass ? g.b1 : g.f1;
     const bool wout = pass ? g.wb : g.wf;
     if (J0 >= J1) continue;
@@ -1069,13 +1251,13 @@ __device__ __forceinline__ void k_unit_sweeps_body (
   }
 }
 
-template <int NR, int D>
+template <int NR, int D, bool LL>
 __global__ __launch_bounds__(UW * 64) void k_unit_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
     const int* __restrict__ tw, int stage, size_t boff, double* rhsb, int nmaxb, int kd)
 {
-  k_unit_sweeps_body<NR, D> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, kd);
+  k_unit_sweeps_body<NR, D, LL> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb, nmaxb, kd);
 }
 
 // ---------------------------------------------------------------------------
@@ -1173,7 +1355,7 @@ template <int D, int R> __host__ __device__ constexpr int sp_n (int r)
   return base + (r - 1 < extra ? 1 : 0);
 }
 
-template <int NR, int D, int R>
+template <int NR, int D, int R, bool LL = false>
 __device__ __forceinline__ void split_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
@@ -1255,6 +1437,277 @@ __device__ __forceinline__ void split_body (
   };
 
   for (int pass = 0; pass < 2; pass++) {
+    if (LL && pass == 1) {
+      // Left-looking backward (unit_backward_ll) on two workgroups: the
+      // near one forms x_J = T_J y_J - sum_{d<=H} W_{J,J+d} x_{J+d} - f_J and
+      // publishes x_J (one record per lane, direction 0); the far one forms
+      // f_K = sum_{d>H} W_{K,K+d} x_{K+d} as soon as x_{K+H+1} is known and
+      // publishes it by row (direction 1), H steps before the near one needs
+      // it.  Blocks at or past l1 are known (b); the separator's solution of
+      // a twisted solve also goes to the bottom piece's vector.
+      const int l0 = g.l0, l1 = g.l1;
+      if (l0 >= l1) break;
+      const double* P = PF + (size_t) g.mode * pstride + g.poff;
+      const __amdgpu_buffer_rsrc_t prs = __builtin_amdgcn_make_buffer_rsrc (
+          (void*) P, (short) 0, (int) (sizeof (double) * (size_t) (nb + 1) * PS), 0x00020000);
+      constexpr int OOB = 0x7f000000;
+      auto pld = [&] (int voff, int soff) __attribute__ ((always_inline)) {
+        typedef int v4i __attribute__ ((ext_vector_type (4)));
+        const v4i t = __builtin_amdgcn_raw_buffer_load_b128 (prs, voff, soff, 0);
+        return __builtin_bit_cast (double2, t);
+      };
+      auto vbase = [&] (int Jn) __attribute__ ((always_inline)) {
+        return Jn >= l0 && Jn <= nb
+                   ? (int) (sizeof (double) * (size_t) Jn * PS) + 1024 * UPW * w + 16 * lane
+                   : OOB;
+      };
+      const bool dneed = lane <= kd - PB * D + PB - 1;
+      auto pvoff = [&] (int vb, int d, int k) __attribute__ ((always_inline)) {
+        if (d == 0) return lane >= 8 * w + 2 * k ? vb : OOB;
+        if (d == D) return dneed ? vb : OOB;
+        return vb;
+      };
+      // x of block K >= l1 (known), lane c
+      auto xknown = [&] (int K, int r) __attribute__ ((always_inline)) {
+        const int j = K * PB + lane;
+        return j < n ? bg[(size_t) r * rs + j] : 0.0;
+      };
+      if (role == 0) {
+        constexpr int RE = H + 1;                    // x of blocks J..J+H
+        double* hist = sm;                           // [RE][64][NR]
+        auto hist_at = [&] (int K, int row) __attribute__ ((always_inline)) {
+          return hist + ((size_t) (K % RE) * PB + row) * NR;
+        };
+        auto yload = [&] (int J, double (&yq)[NR]) __attribute__ ((always_inline)) {
+          const int j = J * PB + lane;
+#pragma unroll
+          for (int r = 0; r < NR; r++) yq[r] = J >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
+        };
+        if (w < H) {
+          const int K = l1 + w;
+#pragma unroll
+          for (int r = 0; r < NR; r++) hist_at (K, lane)[r] = xknown (K, r);
+        }
+        {
+          const int v0 = vbase (l1 - 1), v1 = vbase (l1 - 2);
+#pragma unroll
+          for (int k = 0; k < UPW; k++)
+#pragma unroll
+            for (int d = 0; d <= H; d++)
+              ua[k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+          yload (l1 - 1, ba);
+#pragma unroll
+          for (int k = 0; k < UPW; k++)
+#pragma unroll
+            for (int d = 0; d <= H; d++)
+              ub[k * (H + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
+          yload (l1 - 2, bb);
+        }
+        __builtin_amdgcn_s_waitcnt (0);
+        __syncthreads ();
+        // f_J of row 8w + 4 ((lane >> 3) & 1) + (lane >> 4): the row whose sum
+        // lane_reduce4 leaves in lane (lane & 15) == 0 (first half) or 8
+        // (second half) of the lane's sixteen
+        auto foff = [&] (int J) __attribute__ ((always_inline)) {
+          return xoff (1, J) - 16 * lane + 16 * (8 * w + 4 * ((lane >> 3) & 1) + (lane >> 4));
+        };
+        auto step = [&] (int J, double2 (&u)[NP], double (&yq)[NR]) __attribute__ ((always_inline)) {
+          // f_J's records read before this step's panel reloads: waiting for
+          // them below waits for no reload
+          typedef int v4i __attribute__ ((ext_vector_type (4)));
+          v4i fr[NR];
+#pragma unroll
+          for (int r = 0; r < NR; r++)
+            fr[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, foff (J) + r * MS, 0, 17);
+          const int vb = vbase (J - 2);
+          double xo[2][NR];
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            double pp[NR][4];
+#pragma unroll
+            for (int d = 0; d <= H; d++) {
+              double xd[NR];
+#pragma unroll
+              for (int r = 0; r < NR; r++) xd[r] = d == 0 ? yq[r] : -hist_at (J + d, lane)[r];
+#pragma unroll
+              for (int kk = 0; kk < 2; kk++) {
+                const int k = 2 * h + kk;
+                const double2 v = u[k * (H + 1) + d];
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                  if (d == 0) {
+                    pp[r][2 * kk] = v.x * xd[r];
+                    pp[r][2 * kk + 1] = v.y * xd[r];
+                  } else {
+                    pp[r][2 * kk] = fma (v.x, xd[r], pp[r][2 * kk]);
+                    pp[r][2 * kk + 1] = fma (v.y, xd[r], pp[r][2 * kk + 1]);
+                  }
+                }
+                int so = 32768 * d + 1024 * k;
+#pragma unroll
+                for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(pp[r][2 * kk + 1]));
+                u[k * (H + 1) + d] = pld (pvoff (vb, d, k), so);
+              }
+            }
+#pragma unroll
+            for (int r = 0; r < NR; r++) xo[h][r] = lane_reduce4 (pp[r], lane);
+          }
+          {
+            int off = 0;
+#pragma unroll
+            for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(xo[1][r]), "v"(yq[r]));
+            const int j = (J - 2) * PB + lane + off;
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              yq[r] = J - 2 >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
+          }
+          // f_J: the early records when every lane's tag matches, else wait
+          double fv[NR];
+          {
+            bool ok = !dead;
+            if (ok) {
+              bool lane_ok = true;
+#pragma unroll
+              for (int r = 0; r < NR; r++) {
+                fv[r] = __longlong_as_double (((long long) fr[r].y << 32) | (unsigned) fr[r].x);
+                lane_ok = lane_ok && (unsigned) fr[r].z == ep && (unsigned) fr[r].w == seq (1, J);
+              }
+              ok = __builtin_amdgcn_ballot_w64 (!lane_ok) == 0;
+            }
+            if (!ok) msg_wait<NR> (xrs, foff (J), MS, ep, seq (1, J), fv, dead, spin);
+          }
+          const int hw = (lane & 15) == 0 ? 0 : ((lane & 15) == 8 ? 1 : -1);
+          if (hw >= 0) {
+            const int row = 8 * w + 4 * hw + (lane >> 4), j = J * PB + row;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+              const double x = (hw ? xo[1][r] : xo[0][r]) - fv[r];
+              hist_at (J, row)[r] = x;
+              if (j < n) {
+                bg[(size_t) r * rs + j] = x;
+                if (rbS) rbS[(size_t) r * nmaxb + (tnb - 1 - (j - tm))] = x;
+              }
+            }
+          }
+          lds_sync ();
+          if (w == 0)
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              msg_put (xrs, xoff (0, J) + r * MS, hist_at (J, lane)[r], ep, seq (1, J));
+        };
+        int J = l1 - 1;
+        for (; J - 1 >= l0; J -= 2) {
+          step (J, ua, ba);
+          step (J - 1, ub, bb);
+        }
+        if (J >= l0) step (J, ua, ba);
+      } else {
+        constexpr int FB = D - H;                    // far blocks H+1..D
+        constexpr int RE = FB + 1;                   // x of blocks K+H+1..K+D
+        double* hist = sm;                           // [RE][64][NR]
+        auto hist_at = [&] (int K) __attribute__ ((always_inline)) {
+          return hist + ((size_t) (K % RE) * PB + lane) * NR;
+        };
+        auto fvoff = [&] (int vb, int dd, int k) __attribute__ ((always_inline)) {
+          return dd < FB ? pvoff (vb, H + 1 + dd, k) : OOB;
+        };
+        // the near side must be done with the previous pass's records
+        if (ran) {
+          double dv[NR];
+          const int Jp = g.f1;
+          msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (0, Jp), dv, dead, spin);
+        }
+        {
+          const int v0 = vbase (l1 - 1), v1 = vbase (l1 - 2);
+#pragma unroll
+          for (int k = 0; k < UPW; k++)
+#pragma unroll
+            for (int dd = 0; dd < FBM; dd++)
+              ua[k * FBM + dd] = pld (fvoff (v0, dd, k), 32768 * (H + 1 + dd) + 1024 * k);
+#pragma unroll
+          for (int k = 0; k < UPW; k++)
+#pragma unroll
+            for (int dd = 0; dd < FBM; dd++)
+              ub[k * FBM + dd] = pld (fvoff (v1, dd, k), 32768 * (H + 1 + dd) + 1024 * k);
+        }
+        __builtin_amdgcn_s_waitcnt (0);
+        __syncthreads ();
+        auto step = [&] (int K, double2 (&u)[NP]) __attribute__ ((always_inline)) {
+          const int Kx = K + H + 1;
+          // x_{K+H+1}'s records read before this step's panel reloads; the
+          // older blocks' products run first
+          typedef int v4i __attribute__ ((ext_vector_type (4)));
+          v4i xr[NR];
+          if (Kx < l1)
+#pragma unroll
+            for (int r = 0; r < NR; r++)
+              xr[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, xoff (0, Kx) + r * MS, 0, 17);
+          const int vb = vbase (K - 2);
+          double pp[NR][2 * UPW];
+#pragma unroll
+          for (int dd = FB - 1; dd >= 0; dd--) {
+            double xd[NR];
+            if (dd > 0) {
+#pragma unroll
+              for (int r = 0; r < NR; r++)
+                xd[r] = Kx + dd < l1 ? hist_at (Kx + dd)[r] : xknown (Kx + dd, r);
+            } else if (Kx < l1) {
+              bool ok = !dead;
+              if (ok) {
+                bool lane_ok = true;
+#pragma unroll
+                for (int r = 0; r < NR; r++) {
+                  xd[r] = __longlong_as_double (((long long) xr[r].y << 32) | (unsigned) xr[r].x);
+                  lane_ok = lane_ok && (unsigned) xr[r].z == ep && (unsigned) xr[r].w == seq (1, Kx);
+                }
+                ok = __builtin_amdgcn_ballot_w64 (!lane_ok) == 0;
+              }
+              if (!ok) msg_wait<NR> (xrs, xoff (0, Kx), MS, ep, seq (1, Kx), xd, dead, spin);
+              if (w == 0)
+#pragma unroll
+                for (int r = 0; r < NR; r++) hist_at (Kx)[r] = xd[r];
+            } else {
+#pragma unroll
+              for (int r = 0; r < NR; r++) xd[r] = xknown (Kx, r);
+            }
+#pragma unroll
+            for (int k = 0; k < UPW; k++) {
+              const double2 v = u[k * FBM + dd];
+#pragma unroll
+              for (int r = 0; r < NR; r++) {
+                if (dd == FB - 1) {
+                  pp[r][2 * k] = v.x * xd[r];
+                  pp[r][2 * k + 1] = v.y * xd[r];
+                } else {
+                  pp[r][2 * k] = fma (v.x, xd[r], pp[r][2 * k]);
+                  pp[r][2 * k + 1] = fma (v.y, xd[r], pp[r][2 * k + 1]);
+                }
+              }
+              int so = 32768 * (H + 1 + dd) + 1024 * k;
+#pragma unroll
+              for (int r = 0; r < NR; r++) asm ("" : "+s"(so) : "v"(pp[r][2 * k + 1]));
+              u[k * FBM + dd] = pld (fvoff (vb, dd, k), so);
+            }
+          }
+          // f_K row t to record slot t
+#pragma unroll
+          for (int r = 0; r < NR; r++) {
+            const double f = lane_reduce8 (pp[r], lane);
+            if ((lane & 7) == 0)
+              msg_put (xrs, xoff (1, K) - 16 * lane + 16 * (8 * w + (lane >> 3)) + r * MS, f, ep,
+                       seq (1, K));
+          }
+          lds_sync ();
+        };
+        int K = l1 - 1;
+        for (; K - 1 >= l0; K -= 2) {
+          step (K, ua);
+          step (K - 1, ub);
+        }
+        if (K >= l0) step (K, ua);
+      }
+      break;
+    }
     const int J0 = pass ? g.b0 : g.f0, J1 = pass ? g.b1 : g.f1;
     const bool wout = pass ? g.wb : g.wf;
     if (J0 >= J1) continue;
@@ -1577,7 +2030,7 @@ __device__ __forceinline__ void split_body (
 // always use one epoch (also when one of them starts only after its partner
 // has given up and finished), and a record written under an earlier launch
 // of any layout never carries the current epoch.
-template <int NR, int D, int R>
+template <int NR, int D, int R, bool LL>
 __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
@@ -1590,7 +2043,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
   // would make every workgroup invalidate or write back its L2 at its end)
   const unsigned ep = __hip_atomic_load (sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   bool dead = false;
-  split_body<NR, D, R> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
+  split_body<NR, D, R, LL> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
                      nmaxb, kd, xch, ep, spin, dead);
   const int any_dead = __syncthreads_or (dead ? 1 : 0);
   if (threadIdx.x == 0) {
@@ -1632,11 +2085,17 @@ __global__ __launch_bounds__(256) void k_panel_unit (double* P, size_t pstride, 
 
 bool unit_ok (int D) { return D >= 3 && D <= 5; }
 
+bool band_ll ()
+{
+  static const bool ll = [] { const char* e = std::getenv ("SX_BAND_LL"); return !e || std::atoi (e); } ();
+  return ll;
+}
+
 template <int NR, int D>
 int launch_unit_d (const sx_msys* s, int G, int stage, hipStream_t st)
 {
   const size_t lds = sizeof (double) * ((size_t) (D + 3) * UW * PB * NR);
-  auto kern = k_unit_sweeps<NR, D>;
+  auto kern = band_ll () ? k_unit_sweeps<NR, D, true> : k_unit_sweeps<NR, D, false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
@@ -1681,13 +2140,13 @@ size_t split_lds ()
 // the kernel uses scratch (private memory): the occupancy query does not
 // count the runtime's limit on waves holding scratch at once, so a grid it
 // reports as fitting could leave a partner workgroup waiting for scratch
-template <int NR, int D, int R>
+template <int NR, int D, int R, bool LL>
 int split_capacity ()
 {
   static int cap = -1;
   if (cap < 0) {
     int dev = 0, cus = 0, per = 0;
-    auto kern = k_split_sweeps<NR, D, R>;
+    auto kern = k_split_sweeps<NR, D, R, LL>;
     const int lds = (int) split_lds<NR, D, R> ();
     hipFuncAttributes fa;
     if (hipGetDevice (&dev) != hipSuccess ||
@@ -1709,9 +2168,12 @@ int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool pro
 {
   const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
   const int nblocks = (s->nmodes + 7) / 8 * 8 * G * R * pieces;
-  if (probe) return nblocks <= split_capacity<NR, D, R> () ? SX_OK : SX_ERR_LIMIT;
+  const bool ll = band_ll ();
+  if (probe)
+    return nblocks <= (ll ? split_capacity<NR, D, R, true> () : split_capacity<NR, D, R, false> ())
+               ? SX_OK : SX_ERR_LIMIT;
   const size_t lds = split_lds<NR, D, R> ();
-  auto kern = k_split_sweeps<NR, D, R>;
+  auto kern = ll ? k_split_sweeps<NR, D, R, true> : k_split_sweeps<NR, D, R, false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
@@ -1877,15 +2339,21 @@ static bool twist_split (const sx_msys* s, std::vector<int>& tw)
 
 // Panels of one piece: T_J and the off-diagonal blocks of the factor U
 // (band `src`, per-mode sizes `d_n`) and of its reversal, into P + off.
+// the backward sweeps read a reversed copy of the panels unless they are
+// the left-looking sweeps on the forward panels (unit-diagonal systems)
+static bool reversed_panels (const sx_msys* s) { return !(s->sw_unit && band_ll ()); }
+
 static int build_panels (sx_msys* s, const double* src, double* V, size_t stride,
                          const int* d_n, int nbp, size_t off, hipStream_t st)
 {
   const int ld = s->m->nband, D = s->sw_D;
-  hipLaunchKernelGGL (k_band_reverse, dim3 ((unsigned) ((stride + 255) / 256), s->nmodes),
-                      dim3 (256), 0, st, src, V, stride, ld, d_n);
+  const bool rev = reversed_panels (s);
+  if (rev)
+    hipLaunchKernelGGL (k_band_reverse, dim3 ((unsigned) ((stride + 255) / 256), s->nmodes),
+                        dim3 (256), 0, st, src, V, stride, ld, d_n);
   const double* from[2] = {src, V};
-  double* to[2] = {s->d_swF + off, s->d_swR + off};
-  for (int k = 0; k < 2; k++) {
+  double* to[2] = {s->d_swF + off, rev ? s->d_swR + off : nullptr};
+  for (int k = 0; k < (rev ? 2 : 1); k++) {
     hipLaunchKernelGGL (k_panel_tinv, dim3 (nbp, s->nmodes), dim3 (64), 0, st, from[k],
                         stride, ld, d_n, to[k], s->sw_stride, D);
     const size_t tot = (size_t) nbp * D * PB * PB;
@@ -1949,7 +2417,7 @@ static int band_factor_twisted (sx_msys* s, const std::vector<int>& tw, hipStrea
   s->tw_boff = (size_t) (nbtm + 1) * (D + 1) * PB * PB;
   s->sw_stride = s->tw_boff + (size_t) (nbbm + 1) * (D + 1) * PB * PB;
   SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * nm * s->sw_stride));
-  SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * nm * s->sw_stride));
+  if (reversed_panels (s)) SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * nm * s->sw_stride));
   if (int rc = build_panels (s, s->d_bandU, V, s->band_stride, d_ntop, nbtm + 1, 0, st))
     return rc;
   SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * nm * vstride, st));
@@ -1977,28 +2445,33 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
   SX_HIP (hipGetLastError ());
   const int B = panels ? PB : band_block (ld);
   const int nblk = (s->nmax + B - 1) / B;
+  if (panels) {
+    s->sw_unit = use_unit (panel_D (ld));
+  }
+  const bool rev = !panels || reversed_panels (s);
   double* V = nullptr;
-  SX_HIP (hipMalloc (&V, sizeof (double) * s->nmodes * s->band_stride));
-  SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * s->nmodes * s->band_stride, st));
-  hipLaunchKernelGGL (k_band_reverse,
-                      dim3 ((unsigned) ((s->band_stride + 255) / 256), s->nmodes),
-                      dim3 (256), 0, st, s->d_bandU, V, s->band_stride, ld, s->d_nsolve);
+  if (rev) {
+    SX_HIP (hipMalloc (&V, sizeof (double) * s->nmodes * s->band_stride));
+    SX_HIP (hipMemsetAsync (V, 0, sizeof (double) * s->nmodes * s->band_stride, st));
+    hipLaunchKernelGGL (k_band_reverse,
+                        dim3 ((unsigned) ((s->band_stride + 255) / 256), s->nmodes),
+                        dim3 (256), 0, st, s->d_bandU, V, s->band_stride, ld, s->d_nsolve);
+  }
   if (panels) {
     const int D = panel_D (ld);
     s->sw_panel = 1;
     s->sw_D = D;
-    s->sw_unit = use_unit (D);
     if (s->sw_unit)
       if (int rc = split_alloc (s, st)) return rc;
     s->sw_block = PB;
     s->sw_stride = band_sweep_bytes_per_mode (s->nmax, ld);
-      SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
-    SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * s->nmodes * s->sw_stride));
+    SX_HIP (hipMalloc (&s->d_swF, sizeof (double) * s->nmodes * s->sw_stride));
+    if (rev) SX_HIP (hipMalloc (&s->d_swR, sizeof (double) * s->nmodes * s->sw_stride));
     const double* src[2] = {s->d_bandU, V};
     double* dst[2] = {s->d_swF, s->d_swR};
     // panels 0..nblk (per-mode nb <= nblk; everything past a mode's own
     // blocks is zero, including its zero panel at index nb)
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < (rev ? 2 : 1); k++) {
       hipLaunchKernelGGL (k_panel_tinv, dim3 (nblk + 1, s->nmodes), dim3 (64), 0, st, src[k],
                           s->band_stride, ld, s->d_nsolve, dst[k], s->sw_stride, D);
       const size_t tot = (size_t) (nblk + 1) * D * PB * PB;
@@ -2153,9 +2626,9 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
 // solves run side by side, so no launch may need its whole grid resident)
 // ---------------------------------------------------------------------------
 namespace {
-template <int NR, int D> struct FnUnit {
+template <int NR, int D, bool LL> struct FnUnit {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
-  { k_unit_sweeps_body<NR, D> (a...); }
+  { k_unit_sweeps_body<NR, D, LL> (a...); }
 };
 template <int NR, int D> struct FnPanel {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
@@ -2175,11 +2648,15 @@ void plan_unit_d (BatchPlan& P, const sx_msys* s, int G, int stage)
   const size_t lds = sizeof (double) * ((size_t) (D + 3) * UW * PB * NR);
   const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
   const int nblocks = (s->nmodes + 7) / 8 * 8 * G * pieces;
-  batch_rec<UW * 64, FnUnit<NR, D>> (P, dim3 (nblocks), UW * 64, lds,
-                                     (const double*) s->d_swF, (const double*) s->d_swR,
-                                     s->sw_stride, (const int*) s->d_nsolve, s->d_rhs, s->nmax,
-                                     s->nmodes, G, (const int*) s->d_tw, stage, s->tw_boff,
-                                     s->d_rhsb, s->nmaxb, s->m->nband - 1);
+  auto rec = [&] (auto fn) {
+    batch_rec<UW * 64, decltype (fn)> (P, dim3 (nblocks), UW * 64, lds,
+                                       (const double*) s->d_swF, (const double*) s->d_swR,
+                                       s->sw_stride, (const int*) s->d_nsolve, s->d_rhs, s->nmax,
+                                       s->nmodes, G, (const int*) s->d_tw, stage, s->tw_boff,
+                                       s->d_rhsb, s->nmaxb, s->m->nband - 1);
+  };
+  if (band_ll ()) rec (FnUnit<NR, D, true> {});
+  else rec (FnUnit<NR, D, false> {});
 }
 
 template <int NR, int D>
