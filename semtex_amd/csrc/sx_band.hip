@@ -2526,9 +2526,10 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
     if (per < 1 || nr % per) per = nr;
     // Split sweeps (two workgroups per chain) for twisted systems, where the
     // pieces leave the chain short: the right-hand sides per workgroup
-    // (1, 2 or 3) bring the grid closest to 128 workgroups, the count past
-    // which the sweeps measured slower (memory-system bound), among the
-    // launches that fit on the device at once.  SX_BAND_SPLIT=0/1 in the
+    // (1, 2 or 3) bring the grid closest to one workgroup per CU (256),
+    // among the launches that fit on the device at once (with the
+    // left-looking backward sweeps 256 workgroups measured faster than 128:
+    // 0.208 against 0.231 ms at 32 modes).  SX_BAND_SPLIT=0/1 in the
     // environment forces them off/on (on: also for single-piece systems).
     int sper = 0;
     if (s->sw_unit && s->d_xch && nr <= XMAXR && !s->no_split) {
@@ -2543,7 +2544,7 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
           const int fits = c == 1 ? launch_split<1> (s, nr, stg, st, true)
                          : c == 2 ? launch_split<2> (s, nr / 2, stg, st, true)
                                   : launch_split<3> (s, nr / 3, stg, st, true);
-          const int dist = std::abs (slots1 * (nr / c) * SPLIT_ROLES - 128);
+          const int dist = std::abs (slots1 * (nr / c) * SPLIT_ROLES - 256);
           if (fits == SX_OK && dist <= best) { best = dist; sper = c; }
         }
       }
