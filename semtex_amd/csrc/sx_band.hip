@@ -1355,7 +1355,7 @@ template <int D, int R> __host__ __device__ constexpr int sp_n (int r)
   return base + (r - 1 < extra ? 1 : 0);
 }
 
-template <int NR, int D, int R, bool LL = false>
+template <int NR, int D, int R, bool LL = false, int PD = 2>
 __device__ __forceinline__ void split_body (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
@@ -1402,8 +1402,9 @@ __device__ __forceinline__ void split_body (
   auto seq = [&] (int pass, int K) __attribute__ ((always_inline)) {
     return 0x80000000u | ((unsigned) pass << 24) | (unsigned) K;
   };
-  double2 ua[NP], ub[NP];
-  double ba[NR], bb[NR];
+  // panels of the next PD steps and b of the blocks entering with them
+  double2 ubuf[PD][NP];
+  double bbuf[PD][NR];
   bool ran = false;                                  // a previous pass ran
   // The next step's records, read at the start of a step before its panel
   // reloads: a wait for them then leaves those reloads in flight (vector
@@ -1489,19 +1490,16 @@ __device__ __forceinline__ void split_body (
           for (int r = 0; r < NR; r++) hist_at (K, lane)[r] = xknown (K, r);
         }
         {
-          const int v0 = vbase (l1 - 1), v1 = vbase (l1 - 2);
 #pragma unroll
-          for (int k = 0; k < UPW; k++)
+          for (int i = 0; i < PD; i++) {
+            const int v0 = vbase (l1 - 1 - i);
 #pragma unroll
-            for (int d = 0; d <= H; d++)
-              ua[k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
-          yload (l1 - 1, ba);
+            for (int k = 0; k < UPW; k++)
 #pragma unroll
-          for (int k = 0; k < UPW; k++)
-#pragma unroll
-            for (int d = 0; d <= H; d++)
-              ub[k * (H + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
-          yload (l1 - 2, bb);
+              for (int d = 0; d <= H; d++)
+                ubuf[i][k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+            yload (l1 - 1 - i, bbuf[i]);
+          }
         }
         __builtin_amdgcn_s_waitcnt (0);
         __syncthreads ();
@@ -1519,7 +1517,7 @@ __device__ __forceinline__ void split_body (
 #pragma unroll
           for (int r = 0; r < NR; r++)
             fr[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, foff (J) + r * MS, 0, 17);
-          const int vb = vbase (J - 2);
+          const int vb = vbase (J - PD);
           double xo[2][NR];
 #pragma unroll
           for (int h = 0; h < 2; h++) {
@@ -1556,10 +1554,10 @@ __device__ __forceinline__ void split_body (
             int off = 0;
 #pragma unroll
             for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(xo[1][r]), "v"(yq[r]));
-            const int j = (J - 2) * PB + lane + off;
+            const int j = (J - PD) * PB + lane + off;
 #pragma unroll
             for (int r = 0; r < NR; r++)
-              yq[r] = J - 2 >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
+              yq[r] = J - PD >= l0 && j < n ? bg[(size_t) r * rs + j] : 0.0;
           }
           // f_J: the early records when every lane's tag matches, else wait
           double fv[NR];
@@ -1596,11 +1594,13 @@ __device__ __forceinline__ void split_body (
               msg_put (xrs, xoff (0, J) + r * MS, hist_at (J, lane)[r], ep, seq (1, J));
         };
         int J = l1 - 1;
-        for (; J - 1 >= l0; J -= 2) {
-          step (J, ua, ba);
-          step (J - 1, ub, bb);
+        for (; J - (PD - 1) >= l0; J -= PD) {
+#pragma unroll
+          for (int i = 0; i < PD; i++) step (J - i, ubuf[i], bbuf[i]);
         }
-        if (J >= l0) step (J, ua, ba);
+#pragma unroll
+        for (int i = 0; i < PD - 1; i++)
+          if (J - i >= l0) step (J - i, ubuf[i], bbuf[i]);
       } else {
         constexpr int FB = D - H;                    // far blocks H+1..D
         constexpr int RE = FB + 1;                   // x of blocks K+H+1..K+D
@@ -1618,17 +1618,15 @@ __device__ __forceinline__ void split_body (
           msg_wait<NR> (xrs, xoff (0, Jp), MS, ep, seq (0, Jp), dv, dead, spin);
         }
         {
-          const int v0 = vbase (l1 - 1), v1 = vbase (l1 - 2);
 #pragma unroll
-          for (int k = 0; k < UPW; k++)
+          for (int i = 0; i < PD; i++) {
+            const int v0 = vbase (l1 - 1 - i);
 #pragma unroll
-            for (int dd = 0; dd < FBM; dd++)
-              ua[k * FBM + dd] = pld (fvoff (v0, dd, k), 32768 * (H + 1 + dd) + 1024 * k);
+            for (int k = 0; k < UPW; k++)
 #pragma unroll
-          for (int k = 0; k < UPW; k++)
-#pragma unroll
-            for (int dd = 0; dd < FBM; dd++)
-              ub[k * FBM + dd] = pld (fvoff (v1, dd, k), 32768 * (H + 1 + dd) + 1024 * k);
+              for (int dd = 0; dd < FBM; dd++)
+                ubuf[i][k * FBM + dd] = pld (fvoff (v0, dd, k), 32768 * (H + 1 + dd) + 1024 * k);
+          }
         }
         __builtin_amdgcn_s_waitcnt (0);
         __syncthreads ();
@@ -1642,7 +1640,7 @@ __device__ __forceinline__ void split_body (
 #pragma unroll
             for (int r = 0; r < NR; r++)
               xr[r] = __builtin_amdgcn_raw_buffer_load_b128 (xrs, xoff (0, Kx) + r * MS, 0, 17);
-          const int vb = vbase (K - 2);
+          const int vb = vbase (K - PD);
           double pp[NR][2 * UPW];
 #pragma unroll
           for (int dd = FB - 1; dd >= 0; dd--) {
@@ -1700,11 +1698,13 @@ __device__ __forceinline__ void split_body (
           lds_sync ();
         };
         int K = l1 - 1;
-        for (; K - 1 >= l0; K -= 2) {
-          step (K, ua);
-          step (K - 1, ub);
+        for (; K - (PD - 1) >= l0; K -= PD) {
+#pragma unroll
+          for (int i = 0; i < PD; i++) step (K - i, ubuf[i]);
         }
-        if (K >= l0) step (K, ua);
+#pragma unroll
+        for (int i = 0; i < PD - 1; i++)
+          if (K - i >= l0) step (K - i, ubuf[i]);
       }
       break;
     }
@@ -1772,25 +1772,19 @@ __device__ __forceinline__ void split_body (
                                                : 0.0;
       }
       {
-        const int v0 = vbase (J0), v1 = vbase (J0 + 1);
 #pragma unroll
-        for (int k = 0; k < UPW; k++)
+        for (int i = 0; i < PD; i++) {
+          const int v0 = vbase (J0 + i);
 #pragma unroll
-          for (int d = 0; d <= H; d++)
-            ua[k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
+          for (int k = 0; k < UPW; k++)
 #pragma unroll
-        for (int r = 0; r < NR; r++)
-          ba[r] = bg[(size_t) r * rs + gidx ((J0 + H) * PB + lane)] +
-                  (pass ? 0.0 : bS (r, gidx ((J0 + H) * PB + lane)));
+            for (int d = 0; d <= H; d++)
+              ubuf[i][k * (H + 1) + d] = pld (pvoff (v0, d, k), 32768 * d + 1024 * k);
 #pragma unroll
-        for (int k = 0; k < UPW; k++)
-#pragma unroll
-          for (int d = 0; d <= H; d++)
-            ub[k * (H + 1) + d] = pld (pvoff (v1, d, k), 32768 * d + 1024 * k);
-#pragma unroll
-        for (int r = 0; r < NR; r++)
-          bb[r] = bg[(size_t) r * rs + gidx ((J0 + H + 1) * PB + lane)] +
-                  (pass ? 0.0 : bS (r, gidx ((J0 + H + 1) * PB + lane)));
+          for (int r = 0; r < NR; r++)
+            bbuf[i][r] = bg[(size_t) r * rs + gidx ((J0 + H + i) * PB + lane)] +
+                         (pass ? 0.0 : bS (r, gidx ((J0 + H + i) * PB + lane)));
+        }
       }
       __builtin_amdgcn_s_waitcnt (0);
       __syncthreads ();
@@ -1822,7 +1816,7 @@ __device__ __forceinline__ void split_body (
         for (int d = 0; d <= H; d++)
 #pragma unroll
           for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
-        const int vb = vbase (J + 2);
+        const int vb = vbase (J + PD);
 #pragma unroll
         for (int k = 0; k < UPW; k++) {
 #pragma unroll
@@ -1852,7 +1846,7 @@ __device__ __forceinline__ void split_body (
           int off = 0;
 #pragma unroll
           for (int r = 0; r < NR; r++) asm ("" : "+v"(off) : "v"(acc[H][r]), "v"(bq[r]));
-          const int gi = gidx ((J + H + 2) * PB + lane) + off;
+          const int gi = gidx ((J + H + PD) * PB + lane) + off;
 #pragma unroll
           for (int r = 0; r < NR; r++)
             bq[r] = bg[(size_t) r * rs + gi] + (pass ? 0.0 : bS (r, gi));
@@ -1860,11 +1854,13 @@ __device__ __forceinline__ void split_body (
         lds_sync ();
       };
       int J = J0;
-      for (; J + 1 < J1; J += 2) {
-        step (J, ua, ba);
-        step (J + 1, ub, bb);
+      for (; J + PD - 1 < J1; J += PD) {
+#pragma unroll
+        for (int i = 0; i < PD; i++) step (J + i, ubuf[i], bbuf[i]);
       }
-      if (J < J1) step (J, ua, ba);
+#pragma unroll
+      for (int i = 0; i < PD - 1; i++)
+        if (J + i < J1) step (J + i, ubuf[i], bbuf[i]);
       put_y (J1 - 1);
       // the far roles' last records: p_{J1..J1+D-1}; with them, the window
       // handed on (blocks that never entered the near ring keep b)
@@ -1945,17 +1941,15 @@ __device__ __forceinline__ void split_body (
         return d < FB ? pvoff (vb, a + d, k) : OOB;
       };
       {
-        const int v0 = vbase (J0), v1 = vbase (J0 + 1);
 #pragma unroll
-        for (int k = 0; k < UPW; k++)
+        for (int i = 0; i < PD; i++) {
+          const int v0 = vbase (J0 + i);
 #pragma unroll
-          for (int d = 0; d < FBM; d++)
-            ua[k * FBM + d] = pld (fvoff (v0, d, k), 32768 * (a + d) + 1024 * k);
+          for (int k = 0; k < UPW; k++)
 #pragma unroll
-        for (int k = 0; k < UPW; k++)
-#pragma unroll
-          for (int d = 0; d < FBM; d++)
-            ub[k * FBM + d] = pld (fvoff (v1, d, k), 32768 * (a + d) + 1024 * k);
+            for (int d = 0; d < FBM; d++)
+              ubuf[i][k * FBM + d] = pld (fvoff (v0, d, k), 32768 * (a + d) + 1024 * k);
+        }
       }
       __builtin_amdgcn_s_waitcnt (0);
       __syncthreads ();
@@ -1974,7 +1968,7 @@ __device__ __forceinline__ void split_body (
         for (int d = 0; d < FBM; d++)
 #pragma unroll
           for (int r = 0; r < NR; r++) acc[d][r] = 0.0;
-        const int vb = vbase (J + 2);
+        const int vb = vbase (J + PD);
 #pragma unroll
         for (int k = 0; k < UPW; k++) {
 #pragma unroll
@@ -2001,11 +1995,13 @@ __device__ __forceinline__ void split_body (
         lds_sync ();
       };
       int J = J0;
-      for (; J + 1 < J1; J += 2) {
-        step (J, ua);
-        step (J + 1, ub);
+      for (; J + PD - 1 < J1; J += PD) {
+#pragma unroll
+        for (int i = 0; i < PD; i++) step (J + i, ubuf[i]);
       }
-      if (J < J1) step (J, ua);
+#pragma unroll
+      for (int i = 0; i < PD - 1; i++)
+        if (J + i < J1) step (J + i, ubuf[i]);
       // the block completed by the last step and the partial sums still
       // pending: p_{J1+a-1..J1+bl-1}; past its last block this role adds
       // nothing: zero records up to J1+D-1
@@ -2030,7 +2026,7 @@ __device__ __forceinline__ void split_body (
 // always use one epoch (also when one of them starts only after its partner
 // has given up and finished), and a record written under an earlier launch
 // of any layout never carries the current epoch.
-template <int NR, int D, int R, bool LL>
+template <int NR, int D, int R, bool LL, int PD>
 __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
     const double* __restrict__ PF, const double* __restrict__ PR, size_t pstride,
     const int* __restrict__ nsolve, double* rhs, int nmax, int nmodes, int G,
@@ -2043,7 +2039,7 @@ __global__ __launch_bounds__(UW * 64) void k_split_sweeps (
   // would make every workgroup invalidate or write back its L2 at its end)
   const unsigned ep = __hip_atomic_load (sync, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   bool dead = false;
-  split_body<NR, D, R, LL> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
+  split_body<NR, D, R, LL, PD> (PF, PR, pstride, nsolve, rhs, nmax, nmodes, G, tw, stage, boff, rhsb,
                      nmaxb, kd, xch, ep, spin, dead);
   const int any_dead = __syncthreads_or (dead ? 1 : 0);
   if (threadIdx.x == 0) {
@@ -2140,13 +2136,13 @@ size_t split_lds ()
 // the kernel uses scratch (private memory): the occupancy query does not
 // count the runtime's limit on waves holding scratch at once, so a grid it
 // reports as fitting could leave a partner workgroup waiting for scratch
-template <int NR, int D, int R, bool LL>
+template <int NR, int D, int R, bool LL, int PD>
 int split_capacity ()
 {
   static int cap = -1;
   if (cap < 0) {
     int dev = 0, cus = 0, per = 0;
-    auto kern = k_split_sweeps<NR, D, R, LL>;
+    auto kern = k_split_sweeps<NR, D, R, LL, PD>;
     const int lds = (int) split_lds<NR, D, R> ();
     hipFuncAttributes fa;
     if (hipGetDevice (&dev) != hipSuccess ||
@@ -2168,12 +2164,14 @@ int launch_split_d (const sx_msys* s, int G, int stage, hipStream_t st, bool pro
 {
   const int pieces = (stage == 1 || stage == 3) ? 2 : 1;
   const int nblocks = (s->nmodes + 7) / 8 * 8 * G * R * pieces;
+  // panels two steps ahead: three (PD = 3) need more than the 256 VGPRs
+  // of two waves per SIMD for D > 3 and spill
   const bool ll = band_ll ();
   if (probe)
-    return nblocks <= (ll ? split_capacity<NR, D, R, true> () : split_capacity<NR, D, R, false> ())
+    return nblocks <= (ll ? split_capacity<NR, D, R, true, 2> () : split_capacity<NR, D, R, false, 2> ())
                ? SX_OK : SX_ERR_LIMIT;
   const size_t lds = split_lds<NR, D, R> ();
-  auto kern = ll ? k_split_sweeps<NR, D, R, true> : k_split_sweeps<NR, D, R, false>;
+  auto kern = ll ? k_split_sweeps<NR, D, R, true, 2> : k_split_sweeps<NR, D, R, false, 2>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (nblocks), dim3 (UW * 64), lds, st, s->d_swF, s->d_swR,
