@@ -683,20 +683,20 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
                         (int) cons);
     const bool fused = nc == 3 && fourier_fused_ok (d->nz);
     if (fused) {
-      // z terms folded into the transforms: N -= w .* INV(d_z U) and
-      // N -= INV(d_z FWD(u .* w))
+      // z terms folded into the transforms: N -= w .* INV(d_z U) in
+      // physical space; the conservative term d_z (u w) = d_z FWD(u .* w)
+      // subtracted in Fourier space, in the transform that takes N back
+      // (FWD (N - INV (d_z F)) = FWD (N) - d_z F)
       if (conv && (rc = fourier_z_fused (d->nz, d->S, U[i], nullptr, N[i], -1, d->beta, N[i],
                                          d->phys[2], 1.0, 0.0, st)))
         return rc;
-      if (cons &&
-          ((rc = fourier_z_fused (d->nz, d->S, d->phys[i], d->phys[2], d->tmp, +1, 0.0,
-                                  nullptr, nullptr, 1.0, 0.0, st)) ||
-           (rc = fourier_z_fused (d->nz, d->S, d->tmp, nullptr, N[i], -1, d->beta, N[i],
-                                  nullptr, 1.0, 0.0, st))))
+      if (cons && (rc = fourier_z_fused (d->nz, d->S, d->phys[i], d->phys[2], d->tmp, +1, 0.0,
+                                         nullptr, nullptr, 1.0, 0.0, st)))
         return rc;
       // back to Fourier space with the skew-symmetric 1/2 and ConstForce
       const double c = std::fabs (d->cf[i]) > EPSDP ? d->cf[i] : 0.0;
-      if ((rc = fourier_z_fused (d->nz, d->S, N[i], nullptr, N[i], +1, 0.0, nullptr, nullptr,
+      if ((rc = fourier_z_fused (d->nz, d->S, N[i], nullptr, N[i], +1, cons ? d->beta : 0.0,
+                                 cons ? d->tmp : nullptr, nullptr,
                                  d->advection == SX_SKEW ? 0.5 : 1.0, c, st)))
         return rc;
       if ((rc = smooth (d, N[i], st))) return rc;

@@ -179,7 +179,8 @@ __device__ __forceinline__ void dft_regs (double2 (&v)[R], const double2* W, int
 }
 
 // Fused forms (Fused): forward input in .* mul, forward output
-// scale * X (+ add0 on plane 0); inverse input multiplied by i beta k (the
+// scale * (X - d_z fsub) (+ add0 on plane 0; d_z multiplies by i fdz k,
+// planes 0 and 1 drop out); inverse input multiplied by i beta k (the
 // z derivative, AuxField::gradient(2): Nyquist and k = 0 dropped), inverse
 // output acc - amul .* x.  Null pointers / zero values switch a part off.
 struct Fused {
@@ -187,6 +188,10 @@ struct Fused {
   double dz = 0.0, scale = 1.0, add0 = 0.0;
   const double* acc = nullptr;
   const double* amul = nullptr;
+  // forward: the z derivative (times fdz k) of the packed spectra fsub is
+  // subtracted before scaling
+  const double* fsub = nullptr;
+  double fdz = 0.0;
 };
 
 // V16: column pairs moved as 16-byte accesses (S even, rows 16-byte
@@ -322,8 +327,16 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
       const bool re = pp < 2 || (pp & 1) == 0;
       const double ve = re ? 0.5 * (zk.x + zr.x) : 0.5 * (zk.y - zr.y);
       const double vo = re ? 0.5 * (zk.y + zr.y) : -0.5 * (zk.x - zr.x);
-      const double ad = pp == 0 ? fu.add0 : 0.0;
-      *(double2*) (out + (size_t) pp * S + s) = make_double2 (ve * scale + ad, vo * scale + ad);
+      double ae = pp == 0 ? fu.add0 : 0.0, ao = ae;
+      if (fu.fsub && pp >= 2) {
+        // Re_k -= -fdz k Im_k (fsub), Im_k -= fdz k Re_k (fsub); fsub is
+        // normalised, X (ve, vo) is not yet
+        const double bk = fu.scale * fu.fdz * (pp >> 1);
+        const double2 t = *(const double2*) (fu.fsub + (size_t) (pp ^ 1) * S + s);
+        ae = (pp & 1) ? -bk * t.x : bk * t.x;
+        ao = (pp & 1) ? -bk * t.y : bk * t.y;
+      }
+      *(double2*) (out + (size_t) pp * S + s) = make_double2 (ve * scale + ae, vo * scale + ao);
     }
   } else {
     for (int i = tid; i < N * TS; i += TH) {
@@ -335,8 +348,12 @@ __global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, i
       double re, im;
       if ((c & 1) == 0) { re = 0.5 * (zk.x + zr.x); im = 0.5 * (zk.y - zr.y); }
       else              { re = 0.5 * (zk.y + zr.y); im = -0.5 * (zk.x - zr.x); }
-      out[(size_t) pp * S + s] = ((pp < 2 || (pp & 1) == 0) ? re : im) * scale +
-                                 (pp == 0 ? fu.add0 : 0.0);
+      double a = pp == 0 ? fu.add0 : 0.0;
+      if (fu.fsub && pp >= 2) {
+        const double bk = fu.scale * fu.fdz * (pp >> 1), t = fu.fsub[(size_t) (pp ^ 1) * S + s];
+        a = (pp & 1) ? -bk * t : bk * t;
+      }
+      out[(size_t) pp * S + s] = ((pp < 2 || (pp & 1) == 0) ? re : im) * scale + a;
     }
   }
 }
@@ -432,7 +449,7 @@ int launch_fft2_t (int S, const double* in, double* out, int sign, const double2
   // column pairs as 16-byte accesses when every row starts 16-byte aligned
   auto al = [] (const double* p) { return ((uintptr_t) p & 15) == 0; };
   const bool v16 = S % 2 == 0 && al (in) && al (out) && al (fu.mul) &&
-                   al (fu.acc) && al (fu.amul);
+                   al (fu.acc) && al (fu.amul) && al (fu.fsub);
   auto kern = v16 ? k_fft2_z<N, R1, TH, true> : k_fft2_z<N, R1, TH, false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
@@ -500,6 +517,8 @@ int fourier_z_fused (int nz, int S, const double* in, const double* mul, double*
   fu.add0 = sign > 0 ? add0 : 0.0;
   fu.dz = sign < 0 ? dz : 0.0;
   fu.acc = sign < 0 ? acc : nullptr;
+  fu.fsub = sign > 0 ? acc : nullptr;
+  fu.fdz = sign > 0 ? dz : 0.0;
   fu.amul = sign < 0 ? amul : nullptr;
   switch (nz) {
   case 32: return launch_fft2<32, 4> (S, in, out, sign, t->tw, st, fu);

@@ -62,7 +62,8 @@ int    field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const*
 int    fourier_z (int nz, int S, const double* in, double* out, int sign, hipStream_t st);
 int    fourier_prepare (int nz);   // builds the per-device tables (synchronous)
 // Fused z transforms (nz 32, 64, 128, 256: fourier_fused_ok):
-//   sign +1: out = scale FWD(in .* mul) + add0 on plane 0   (mul may be null)
+//   sign +1: out = scale (FWD(in .* mul) - dz-derivative of acc) + add0 on
+//            plane 0 (mul, acc may be null; acc packed spectra, dz = beta)
 //   sign -1: out = acc - amul .* INV(dz-derivative of in)   (dz = beta: multiply
 //            by i beta k first, 0: none; acc null: out = INV(..); amul null: 1)
 bool   fourier_fused_ok (int nz);
