@@ -669,39 +669,61 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
   if (d->dealias) return advection32 (d, U, N, st);
   const int nc = d->ncom;
   int rc;
-  // N needs no clearing: k_adv_xy writes every element entry of it (the
-  // plane padding is never written and stays zero from sx_dns_set_state)
-  for (int i = 0; i < nc; i++)
-    if ((rc = transform_to (d, U[i], d->phys[i], -1, st))) return rc;
   const bool conv = d->advection == SX_SKEW || d->toggle == 1;
   const bool cons = d->advection == SX_SKEW || d->toggle == 0;
   const sx_mesh* m = d->mu;
+  if (nc == 3 && fourier_fused_ok (d->nz)) {
+    // The three components' transforms of each kind in one launch (the
+    // launch has three times the workgroups, so the loads of later ones
+    // overlap the stores of earlier ones); the same operations per
+    // component as one at a time.  N needs no clearing: k_adv_xy writes
+    // every element entry of it (the plane padding stays zero from
+    // sx_dns_set_state); H is free until the time-level combination.
+    const double* Uc[3] = {U[0], U[1], U[2]};
+    const double* Nc[3] = {N[0], N[1], N[2]};
+    const double* P3[3] = {d->phys[2], d->phys[2], d->phys[2]};
+    const double* Pc[3] = {d->phys[0], d->phys[1], d->phys[2]};
+    const double* Hc[3] = {d->H[0], d->H[1], d->H[2]};
+    double* Ho[3] = {d->H[0], d->H[1], d->H[2]};
+    if ((rc = fourier_z_fused_batch (d->nz, d->S, 3, Uc, nullptr, d->phys, -1, 0.0, nullptr,
+                                     nullptr, nullptr, nullptr, st)))
+      return rc;
+    for (int i = 0; i < nc; i++)
+      hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
+                          m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
+                          (int) cons);
+    // N -= w .* INV(d_z U) in physical space
+    if (conv && (rc = fourier_z_fused_batch (d->nz, d->S, 3, Uc, nullptr, N, -1, d->beta, Nc,
+                                             P3, nullptr, nullptr, st)))
+      return rc;
+    // the conservative term d_z (u w) = d_z FWD(u .* w) subtracted in
+    // Fourier space, in the transform that takes N back
+    // (FWD (N - INV (d_z F)) = FWD (N) - d_z F), with the skew-symmetric
+    // 1/2 and ConstForce
+    if (cons && (rc = fourier_z_fused_batch (d->nz, d->S, 3, Pc, P3, Ho, +1, 0.0, nullptr,
+                                             nullptr, nullptr, nullptr, st)))
+      return rc;
+    double sc[3], ad[3];
+    for (int i = 0; i < 3; i++) {
+      sc[i] = d->advection == SX_SKEW ? 0.5 : 1.0;
+      ad[i] = std::fabs (d->cf[i]) > EPSDP ? d->cf[i] : 0.0;
+    }
+    if ((rc = fourier_z_fused_batch (d->nz, d->S, 3, Nc, nullptr, N, +1, cons ? d->beta : 0.0,
+                                     cons ? Hc : nullptr, nullptr, sc, ad, st)))
+      return rc;
+    for (int i = 0; i < nc; i++)
+      if ((rc = smooth (d, N[i], st))) return rc;
+    if (d->advection == SX_ALTSKEW) d->toggle = 1 - d->toggle;
+    SX_HIP (hipGetLastError ());
+    return SX_OK;
+  }
+  for (int i = 0; i < nc; i++)
+    if ((rc = transform_to (d, U[i], d->phys[i], -1, st))) return rc;
   for (int i = 0; i < nc; i++) {
     // x and y terms in one pass per element-plane, then the z terms
     hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
                         m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
                         (int) cons);
-    const bool fused = nc == 3 && fourier_fused_ok (d->nz);
-    if (fused) {
-      // z terms folded into the transforms: N -= w .* INV(d_z U) in
-      // physical space; the conservative term d_z (u w) = d_z FWD(u .* w)
-      // subtracted in Fourier space, in the transform that takes N back
-      // (FWD (N - INV (d_z F)) = FWD (N) - d_z F)
-      if (conv && (rc = fourier_z_fused (d->nz, d->S, U[i], nullptr, N[i], -1, d->beta, N[i],
-                                         d->phys[2], 1.0, 0.0, st)))
-        return rc;
-      if (cons && (rc = fourier_z_fused (d->nz, d->S, d->phys[i], d->phys[2], d->tmp, +1, 0.0,
-                                         nullptr, nullptr, 1.0, 0.0, st)))
-        return rc;
-      // back to Fourier space with the skew-symmetric 1/2 and ConstForce
-      const double c = std::fabs (d->cf[i]) > EPSDP ? d->cf[i] : 0.0;
-      if ((rc = fourier_z_fused (d->nz, d->S, N[i], nullptr, N[i], +1, cons ? d->beta : 0.0,
-                                 cons ? d->tmp : nullptr, nullptr,
-                                 d->advection == SX_SKEW ? 0.5 : 1.0, c, st)))
-        return rc;
-      if ((rc = smooth (d, N[i], st))) return rc;
-      continue;
-    }
     if (nc == 3) {
       // z terms through Fourier space: w d_z u and d_z (u w)
       if (conv) {

@@ -192,15 +192,24 @@ struct Fused {
   // subtracted before scaling
   const double* fsub = nullptr;
   double fdz = 0.0;
+  const double* in = nullptr;      // the transform's input and output
+  double* out = nullptr;
+};
+// up to FB_MAX transforms of one shape in one launch (blockIdx.y selects)
+constexpr int FB_MAX = 3;
+struct FusedBatch {
+  Fused f[FB_MAX];
 };
 
 // V16: column pairs moved as 16-byte accesses (S even, rows 16-byte
 // aligned).
 template <int N, int R1, int TH, bool V16>
-__global__ __launch_bounds__(TH) void k_fft2_z (const double* in, double* out, int S,
-                                                        int sign, const double2* __restrict__ tw,
-                                                        Fused fu)
+__global__ __launch_bounds__(TH) void k_fft2_z (FusedBatch bt, int S, int sign,
+                                                        const double2* __restrict__ tw)
 {
+  const Fused fu = bt.f[blockIdx.y];
+  const double* in = fu.in;
+  double* out = fu.out;
   constexpr int R2 = Fft2<N, R1, TH>::R2, NQ = Fft2<N, R1, TH>::NQ, TS = Fft2<N, R1, TH>::TS;
   static_assert (NQ * R2 == TH && NQ * R1 <= TH, "tile shape");
   extern __shared__ double2 lds[];
@@ -441,19 +450,23 @@ int tables (int nz, Tables** out)
 }
 
 template <int N, int R1, int TH>
-int launch_fft2_t (int S, const double* in, double* out, int sign, const double2* tw,
-                   hipStream_t st, Fused fu)
+int launch_fft2_t (int S, int nb, const FusedBatch& bt, int sign, const double2* tw,
+                   hipStream_t st)
 {
   const size_t lds = Fft2<N, R1, TH>::lds ();
   const int TS = Fft2<N, R1, TH>::TS, ntiles = (S + TS - 1) / TS;
   // column pairs as 16-byte accesses when every row starts 16-byte aligned
   auto al = [] (const double* p) { return ((uintptr_t) p & 15) == 0; };
-  const bool v16 = S % 2 == 0 && al (in) && al (out) && al (fu.mul) &&
-                   al (fu.acc) && al (fu.amul) && al (fu.fsub);
+  bool v16 = S % 2 == 0;
+  for (int b = 0; b < nb; b++) {
+    const Fused& fu = bt.f[b];
+    v16 = v16 && al (fu.in) && al (fu.out) && al (fu.mul) && al (fu.acc) && al (fu.amul) &&
+          al (fu.fsub);
+  }
   auto kern = v16 ? k_fft2_z<N, R1, TH, true> : k_fft2_z<N, R1, TH, false>;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
-  hipLaunchKernelGGL (kern, dim3 (ntiles), dim3 (TH), lds, st, in, out, S, sign, tw, fu);
+  hipLaunchKernelGGL (kern, dim3 (ntiles, nb), dim3 (TH), lds, st, bt, S, sign, tw);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -468,11 +481,22 @@ int fft_threads ()
 }
 
 template <int N, int R1>
+int launch_fft2b (int S, int nb, const FusedBatch& bt, int sign, const double2* tw,
+                  hipStream_t st)
+{
+  if (fft_threads () == 128) return launch_fft2_t<N, R1, 128> (S, nb, bt, sign, tw, st);
+  return launch_fft2_t<N, R1, 256> (S, nb, bt, sign, tw, st);
+}
+
+template <int N, int R1>
 int launch_fft2 (int S, const double* in, double* out, int sign, const double2* tw,
                  hipStream_t st, Fused fu = Fused ())
 {
-  if (fft_threads () == 128) return launch_fft2_t<N, R1, 128> (S, in, out, sign, tw, st, fu);
-  return launch_fft2_t<N, R1, 256> (S, in, out, sign, tw, st, fu);
+  FusedBatch bt;
+  fu.in = in;
+  fu.out = out;
+  bt.f[0] = fu;
+  return launch_fft2b<N, R1> (S, 1, bt, sign, tw, st);
 }
 
 template <int N>
@@ -525,6 +549,39 @@ int fourier_z_fused (int nz, int S, const double* in, const double* mul, double*
   case 64: return launch_fft2<64, 8> (S, in, out, sign, t->tw, st, fu);
   case 128: return launch_fft2<128, 8> (S, in, out, sign, t->tw, st, fu);
   default: return launch_fft2<256, 16> (S, in, out, sign, t->tw, st, fu);
+  }
+}
+
+int fourier_z_fused_batch (int nz, int S, int nb, const double* const* in,
+                           const double* const* mul, double* const* out, int sign, double dz,
+                           const double* const* acc, const double* const* amul,
+                           const double* scale, const double* add0, hipStream_t st)
+{
+  if (!fourier_fused_ok (nz) || S < 1 || (sign != 1 && sign != -1) || nb < 1 || nb > FB_MAX) {
+    set_error ("fourier_z_fused_batch: nz must be 32, 64, 128 or 256, 1..3 transforms");
+    return SX_ERR_ARG;
+  }
+  Tables* t = nullptr;
+  if (int rc = tables (nz, &t)) return rc;
+  FusedBatch bt;
+  for (int b = 0; b < nb; b++) {
+    Fused& fu = bt.f[b];
+    fu.in = in[b];
+    fu.out = out[b];
+    fu.mul = sign > 0 && mul ? mul[b] : nullptr;
+    fu.scale = sign > 0 && scale ? scale[b] : 1.0;
+    fu.add0 = sign > 0 && add0 ? add0[b] : 0.0;
+    fu.dz = sign < 0 ? dz : 0.0;
+    fu.acc = sign < 0 && acc ? acc[b] : nullptr;
+    fu.fsub = sign > 0 && acc ? acc[b] : nullptr;
+    fu.fdz = sign > 0 ? dz : 0.0;
+    fu.amul = sign < 0 && amul ? amul[b] : nullptr;
+  }
+  switch (nz) {
+  case 32: return launch_fft2b<32, 4> (S, nb, bt, sign, t->tw, st);
+  case 64: return launch_fft2b<64, 8> (S, nb, bt, sign, t->tw, st);
+  case 128: return launch_fft2b<128, 8> (S, nb, bt, sign, t->tw, st);
+  default: return launch_fft2b<256, 16> (S, nb, bt, sign, t->tw, st);
   }
 }
 

@@ -70,6 +70,12 @@ bool   fourier_fused_ok (int nz);
 int    fourier_z_fused (int nz, int S, const double* in, const double* mul, double* out,
                         int sign, double dz, const double* acc, const double* amul,
                         double scale, double add0, hipStream_t st);
+// nb (1..3) fused transforms of one shape in one launch: array arguments
+// per transform (null arrays: no mul / acc / amul; scale, add0 null: 1, 0)
+int    fourier_z_fused_batch (int nz, int S, int nb, const double* const* in,
+                              const double* const* mul, double* const* out, int sign, double dz,
+                              const double* const* acc, const double* const* amul,
+                              const double* scale, const double* add0, hipStream_t st);
 // Mixed-precision solve with a fixed number of refinement passes, no host
 // synchronisation (sx_mixed.hip).
 int    field_solve_mixed_passes (sx_msys* s, int nz, int stride, double* d_f, double* d_u,
