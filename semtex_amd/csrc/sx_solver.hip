@@ -1077,6 +1077,35 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_r (
                          nmax, nz, base_mode, rnr, roff);
 }
 
+// Up to three fields of one system in one launch (blockIdx.z selects the
+// field): the grouped solve's condensations and recoveries.
+struct FieldPtrs {
+  double* p[3];
+};
+struct FieldCPtrs {
+  const double* p[3];
+};
+template <int NPM>
+__global__ __launch_bounds__(CT_THREADS) void k_condense_rb (
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
+    const double* __restrict__ geo, const double* __restrict__ Mcon, FieldPtrs F, int stride,
+    int nz, int base_mode, int weak)
+{
+  k_condense_r_body<NPM> (np, nclass, ttab, emap_g, geo, Mcon, F.p[blockIdx.z], stride, nz,
+                          base_mode, weak);
+}
+template <int NPM>
+__global__ __launch_bounds__(CT_THREADS) void k_recover_rb (
+    int np, int nclass, const int* __restrict__ ttab, const int* __restrict__ emap_g,
+    const double* __restrict__ Mrec, FieldCPtrs G, int nglobal, FieldCPtrs F, FieldPtrs U,
+    int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
+    int nz, int base_mode, int rnr)
+{
+  k_recover_r_body<NPM> (np, nclass, ttab, emap_g, Mrec, G.p[blockIdx.z], nglobal,
+                         F.p[blockIdx.z], U.p[blockIdx.z], stride, nsolve, rhs, nmax, nz,
+                         base_mode, rnr, 2 * blockIdx.z);
+}
+
 // FP32 forms of the tiled condensation and recovery for the mixed solve's
 // inner passes (FP32 operands and MFMA tiles, FP32 accumulation; the
 // boundary values stay FP64).
@@ -1687,6 +1716,39 @@ int launch_recover_r (const sx_msys* s, int nz, int stride, const double* F, dou
   return SX_OK;
 }
 
+// the register-operand condensation / recovery of nf <= 3 fields of one
+// system in one launch each (the grouped solve)
+template <int NPM>
+int launch_condense_group (const sx_msys* s, int nf, int nz, int stride, const FieldPtrs& F,
+                           hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const size_t lds = reg_lds (m, false);
+  auto kern = k_condense_rb<NPM>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz), nf), dim3 (CT_THREADS), lds, st,
+                      m->np, m->nclass, m->d_ttab, m->d_emap, m->d_geo, s->d_Mcon, F, stride, nz,
+                      s->base_mode, 0);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+template <int NPM>
+int launch_recover_group (const sx_msys* s, int nf, int nz, int stride, const FieldCPtrs& F,
+                          const FieldPtrs& U, const FieldCPtrs& G, int rnr, hipStream_t st)
+{
+  const sx_mesh* m = s->m;
+  const size_t lds = reg_lds (m, true);
+  auto kern = k_recover_rb<NPM>;
+  SX_HIP (hipFuncSetAttribute ((const void*) kern,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
+  hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz), nf), dim3 (CT_THREADS), lds, st,
+                      m->np, m->nclass, m->d_ttab, m->d_emap, s->d_Mrec, G, m->nglobal, F, U,
+                      stride, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr);
+  SX_HIP (hipGetLastError ());
+  return SX_OK;
+}
+
 int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st,
               int weak = 0)
 {
@@ -2068,12 +2130,35 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
   const int npl = n_solved_planes (nz, s->base_mode), nr = 2 * nf;
   const int lstride = m->nbound * m->np;
   const size_t gstride = (size_t) 2 * s->nmodes * m->nglobal;
+  // the register-operand element kernels take up to three fields per launch
+  const bool grouped = nf <= 3 && tiled_ok (m) && s->tile_mf && !elem_lds ();
+  const int npmr = reg_npm (m->np);
+  if (grouped) {
+    for (int f = 0; f < nf; f++) {
+      const double* lines = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
+      double* G = s->d_G + f * gstride;
+      hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                          st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
+      if (m->ness)
+        hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
+                            s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
+                            m->d_ess_elem, G, m->nglobal, d_f[f], stride, nz, s->base_mode);
+    }
+    FieldPtrs F {};
+    for (int f = 0; f < nf; f++) F.p[f] = d_f[f];
+    const int rc = npmr == 8 ? launch_condense_group<8> (s, nf, nz, stride, F, st)
+                 : npmr == 10 ? launch_condense_group<10> (s, nf, nz, stride, F, st)
+                              : launch_condense_group<12> (s, nf, nz, stride, F, st);
+    if (rc) return rc;
+  }
   for (int f = 0; f < nf; f++) {
     const double* lines = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
     double* G = s->d_G + f * gstride;
-    hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                        st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
-    if (int rc = condense (s, nz, stride, d_f[f], G, st)) return rc;
+    if (!grouped) {
+      hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                          st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
+      if (int rc = condense (s, nz, stride, d_f[f], G, st)) return rc;
+    }
     hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                         st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
                         d_f[f], stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
@@ -2081,6 +2166,18 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
   }
   SX_HIP (hipGetLastError ());
   if (int rc = band_solve (s, nr, st)) return rc;
+  if (grouped) {
+    FieldCPtrs F {}, G {};
+    FieldPtrs U {};
+    for (int f = 0; f < nf; f++) {
+      F.p[f] = d_f[f];
+      U.p[f] = d_u[f];
+      G.p[f] = s->d_G + f * gstride;
+    }
+    return npmr == 8 ? launch_recover_group<8> (s, nf, nz, stride, F, U, G, nr, st)
+         : npmr == 10 ? launch_recover_group<10> (s, nf, nz, stride, F, U, G, nr, st)
+                      : launch_recover_group<12> (s, nf, nz, stride, F, U, G, nr, st);
+  }
   for (int f = 0; f < nf; f++)
     if (int rc = recover (s, nz, stride, d_f[f], d_u[f], s->d_G + f * gstride, nr, 2 * f, st))
       return rc;
