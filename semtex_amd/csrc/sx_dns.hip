@@ -76,13 +76,20 @@ __global__ __launch_bounds__(64) void k_grad (int np, const double* __restrict__
 // derivative):
 //   N = -[conv (ux d_x u + uy d_y u) + cons (d_x (u ux) + d_y (u uy))]
 // written, not accumulated: the z terms are subtracted afterwards.
+// per-component arguments of a launch over up to three components
+// (blockIdx.z selects one)
+struct Comp3 {
+  const double* in[3];
+  double* out[3];
+};
 __global__ __launch_bounds__(64) void k_adv_xy (int np, const double* __restrict__ geo,
                                                 const double* __restrict__ Dg,
                                                 const double* __restrict__ ux,
-                                                const double* __restrict__ uy,
-                                                const double* __restrict__ ui, double* N,
+                                                const double* __restrict__ uy, Comp3 cmp,
                                                 int stride, int conv, int cons)
 {
+  const double* __restrict__ ui = cmp.in[blockIdx.z];
+  double* N = cmp.out[blockIdx.z];
   const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
   __shared__ double D[256], fu[256], fx[256], fy[256];     // u, u ux, u uy
   const size_t off = (size_t) k * stride + (size_t) e * npnp;
@@ -316,11 +323,19 @@ struct Levels {
 struct Coef3 { double c[3]; };
 // waveProp (dns/integrate.cpp:204-239): H = sum_q -alpha[q+1] Us[q] + beta[q] dt Uf[q],
 // summed in the reference's order (q ascending, the Us term first)
-__global__ void k_ew_wave (double* H, Levels us, Levels uf, int Je, size_t n, double* unyq,
-                           size_t S)
+struct Wave3 {
+  double* H[3];
+  Levels us[3], uf[3];
+  double* unyq[3];
+};
+__global__ void k_ew_wave (Wave3 wv, int Je, size_t n, size_t S)
 {
   const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  double* H = wv.H[blockIdx.y];
+  const Levels& us = wv.us[blockIdx.y];
+  const Levels& uf = wv.uf[blockIdx.y];
+  double* unyq = wv.unyq[blockIdx.y];
   // unyq (when given): the Nyquist plane 1 of us0 carried into u, which the
   // solves leave alone
   if (unyq && i >= S && i < 2 * S) unyq[i] = us.f[0][i];
@@ -339,8 +354,9 @@ __global__ void k_ew_wave (double* H, Levels us, Levels uf, int Je, size_t n, do
 __global__ void k_smooth (int npnp, int nglobal, const int* __restrict__ gptr,
                           const int* __restrict__ gsrc, const int* __restrict__ emap,
                           const double* __restrict__ geo, const double* __restrict__ imass,
-                          double* f, int S)
+                          Comp3 cmp, int S)
 {
+  double* f = cmp.out[blockIdx.z];
   const int g = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (g >= nglobal) return;
   double* fk = f + (size_t) k * S;
@@ -591,14 +607,32 @@ int grad (sx_dns* d, const double* in, double* out, int dir, hipStream_t st)
   return SX_OK;
 }
 
-int smooth (sx_dns* d, double* f, hipStream_t st)
+// smoothing of nf <= 3 fields in one launch
+int smooth_n (sx_dns* d, double* const* f, int nf, hipStream_t st)
 {
   const sx_mesh* m = d->mu;
-  hipLaunchKernelGGL (k_smooth, dim3 ((m->nglobal + 255) / 256, d->nz), dim3 (256), 0, st,
+  Comp3 c {};
+  for (int i = 0; i < nf; i++) c.out[i] = f[i];
+  hipLaunchKernelGGL (k_smooth, dim3 ((m->nglobal + 255) / 256, d->nz, nf), dim3 (256), 0, st,
                       m->npnp, m->nglobal, m->d_gptr, m->d_gsrc, m->d_emap, m->d_geo, d->imass,
-                      f, d->S);
+                      c, d->S);
   SX_HIP (hipGetLastError ());
   return SX_OK;
+}
+int smooth (sx_dns* d, double* f, hipStream_t st) { return smooth_n (d, &f, 1, st); }
+
+// the x/y advection terms of nf <= 3 components in one launch
+void adv_xy (sx_dns* d, double* const* ui, double* const* N, int nf, bool conv, bool cons,
+             hipStream_t st)
+{
+  const sx_mesh* m = d->mu;
+  Comp3 c {};
+  for (int i = 0; i < nf; i++) {
+    c.in[i] = ui[i];
+    c.out[i] = N[i];
+  }
+  hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz, nf), dim3 (64), 0, st, m->np, m->d_geo,
+                      m->d_D, d->phys[0], d->phys[1], c, d->S, (int) conv, (int) cons);
 }
 
 #define EW(kern, ...) \
@@ -643,8 +677,11 @@ int advection32 (sx_dns* d, double** U, double** N, hipStream_t st)
     if ((rc = zgemm (d, U[i], d->Wi, d->physM[i], nz, M, 1.0))) return rc;
   const unsigned gM = nb256 (d->nM);
   for (int i = 0; i < nc; i++) {
+    Comp3 c {};
+    c.in[0] = d->physM[i];
+    c.out[0] = d->tmpM;
     hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, M), dim3 (64), 0, st, m->np, m->d_geo, m->d_D,
-                        d->physM[0], d->physM[1], d->physM[i], d->tmpM, d->S, 1, 1);
+                        d->physM[0], d->physM[1], c, d->S, 1, 1);
     if ((rc = zgemm (d, U[i], d->Wdz, d->tmpM2, nz, M, 1.0))) return rc;
     hipLaunchKernelGGL (k_ew_submul, dim3 (gM), dim3 (256), 0, st, d->tmpM, d->physM[2],
                         d->tmpM2, d->nM);
@@ -688,10 +725,7 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
     if ((rc = fourier_z_fused_batch (d->nz, d->S, 3, Uc, nullptr, d->phys, -1, 0.0, nullptr,
                                      nullptr, nullptr, nullptr, st)))
       return rc;
-    for (int i = 0; i < nc; i++)
-      hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
-                          m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
-                          (int) cons);
+    adv_xy (d, d->phys, N, 3, conv, cons, st);
     // N -= w .* INV(d_z U) in physical space
     if (conv && (rc = fourier_z_fused_batch (d->nz, d->S, 3, Uc, nullptr, N, -1, d->beta, Nc,
                                              P3, nullptr, nullptr, st)))
@@ -711,8 +745,7 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
     if ((rc = fourier_z_fused_batch (d->nz, d->S, 3, Nc, nullptr, N, +1, cons ? d->beta : 0.0,
                                      cons ? Hc : nullptr, nullptr, sc, ad, st)))
       return rc;
-    for (int i = 0; i < nc; i++)
-      if ((rc = smooth (d, N[i], st))) return rc;
+    if ((rc = smooth_n (d, N, 3, st))) return rc;
     if (d->advection == SX_ALTSKEW) d->toggle = 1 - d->toggle;
     SX_HIP (hipGetLastError ());
     return SX_OK;
@@ -721,9 +754,7 @@ int advection (sx_dns* d, double** U, double** N, hipStream_t st)
     if ((rc = transform_to (d, U[i], d->phys[i], -1, st))) return rc;
   for (int i = 0; i < nc; i++) {
     // x and y terms in one pass per element-plane, then the z terms
-    hipLaunchKernelGGL (k_adv_xy, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo,
-                        m->d_D, d->phys[0], d->phys[1], d->phys[i], N[i], d->S, (int) conv,
-                        (int) cons);
+    adv_xy (d, &d->phys[i], &N[i], 1, conv, cons, st);
     if (nc == 3) {
       // z terms through Fourier space: w d_z u and d_z (u w)
       if (conv) {
@@ -817,17 +848,23 @@ int dns_step (sx_dns* d, hipStream_t st)
   }
   // -- waveProp
   const int Je = std::min (d->step, nord);
-  for (int i = 0; i < nc; i++) {
-    // D->u after swapData holds u^n = Us[0]: only the Nyquist plane, which
-    // the solves leave alone, has to be carried over (by the same kernel)
-    Levels ls = {}, lf = {};
-    for (int q = 0; q < Je; q++) {
-      ls.f[q] = d->Us[q][i];
-      ls.c[q] = -SSc[Je][q + 1];
-      lf.f[q] = d->Uf[q][i];
-      lf.c[q] = EXc[Je][q] * d->dt;
+  {
+    // all components in one launch (blockIdx.y selects).  D->u after
+    // swapData holds u^n = Us[0]: only the Nyquist plane, which the solves
+    // leave alone, has to be carried over (by the same kernel)
+    Wave3 wv {};
+    for (int i = 0; i < nc; i++) {
+      wv.H[i] = d->H[i];
+      wv.unyq[i] = d->nz > 1 ? d->u[i] : nullptr;
+      for (int q = 0; q < Je; q++) {
+        wv.us[i].f[q] = d->Us[q][i];
+        wv.us[i].c[q] = -SSc[Je][q + 1];
+        wv.uf[i].f[q] = d->Uf[q][i];
+        wv.uf[i].c[q] = EXc[Je][q] * d->dt;
+      }
     }
-    EW (k_ew_wave, d->H[i], ls, lf, Je, d->n, d->nz > 1 ? d->u[i] : nullptr, (size_t) d->S);
+    hipLaunchKernelGGL (k_ew_wave, dim3 (nb256 (d->n), nc), dim3 (256), 0, st, wv, Je, d->n,
+                        (size_t) d->S);
   }
   roll (d->Uf, nord, nc);
   // -- setPForce: div u^ / dt

@@ -538,6 +538,22 @@ __global__ __launch_bounds__(64) void k_weak_ess (
   k_weak_ess_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, ess_elem, G, nglobal, F, stride, nz, base_mode);
 }
 
+// the same for up to three fields (blockIdx.z selects; G and F per field)
+struct FieldGF {
+  const double* G[3];
+  double* F[3];
+};
+__global__ __launch_bounds__(64) void k_weak_ess3 (
+    int np, int cyl, double lambda2, const double* __restrict__ betak2,
+    const double* __restrict__ geo, const double* __restrict__ Dg,
+    const int* __restrict__ emap_g, const int* __restrict__ btog,
+    const int* __restrict__ ess_elem, FieldGF gf, int nglobal, int stride, int nz,
+    int base_mode)
+{
+  k_weak_ess_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, ess_elem, gf.G[blockIdx.z],
+                   nglobal, gf.F[blockIdx.z], stride, nz, base_mode);
+}
+
 // f (row-major; already weak on essential elements, else -M f is formed
 // here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i.
 template <bool MF>
@@ -1297,6 +1313,31 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
                             int nmax, int nz, int base_mode, int rnr, int roff)
 {
   k_assemble_body (nglobal, gptr, gsrc, nptr, nidx, nw, F, stride, lines, lstride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+}
+
+// The grouped solve's per-field kernels for up to three fields in one
+// launch (blockIdx.z selects the field; its right-hand sides are 2z, 2z+1)
+struct FieldSet {
+  const double* lines[3];
+  double* G[3];
+  double* F[3];
+};
+__global__ void k_essential3 (int nglobal, const int* __restrict__ ess, FieldSet fs,
+                              int lstride, int nz, int base_mode)
+{
+  k_essential_body (nglobal, ess, fs.lines[blockIdx.z], lstride, fs.G[blockIdx.z], nz,
+                    base_mode);
+}
+__global__ void k_assemble3 (int nglobal, const int* __restrict__ gptr,
+                             const int* __restrict__ gsrc, const int* __restrict__ nptr,
+                             const int* __restrict__ nidx, const double* __restrict__ nw,
+                             FieldSet fs, int stride, int lstride,
+                             const int* __restrict__ nsolve, double* rhs, int nmax, int nz,
+                             int base_mode, int rnr)
+{
+  k_assemble_body (nglobal, gptr, gsrc, nptr, nidx, nw, fs.F[blockIdx.z], stride,
+                   fs.lines[blockIdx.z], lstride, nsolve, rhs, nmax, nz, base_mode, rnr,
+                   2 * blockIdx.z);
 }
 
 // One wavefront per (element, solved plane): interior recovery and
@@ -2134,31 +2175,37 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
   const bool grouped = nf <= 3 && tiled_ok (m) && s->tile_mf && !elem_lds ();
   const int npmr = reg_npm (m->np);
   if (grouped) {
-    for (int f = 0; f < nf; f++) {
-      const double* lines = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
-      double* G = s->d_G + f * gstride;
-      hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                          st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
-      if (m->ness)
-        hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
-                            s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
-                            m->d_ess_elem, G, m->nglobal, d_f[f], stride, nz, s->base_mode);
-    }
+    FieldSet fs {};
+    FieldGF gf {};
     FieldPtrs F {};
-    for (int f = 0; f < nf; f++) F.p[f] = d_f[f];
+    for (int f = 0; f < nf; f++) {
+      fs.lines[f] = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
+      fs.G[f] = s->d_G + f * gstride;
+      fs.F[f] = d_f[f];
+      gf.G[f] = fs.G[f];
+      gf.F[f] = d_f[f];
+      F.p[f] = d_f[f];
+    }
+    hipLaunchKernelGGL (k_essential3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256), 0,
+                        st, m->nglobal, m->d_ess, fs, lstride, nz, s->base_mode);
+    if (m->ness)
+      hipLaunchKernelGGL (k_weak_ess3, dim3 (m->ness, npl, nf), dim3 (64), 0, st, m->np, m->cyl,
+                          s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
+                          m->d_ess_elem, gf, m->nglobal, stride, nz, s->base_mode);
     const int rc = npmr == 8 ? launch_condense_group<8> (s, nf, nz, stride, F, st)
                  : npmr == 10 ? launch_condense_group<10> (s, nf, nz, stride, F, st)
                               : launch_condense_group<12> (s, nf, nz, stride, F, st);
     if (rc) return rc;
+    hipLaunchKernelGGL (k_assemble3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256), 0,
+                        st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw, fs,
+                        stride, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, nr);
   }
-  for (int f = 0; f < nf; f++) {
+  for (int f = 0; f < nf && !grouped; f++) {
     const double* lines = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
     double* G = s->d_G + f * gstride;
-    if (!grouped) {
-      hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                          st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
-      if (int rc = condense (s, nz, stride, d_f[f], G, st)) return rc;
-    }
+    hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                        st, m->nglobal, m->d_ess, lines, lstride, G, nz, s->base_mode);
+    if (int rc = condense (s, nz, stride, d_f[f], G, st)) return rc;
     hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                         st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
                         d_f[f], stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
