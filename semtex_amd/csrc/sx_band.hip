@@ -471,8 +471,7 @@ struct PanelSweep {
 // of blocks of each pass; a segment that stops early writes its pending
 // window r_{J1..J1+D-1} back into the right-hand side, where the next
 // segment reads it as its b.  Per-mode parameters, TW ints per mode:
-constexpr int TW = 8;
-enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
+// (TW, TW_* in sx_internal.h)
 
 // One sweep launch's work for this workgroup: its mode and right-hand sides,
 // the piece's size, panels and block ranges per pass ([f0, f1) forward,
@@ -2508,7 +2507,7 @@ int band_factor (sx_msys* s, hipStream_t st, int* d_status)
   return SX_OK;
 }
 
-int band_solve (const sx_msys* s, int nr, hipStream_t st)
+int band_solve (const sx_msys* s, int nr, hipStream_t st, int moves)
 {
   if (s->sw_panel) {
     // right-hand sides per workgroup: SX_PANEL_NR in the environment, else
@@ -2584,17 +2583,19 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st)
       const hipError_t e = hipGetLastError ();
       return e == hipSuccess ? (int) SX_OK : hip_fail (e, "k_twist_move");
     };
+    // moves & 1: gather B here (else the caller's assembly wrote the bottom
+    // piece's right-hand side); moves & 2: scatter B (else the caller's
+    // recovery reads the bottom piece's solution)
     int rc;
+    if ((moves & 1) && (rc = move (0))) return rc;
     if (sper) {
       // the split sweeps fold the separator's combine and x_S copy into stage 2
-      if ((rc = move (0)) || (rc = sweep (1)) || (rc = sweep (2)) || (rc = sweep (3)) ||
-          (rc = move (3)))
-        return rc;
-      return SX_OK;
-    }
-    if ((rc = move (0)) || (rc = sweep (1)) || (rc = move (1)) || (rc = sweep (2)) ||
-        (rc = move (2)) || (rc = sweep (3)) || (rc = move (3)))
+      if ((rc = sweep (1)) || (rc = sweep (2)) || (rc = sweep (3))) return rc;
+    } else if ((rc = sweep (1)) || (rc = move (1)) || (rc = sweep (2)) || (rc = move (2)) ||
+               (rc = sweep (3))) {
       return rc;
+    }
+    if ((moves & 2) && (rc = move (3))) return rc;
     return SX_OK;
   }
   if (nr != 2) {

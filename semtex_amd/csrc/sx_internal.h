@@ -47,7 +47,13 @@ namespace sx {
 int    band_block (int nband);
 size_t band_sweep_bytes_per_mode (int nmax, int nband);
 int    band_factor (sx_msys* s, hipStream_t st, int* d_status);
-int    band_solve (const sx_msys* s, int nr, hipStream_t st);   // nr RHS per mode
+// nr RHS per mode; moves (twisted systems): 1 gather the bottom piece's
+// right-hand side, 2 scatter its solution (3: both; 0 when the caller's
+// assembly and recovery address the bottom piece themselves)
+int    band_solve (const sx_msys* s, int nr, hipStream_t st, int moves = 3);
+// per-mode split of a twisted system (d_tw): TW ints per mode
+constexpr int TW = 8;
+enum { TW_NBT = 0, TW_NBB, TW_PAD, TW_M, TW_NTOP, TW_NBOT, TW_N };
 // SX_OK, or SX_ERR_SYNC when a split-sweep wait gave up since the last call
 // (synchronises st; clears the status word)
 int    split_status (sx_msys* s, hipStream_t st);

@@ -960,7 +960,8 @@ __device__ __forceinline__ void k_recover_r_body (
     const double* __restrict__ Mrec,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
-    int nz, int base_mode, int rnr, int roff)
+    int nz, int base_mode, int rnr, int roff, const int* __restrict__ tw,
+    const double* __restrict__ rhsb, int nmaxb, int tS)
 {
   typedef RegShape<NPM> RS;
   extern __shared__ double sm[];
@@ -996,6 +997,15 @@ __device__ __forceinline__ void k_recover_r_body (
     ev[u] = T[TT_EL + min (i / nint / npm, ETILE - 1)];
   }
   for (int i = npnp * CG_EP + tid; i < R.KR * CG_EP; i += CT_THREADS) Xs[i] = 0.0;
+  // twisted system whose band solve left B's solution in the bottom
+  // piece (tw given): rows g >= m+S come from bottom row p + n-1-g
+  int tbase = 1 << 30, tp = 0, tn = 0;
+  if (tw) {
+    const int* t = tw + TW * mode;
+    tbase = t[TW_M] + tS;
+    tp = t[TW_PAD];
+    tn = t[TW_N];
+  }
   // round 2: u_b from the band solution (zero on essential nodes, whose
   // values entered the right-hand side; the output takes G there), F_i and
   // the operator strip
@@ -1010,7 +1020,13 @@ __device__ __forceinline__ void k_recover_r_body (
     bout[u] = 0.0;
     if (i < nep * next) {
       const int k = k0 + (i / next) % npm;
-      if (gg[u] < ns) ub[u] = rhs[((size_t) mode * rnr + roff + (k & 1)) * nmax + gg[u]];
+      const size_t col = (size_t) mode * rnr + roff + (k & 1);
+      if (gg[u] < ns) {
+        const bool bot = gg[u] >= tbase;
+        const double* src = bot ? rhsb : rhs;
+        const size_t at = bot ? col * nmaxb + (size_t) (tp + tn - 1 - gg[u]) : col * nmax + gg[u];
+        ub[u] = src[at];
+      }
       if (ess[u]) bout[u] = G[(size_t) k * nglobal + gg[u]];
     }
   }
@@ -1087,10 +1103,11 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_r (
     const double* __restrict__ Mrec,
     const double* __restrict__ G, int nglobal, const double* __restrict__ F, double* U,
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
-    int nz, int base_mode, int rnr, int roff)
+    int nz, int base_mode, int rnr, int roff, const int* __restrict__ tw,
+    const double* __restrict__ rhsb, int nmaxb, int tS)
 {
   k_recover_r_body<NPM> (np, nclass, ttab, emap_g, Mrec, G, nglobal, F, U, stride, nsolve, rhs,
-                         nmax, nz, base_mode, rnr, roff);
+                         nmax, nz, base_mode, rnr, roff, tw, rhsb, nmaxb, tS);
 }
 
 // Up to three fields of one system in one launch (blockIdx.z selects the
@@ -1119,7 +1136,8 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_rb (
 {
   k_recover_r_body<NPM> (np, nclass, ttab, emap_g, Mrec, G.p[blockIdx.z], nglobal,
                          F.p[blockIdx.z], U.p[blockIdx.z], stride, nsolve, rhs, nmax, nz,
-                         base_mode, rnr, 2 * blockIdx.z);
+                         base_mode, rnr, 2 * blockIdx.z, (const int*) nullptr,
+                         (const double*) nullptr, 0, 0);
 }
 
 // FP32 forms of the tiled condensation and recovery for the mixed solve's
@@ -1313,6 +1331,32 @@ __global__ void k_assemble (int nglobal, const int* __restrict__ gptr,
                             int nmax, int nz, int base_mode, int rnr, int roff)
 {
   k_assemble_body (nglobal, gptr, gsrc, nptr, nidx, nw, F, stride, lines, lstride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+}
+
+// Assembly for a twisted system that also writes the bottom piece's
+// right-hand side (the gather the band solve would otherwise do): bottom row
+// q = p + n-1-g holds row g of B = [m+S, n); its separator rows
+// [m, m+S) and its p identity rows start at zero.
+__global__ void k_assemble_tw (int nglobal, const int* __restrict__ gptr,
+                               const int* __restrict__ gsrc, const int* __restrict__ nptr,
+                               const int* __restrict__ nidx, const double* __restrict__ nw,
+                               const double* __restrict__ F, int stride,
+                               const double* __restrict__ lines, int lstride,
+                               const int* __restrict__ nsolve, double* rhs, int nmax, int nz,
+                               int base_mode, int rnr, const int* __restrict__ tw,
+                               double* rhsb, int nmaxb, int S)
+{
+  k_assemble_body (nglobal, gptr, gsrc, nptr, nidx, nw, F, stride, lines, lstride, nsolve, rhs,
+                   nmax, nz, base_mode, rnr, 0);
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = solved_plane (blockIdx.y, nz, base_mode);
+  const int mode = k >> 1;
+  const int* t = tw + TW * mode;
+  const int p = t[TW_PAD], m = t[TW_M], n = t[TW_N];
+  if (g >= n) return;
+  double* rb = rhsb + ((size_t) mode * rnr + (k & 1)) * nmaxb;
+  if (g < p) rb[g] = 0.0;
+  if (g >= m) rb[p + n - 1 - g] = g >= m + S ? rhs[((size_t) mode * rnr + (k & 1)) * nmax + g] : 0.0;
 }
 
 // The grouped solve's per-field kernels for up to three fields in one
@@ -1743,7 +1787,7 @@ int launch_condense_r (const sx_msys* s, int nz, int stride, double* F, int weak
 
 template <int NPM>
 int launch_recover_r (const sx_msys* s, int nz, int stride, const double* F, double* U,
-                      const double* G, int rnr, int roff, hipStream_t st)
+                      const double* G, int rnr, int roff, hipStream_t st, bool bottom = false)
 {
   const sx_mesh* m = s->m;
   const size_t lds = reg_lds (m, true);
@@ -1752,7 +1796,9 @@ int launch_recover_r (const sx_msys* s, int nz, int stride, const double* F, dou
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds, st,
                       m->np, m->nclass, m->d_ttab, m->d_emap, s->d_Mrec, G, m->nglobal, F, U,
-                      stride, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff);
+                      stride, s->d_nsolve, s->d_rhs, s->nmax, nz, s->base_mode, rnr, roff,
+                      bottom ? (const int*) s->d_tw : nullptr, (const double*) s->d_rhsb,
+                      s->nmaxb, s->tw_S);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }
@@ -1889,21 +1935,39 @@ int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, doubl
     PhaseScope ps (1, st, timed);
     if ((rc = condense (s, nz, stride, d_f, s->d_G, st))) return rc;
   }
+  // a twisted system's bottom piece addressed by the assembly and the
+  // recovery themselves (no gather and scatter launches around the sweeps)
+  const bool fold = s->tw && s->sw_panel && tiled_ok (m) && s->tile_mf && !elem_lds ();
   {
     PhaseScope ps (2, st, timed);
-    hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                        st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
-                        d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
-                        s->base_mode, 2, 0);
+    if (fold)
+      hipLaunchKernelGGL (k_assemble_tw, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                          st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
+                          d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                          s->base_mode, 2, s->d_tw, s->d_rhsb, s->nmaxb, s->tw_S);
+    else
+      hipLaunchKernelGGL (k_assemble, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                          st, m->nglobal, m->d_gptr, m->d_gsrc, m->d_nptr, m->d_nidx, m->d_nw,
+                          d_f, stride, lines, lstride, s->d_nsolve, s->d_rhs, s->nmax, nz,
+                          s->base_mode, 2, 0);
   }
   SX_HIP (hipGetLastError ());
   {
     PhaseScope ps (3, st, timed);
-    if ((rc = band_solve (s, 2, st))) return rc;
+    if ((rc = band_solve (s, 2, st, fold ? 0 : 3))) return rc;
   }
   {
     PhaseScope ps (4, st, timed);
-    if ((rc = recover (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st))) return rc;
+    if (fold) {
+      switch (reg_npm (m->np)) {
+      case 8: rc = launch_recover_r<8> (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st, true); break;
+      case 10: rc = launch_recover_r<10> (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st, true); break;
+      default: rc = launch_recover_r<12> (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st, true); break;
+      }
+      if (rc) return rc;
+    } else if ((rc = recover (s, nz, stride, d_f, d_u, s->d_G, 2, 0, st))) {
+      return rc;
+    }
   }
   SX_HIP (hipGetLastError ());
   return SX_OK;
@@ -2097,7 +2161,7 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
         (const int*) m->d_emap, (const double*) s->d_Mrec,
         (const double*) s->d_G, m->nglobal, (const double*) d_f, d_u, stride,
         (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
-        roff);
+        roff, (const int*) nullptr, (const double*) nullptr, 0, 0);
   };
   if (s->tile_mf && !elem_lds ()) {
     if (npmr == 8) rec_r (FnRecoverR<8> {});
