@@ -2308,6 +2308,7 @@ static bool twist_split (const sx_msys* s, std::vector<int>& tw)
     return false;
   }
   const int D = panel_D (s->m->nband), S = PB * D;
+  const bool ll = band_ll () && use_unit (D);
   tw.assign ((size_t) TW * s->nmodes, 0);
   for (int md = 0; md < s->nmodes; md++) {
     const int n = s->nsolve[md], nbtot = (n + PB - 1) / PB;
@@ -2316,8 +2317,10 @@ static bool twist_split (const sx_msys* s, std::vector<int>& tw)
       const int nbr = n - nbT * PB - S;
       if (nbr < 1) break;
       const int nbB = (nbr + PB - 1) / PB;
-      // stage 1 (both forward), stage 2 (S both ways), stage 3 (both backward)
-      const int cost = std::max (nbT, nbB) + 2 * D + std::max (nbT, nbB + D);
+      // stage 1 (both forward), stage 2 (S both ways), stage 3 (both
+      // backward; the right-looking sweep of the bottom piece also passes
+      // its D separator blocks, the left-looking one starts below them)
+      const int cost = std::max (nbT, nbB) + 2 * D + std::max (nbT, ll ? nbB : nbB + D);
       if (best < 0 || cost < best) { best = cost; bestT = nbT; }
     }
     if (best < 0 || 4 * best > 3 * 2 * nbtot) return false;
