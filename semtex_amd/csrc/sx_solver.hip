@@ -339,39 +339,13 @@ __host__ __device__ inline TileShape con_shape (int np, bool mf)
 }
 typedef double v4d __attribute__ ((ext_vector_type (4)));
 
-// D += A B over k < K for one 16x16 output tile (v_mfma_f64_16x16x4f64):
-// lane l supplies A[l%16][k + l/16] = M[k + l/16][a0 + l%16] and
-// B[k + l/16][l%16] = X[k + l/16][e0 + l%16]; D[4r + l/16][l%16] is
-// component r of lane l
-// The operands of 8 k-steps are read from LDS before their products, which
-// alternate between two accumulators: the LDS latency and the MFMA latency
-// overlap instead of adding up step by step.
+// v_mfma_f64_16x16x4f64, D += A B for one 16x16 output tile: lane l
+// supplies A[l%16][k + l/16] and B[k + l/16][l%16]; D[4r + l/16][l%16] is
+// component r of lane l.  The products are formed in chunks of MF_U k-steps
+// whose operands are read before their products, alternating between two
+// accumulators (mfma_strip), so the operand latency and the MFMA latency
+// overlap.
 constexpr int MF_U = 8;
-__device__ __forceinline__ v4d mfma_tile (const double* __restrict__ M, int NA,
-                                          const double* __restrict__ X, int K, int lane)
-{
-  v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-  const double* A = M + (lane >> 4) * NA + (lane & 15);
-  const double* B = X + (lane >> 4) * CG_EP + (lane & 15);
-  int k = 0;
-  for (; k + 4 * MF_U <= K; k += 4 * MF_U) {
-    double a[MF_U], b[MF_U];
-#pragma unroll
-    for (int u = 0; u < MF_U; u++) {
-      a[u] = A[(size_t) (k + 4 * u) * NA];
-      b[u] = B[(size_t) (k + 4 * u) * CG_EP];
-    }
-#pragma unroll
-    for (int u = 0; u < MF_U; u += 2) {
-      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[u], b[u], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64 (a[u + 1], b[u + 1], acc1, 0, 0, 0);
-    }
-  }
-  for (; k < K; k += 4)
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64 (A[(size_t) k * NA], B[(size_t) k * CG_EP], acc0,
-                                                 0, 0, 0);
-  return acc0 + acc1;
-}
 
 // dst(i) <- src(i) for i < n with SU loads in flight per thread (the loads
 // of a batch are issued before any of them is waited for)
@@ -556,7 +530,6 @@ __global__ __launch_bounds__(64) void k_weak_ess3 (
 
 // f (row-major; already weak on essential elements, else -M f is formed
 // here) -> [F_b | F_i] in emap order with F_b = w_b - Bm w_i.
-template <bool MF>
 __device__ __forceinline__ void k_condense_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
@@ -565,7 +538,7 @@ __device__ __forceinline__ void k_condense_t_body (
 {
   extern __shared__ double sm[];
   const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
-  const TileShape Cs = con_shape (np, MF);
+  const TileShape Cs = con_shape (np, false);
   const int NE = Cs.NA, WR = next + Cs.KR;           // Ws rows (the k range padded)
   const int mode = blockIdx.y;
   int k0;
@@ -592,38 +565,20 @@ __device__ __forceinline__ void k_condense_t_body (
          },
          [&] (int i, double v) { Ws[(i % npnp) * CG_EP + i / npnp] = v; });
   __syncthreads ();
-  if constexpr (MF) {
-    // 16x16 tiles of F_b (a x element-plane), a wavefront each in turn
-    const int wave = tid >> 6, lane = tid & 63, nat = NE / 16, ntile = nat * ((nep + 15) / 16);
-    for (int t = wave; t < ntile; t += CT_THREADS / 64) {
-      const int a0 = 16 * (t % nat), e0 = 16 * (t / nat);
-      const v4d acc = mfma_tile (As + a0, NE, Ws + (size_t) next * CG_EP + e0, Cs.KR, lane);
-      const int ep = e0 + (lane & 15);
+  const int nag = NE / 4, neg = (nep + 1) / 2;
+  for (int task = tid; task < nag * neg; task += CT_THREADS) {
+    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
+    double acc[4][2] = {};
+    tile_4x2 (As + a0, NE, Ws + (size_t) next * CG_EP + e0, nint, acc);
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int ep = e0 + j;
       if (ep >= nep) continue;
       const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
       double* out = F + (size_t) k * stride + (size_t) e * npnp;
 #pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int a = a0 + 4 * r + (lane >> 4);
-        if (a < next) out[a] = Ws[a * CG_EP + ep] - acc[r];
-      }
-    }
-  } else {
-    const int nag = NE / 4, neg = (nep + 1) / 2;
-    for (int task = tid; task < nag * neg; task += CT_THREADS) {
-      const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
-      double acc[4][2] = {};
-      tile_4x2 (As + a0, NE, Ws + (size_t) next * CG_EP + e0, nint, acc);
-#pragma unroll
-      for (int j = 0; j < 2; j++) {
-        const int ep = e0 + j;
-        if (ep >= nep) continue;
-        const int e = elist[t0 + ep / npm], k = k0 + ep % npm;
-        double* out = F + (size_t) k * stride + (size_t) e * npnp;
-#pragma unroll
-        for (int r = 0; r < 4; r++)
-          if (a0 + r < next) out[a0 + r] = Ws[(a0 + r) * CG_EP + ep] - acc[r][j];
-      }
+      for (int r = 0; r < 4; r++)
+        if (a0 + r < next) out[a0 + r] = Ws[(a0 + r) * CG_EP + ep] - acc[r][j];
     }
   }
   for (int i = tid; i < nep * nint; i += CT_THREADS) {
@@ -632,21 +587,19 @@ __device__ __forceinline__ void k_condense_t_body (
   }
 }
 
-template <bool MF>
 __global__ __launch_bounds__(CT_THREADS) void k_condense_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ emask,
     const double* __restrict__ geo, const double* __restrict__ Mcon, double* F, int stride,
     int nz, int base_mode, int weak)
 {
-  k_condense_t_body<MF> (np, nclass, tiles, elist, emap_g, emask, geo, Mcon, F, stride, nz, base_mode, weak);
+  k_condense_t_body (np, nclass, tiles, elist, emap_g, emask, geo, Mcon, F, stride, nz, base_mode, weak);
 }
 
 // u_i = Hinv^T F_i - Bm^T u_b with u_b from the band solution, then the
 // essential values (Element::global2localSC src/element.cpp:368-407,
 // Field::setEssential src/field.cpp:1105-1124).  One operand
 // M[k][a] = (k < next ? -Bm[k][a] : Hinv[k-next][a]) against [u_b | F_i].
-template <bool MF>
 __device__ __forceinline__ void k_recover_t_body (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -658,7 +611,7 @@ __device__ __forceinline__ void k_recover_t_body (
 {
   extern __shared__ double sm[];
   const int npnp = np * np, next = 4 * (np - 1), nint = npnp - next;
-  const TileShape R = rec_shape (np, MF);
+  const TileShape R = rec_shape (np, false);
   const int NI = R.NA;
   const int mode = blockIdx.y;
   int k0;
@@ -704,51 +657,25 @@ __device__ __forceinline__ void k_recover_t_body (
          },
          [&] (int i, double v) { Xs[(next + i % nint) * CG_EP + i / nint] = v; });
   __syncthreads ();
-  if constexpr (MF) {
-    // 16x16 tiles of u_i (a x element-plane), at most 4 per wavefront
-    // (NI <= 112, nep <= 32), held until every wavefront has read Xs
-    const int wave = tid >> 6, lane = tid & 63, nat = NI / 16, ntile = nat * ((nep + 15) / 16);
-    v4d acc[4];
+  const int nag = NI / 4, neg = (nep + 1) / 2, ntask = nag * neg;
+  // at most two tasks per thread (nint <= 100, nep <= 32): results are
+  // held in registers until every thread has finished reading Xs
+  double acc[2][4][2] = {};
+  for (int s = 0; s < 2; s++) {
+    const int task = tid + s * CT_THREADS;
+    if (task < ntask)
+      tile_4x2 (Ms + 4 * (task % nag), NI, Xs + 2 * (task / nag), npnp, acc[s]);
+  }
+  __syncthreads ();
+  for (int s = 0; s < 2; s++) {
+    const int task = tid + s * CT_THREADS;
+    if (task >= ntask) continue;
+    const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int t = wave + 4 * s;
-      acc[s] = v4d {0.0, 0.0, 0.0, 0.0};
-      if (t < ntile)
-        acc[s] = mfma_tile (Ms + 16 * (t % nat), NI, Xs + 16 * (t / nat), R.KR, lane);
-    }
-    __syncthreads ();
+    for (int r = 0; r < 4; r++)
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-      const int t = wave + 4 * s;
-      if (t >= ntile) continue;
-      const int a0 = 16 * (t % nat), ep = 16 * (t / nat) + (lane & 15);
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int a = a0 + 4 * r + (lane >> 4);
-        if (a < nint && ep < nep) Xs[(next + a) * CG_EP + ep] = acc[s][r];
-      }
-    }
-  } else {
-    const int nag = NI / 4, neg = (nep + 1) / 2, ntask = nag * neg;
-    // at most two tasks per thread (nint <= 100, nep <= 32): results are
-    // held in registers until every thread has finished reading Xs
-    double acc[2][4][2] = {};
-    for (int s = 0; s < 2; s++) {
-      const int task = tid + s * CT_THREADS;
-      if (task < ntask)
-        tile_4x2 (Ms + 4 * (task % nag), NI, Xs + 2 * (task / nag), npnp, acc[s]);
-    }
-    __syncthreads ();
-    for (int s = 0; s < 2; s++) {
-      const int task = tid + s * CT_THREADS;
-      if (task >= ntask) continue;
-      const int a0 = 4 * (task % nag), e0 = 2 * (task / nag);
-#pragma unroll
-      for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          if (a0 + r < nint && e0 + j < nep) Xs[(next + a0 + r) * CG_EP + e0 + j] = acc[s][r][j];
-    }
+      for (int j = 0; j < 2; j++)
+        if (a0 + r < nint && e0 + j < nep) Xs[(next + a0 + r) * CG_EP + e0 + j] = acc[s][r][j];
   }
 #pragma unroll
   for (int u = 0; u < BO; u++) {
@@ -763,7 +690,6 @@ __device__ __forceinline__ void k_recover_t_body (
   }
 }
 
-template <bool MF>
 __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int np, int nclass, const int* __restrict__ tiles, const int* __restrict__ elist,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
@@ -773,7 +699,7 @@ __global__ __launch_bounds__(CT_THREADS) void k_recover_t (
     int stride, const int* __restrict__ nsolve, const double* __restrict__ rhs, int nmax,
     int nz, int base_mode, int rnr, int roff)
 {
-  k_recover_t_body<MF> (np, nclass, tiles, elist, emap_g, btog, bmask, emask, Mrec, G, nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
+  k_recover_t_body (np, nclass, tiles, elist, emap_g, btog, bmask, emask, Mrec, G, nglobal, F, U, stride, nsolve, rhs, nmax, nz, base_mode, rnr, roff);
 }
 
 // ---------------------------------------------------------------------------
@@ -1764,11 +1690,6 @@ size_t reg_lds (const sx_mesh* m, bool recover)
   const TileShape Cs = con_shape (m->np, true);
   return sizeof (double) * (size_t) (m->next + Cs.KR) * CG_EP + sizeof (int) * ETILE;
 }
-bool elem_lds ()
-{
-  static const int v = [] { const char* e = std::getenv ("SX_ELEM_LDS"); return e ? std::atoi (e) : 0; } ();
-  return v != 0;
-}
 
 template <int NPM>
 int launch_condense_r (const sx_msys* s, int nz, int stride, double* F, int weak, hipStream_t st)
@@ -1858,15 +1779,15 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
                         s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
                         m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
   const bool mf = s->tile_mf;
-  if (mf && !elem_lds ()) {
+  if (mf) {
     switch (reg_npm (m->np)) {
     case 8: return launch_condense_r<8> (s, nz, stride, F, weak, st);
     case 10: return launch_condense_r<10> (s, nz, stride, F, weak, st);
     default: return launch_condense_r<12> (s, nz, stride, F, weak, st);
     }
   }
-  const size_t lds = tiled_lds (m, false, mf);
-  auto kern = mf ? k_condense_t<true> : k_condense_t<false>;
+  const size_t lds = tiled_lds (m, false, false);
+  auto kern = k_condense_t;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS),
@@ -1890,15 +1811,15 @@ int recover (const sx_msys* s, int nz, int stride, const double* F, double* U, c
     return SX_OK;
   }
   const bool mf = s->tile_mf;
-  if (mf && !elem_lds ()) {
+  if (mf) {
     switch (reg_npm (m->np)) {
     case 8: return launch_recover_r<8> (s, nz, stride, F, U, G, rnr, roff, st);
     case 10: return launch_recover_r<10> (s, nz, stride, F, U, G, rnr, roff, st);
     default: return launch_recover_r<12> (s, nz, stride, F, U, G, rnr, roff, st);
     }
   }
-  const size_t lds = tiled_lds (m, true, mf);
-  auto kern = mf ? k_recover_t<true> : k_recover_t<false>;
+  const size_t lds = tiled_lds (m, true, false);
+  auto kern = k_recover_t;
   SX_HIP (hipFuncSetAttribute ((const void*) kern,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds));
   hipLaunchKernelGGL (kern, dim3 (m->ntile, n_mode_blocks (nz)), dim3 (CT_THREADS), lds,
@@ -1937,7 +1858,7 @@ int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, doubl
   }
   // a twisted system's bottom piece addressed by the assembly and the
   // recovery themselves (no gather and scatter launches around the sweeps)
-  const bool fold = s->tw && s->sw_panel && tiled_ok (m) && s->tile_mf && !elem_lds ();
+  const bool fold = s->tw && s->sw_panel && tiled_ok (m) && s->tile_mf;
   {
     PhaseScope ps (2, st, timed);
     if (fold)
@@ -2072,13 +1993,13 @@ struct FnAssemble {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
   { k_assemble_body (a...); }
 };
-template <bool MF> struct FnCondenseT {
+struct FnCondenseT {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
-  { k_condense_t_body<MF> (a...); }
+  { k_condense_t_body (a...); }
 };
-template <bool MF> struct FnRecoverT {
+struct FnRecoverT {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
-  { k_recover_t_body<MF> (a...); }
+  { k_recover_t_body (a...); }
 };
 template <int NPM> struct FnCondenseR {
   template <typename... A> __device__ __forceinline__ void operator() (A... a) const
@@ -2130,21 +2051,17 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
         (const int*) m->d_emap, (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride,
         nz, s->base_mode, weak);
   };
-  if (s->tile_mf && !elem_lds ()) {
+  if (s->tile_mf) {
     if (npmr == 8) con_r (FnCondenseR<8> {});
     else if (npmr == 10) con_r (FnCondenseR<10> {});
     else con_r (FnCondenseR<12> {});
-  } else if (s->tile_mf)
-    batch_rec<CT_THREADS, FnCondenseT<true>> (
-        P, tg, CT_THREADS, tiled_lds (m, false, true), m->np, m->nclass, (const int*) m->d_tile,
-        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_emask,
-        (const double*) m->d_geo, (const double*) s->d_Mcon, d_f, stride, nz, s->base_mode, weak);
-  else
-    batch_rec<CT_THREADS, FnCondenseT<false>> (
+  } else {
+    batch_rec<CT_THREADS, FnCondenseT> (
         P, tg, CT_THREADS, tiled_lds (m, false, false), m->np, m->nclass,
         (const int*) m->d_tile, (const int*) m->d_elist, (const int*) m->d_emap,
         (const int*) m->d_emask, (const double*) m->d_geo, (const double*) s->d_Mcon, d_f,
         stride, nz, s->base_mode, weak);
+  }
   P.phase++;
   const int rnr = 2, roff = 0;
   batch_rec<256, FnAssemble> (P, dim3 (gb, npl), 256, 0, m->nglobal, (const int*) m->d_gptr,
@@ -2163,26 +2080,19 @@ int plan_field_solve_direct (BatchPlan& P, const sx_msys* s, int nz, int stride,
         (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
         roff, (const int*) nullptr, (const double*) nullptr, 0, 0);
   };
-  if (s->tile_mf && !elem_lds ()) {
+  if (s->tile_mf) {
     if (npmr == 8) rec_r (FnRecoverR<8> {});
     else if (npmr == 10) rec_r (FnRecoverR<10> {});
     else rec_r (FnRecoverR<12> {});
-  } else if (s->tile_mf)
-    batch_rec<CT_THREADS, FnRecoverT<true>> (
-        P, tg, CT_THREADS, tiled_lds (m, true, true), m->np, m->nclass, (const int*) m->d_tile,
-        (const int*) m->d_elist, (const int*) m->d_emap, (const int*) m->d_btog,
-        (const int*) m->d_bmask, (const int*) m->d_emask, (const double*) s->d_Mrec,
-        (const double*) s->d_G, m->nglobal, (const double*) d_f, d_u, stride,
-        (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz, s->base_mode, rnr,
-        roff);
-  else
-    batch_rec<CT_THREADS, FnRecoverT<false>> (
+  } else {
+    batch_rec<CT_THREADS, FnRecoverT> (
         P, tg, CT_THREADS, tiled_lds (m, true, false), m->np, m->nclass,
         (const int*) m->d_tile, (const int*) m->d_elist, (const int*) m->d_emap,
         (const int*) m->d_btog, (const int*) m->d_bmask, (const int*) m->d_emask,
         (const double*) s->d_Mrec, (const double*) s->d_G, m->nglobal, (const double*) d_f,
         d_u, stride, (const int*) s->d_nsolve, (const double*) s->d_rhs, s->nmax, nz,
         s->base_mode, rnr, roff);
+  }
   P.phase++;
   return SX_OK;
 }
@@ -2236,7 +2146,7 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
   const int lstride = m->nbound * m->np;
   const size_t gstride = (size_t) 2 * s->nmodes * m->nglobal;
   // the register-operand element kernels take up to three fields per launch
-  const bool grouped = nf <= 3 && tiled_ok (m) && s->tile_mf && !elem_lds ();
+  const bool grouped = nf <= 3 && tiled_ok (m) && s->tile_mf;
   const int npmr = reg_npm (m->np);
   if (grouped) {
     FieldSet fs {};
