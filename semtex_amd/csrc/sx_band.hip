@@ -1072,6 +1072,25 @@ __device__ __forceinline__ void k_unit_sweeps_body (
   const size_t PS = (size_t) (D + 1) * PB * PB;     // doubles per panel
   double2 ua[NP], ub[NP];
   double ba[NR], bb[NR];
+  // the separator stage of a twisted solve (top piece rows [m, m+S)) adds
+  // the bottom piece's residual of S, which its forward sweep left in its
+  // last S rows (reversed), before the sweeps, and hands x_S back to those
+  // rows after them: this chain's columns only, so the workgroup's own
+  // barriers order it
+  double* sb = nullptr;
+  int sm0 = 0, snb = 0;
+  if (stage == 2) {
+    const int* t = tw + TW * g.mode;
+    sm0 = t[TW_M];
+    snb = t[TW_NBOT];
+    sb = rhsb + ((size_t) g.mode * G + g.grp) * NR * nmaxb;
+    for (int i = threadIdx.x; i < NR * PB * D; i += UW * 64) {
+      const int r = i / (PB * D), q = i % (PB * D);
+      bg[(size_t) r * rs + sm0 + q] += sb[(size_t) r * nmaxb + snb - 1 - q];
+    }
+    __threadfence_block ();
+    __syncthreads ();
+  }
 
   for (int pass = 0; pass < 2; pass++) {
     if (LL && pass == 1) {
@@ -1247,6 +1266,14 @@ __device__ __forceinline__ void k_unit_sweeps_body (
     // and invalidate the XCD's L2 under the other workgroups' panel streams
     __threadfence_block ();
     __syncthreads ();
+  }
+  if (stage == 2) {
+    __threadfence_block ();
+    __syncthreads ();
+    for (int i = threadIdx.x; i < NR * PB * D; i += UW * 64) {
+      const int r = i / (PB * D), q = i % (PB * D);
+      sb[(size_t) r * nmaxb + snb - 1 - q] = bg[(size_t) r * rs + sm0 + q];
+    }
   }
 }
 
@@ -2298,8 +2325,13 @@ size_t band_sweep_bytes_per_mode (int nmax, int nband)
 // false when some mode is too small for it to shorten the chain of block
 // steps by a quarter.  A sweep step is bound by how many loads one CU
 // keeps in flight, so the second piece pays off while the modes leave CUs
-// idle: systems of at most 32 modes.  SX_BAND_TWIST=0/1 in the environment
-// forces it off/on.
+// idle: systems of at most 32 modes.  (At 64 modes of one field, the DNS
+// pressure, the twisted sweeps run 176 us against 245 us single-piece, but
+// over 100 channel steps the pressure, whose mode 0 is pure Neumann with
+// condition ~2.6e5, then drifted from the oracle to 1.06 times its parity
+// bar, against 0.95 single-piece, whose elimination order is the oracle's;
+// the grouped velocity, 6 right-hand sides per mode, runs 301 us twisted
+// against 271 us.)  SX_BAND_TWIST=0/1 in the environment forces it off/on.
 static bool twist_split (const sx_msys* s, std::vector<int>& tw)
 {
   if (const char* e = std::getenv ("SX_BAND_TWIST")) {
@@ -2591,8 +2623,9 @@ int band_solve (const sx_msys* s, int nr, hipStream_t st, int moves)
     // recovery reads the bottom piece's solution)
     int rc;
     if ((moves & 1) && (rc = move (0))) return rc;
-    if (sper) {
-      // the split sweeps fold the separator's combine and x_S copy into stage 2
+    if (sper || s->sw_unit) {
+      // the split and the unit-diagonal sweeps fold the separator's combine
+      // and x_S copy into stage 2
       if ((rc = sweep (1)) || (rc = sweep (2)) || (rc = sweep (3))) return rc;
     } else if ((rc = sweep (1)) || (rc = move (1)) || (rc = sweep (2)) || (rc = move (2)) ||
                (rc = sweep (3))) {
@@ -2735,7 +2768,13 @@ int plan_band_solve (BatchPlan& P, const sx_msys* s, int nr)
   for (int stage = 1; stage <= 3; stage++) {
     if ((rc = sweep (stage))) return rc;
     P.phase++;
-    if (stage < 3) move (stage);
+    // the unit-diagonal sweeps fold the separator's combine and x_S copy
+    // into stage 2; their phases stay aligned with the other systems'
+    // (the fused launches of one phase serve every system of the batch)
+    if (stage < 3) {
+      if (s->sw_unit) P.phase++;
+      else move (stage);
+    }
   }
   move (3);
   return SX_OK;
