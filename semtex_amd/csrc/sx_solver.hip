@@ -7,7 +7,8 @@
 //   k_band_assemble deterministic direct-stiffness assembly of the band
 //   band_factor     FP64 banded Cholesky + inverted diagonal blocks (sx_band.hip)
 // Solve (the timed path, per call):
-//   k_essential     essential BC values into global numbering
+//   k_essential     essential BC values into global numbering (k_ess_weak:
+//                   with the essential elements' -(M f + H g) in one launch)
 //   k_condense      -(M f + H g), emap gather, F_b -= hbi hii^-1 F_i
 //   k_assemble      direct-stiffness sum of F_b (+ natural BC integrals)
 //   band_solve      FP64 U^T U x = b, one workgroup per mode (sx_band.hip)
@@ -512,20 +513,57 @@ __global__ __launch_bounds__(64) void k_weak_ess (
   k_weak_ess_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, ess_elem, G, nglobal, F, stride, nz, base_mode);
 }
 
-// the same for up to three fields (blockIdx.z selects; G and F per field)
-struct FieldGF {
-  const double* G[3];
+// boundary lines, essential values and field of up to three fields
+// (blockIdx.z selects)
+struct FieldSet {
+  const double* lines[3];
+  double* G[3];
   double* F[3];
 };
-__global__ __launch_bounds__(64) void k_weak_ess3 (
+
+// k_essential and k_weak_ess in one launch, the weak form reading the
+// essential values from the boundary lines itself (the value k_essential
+// writes to G: lines[ess[g]], 0 when the node takes none): blocks
+// [0, ness) form -(M f + H g) of one essential element, the rest fill G,
+// 256 global nodes each.  blockIdx.y: plane, blockIdx.z: field.
+__global__ __launch_bounds__(64) void k_ess_weak (
     int np, int cyl, double lambda2, const double* __restrict__ betak2,
     const double* __restrict__ geo, const double* __restrict__ Dg,
     const int* __restrict__ emap_g, const int* __restrict__ btog,
-    const int* __restrict__ ess_elem, FieldGF gf, int nglobal, int stride, int nz,
-    int base_mode)
+    const int* __restrict__ ess_elem, int ness, const int* __restrict__ ess, FieldSet fs,
+    int lstride, int nglobal, int stride, int nz, int base_mode)
 {
-  k_weak_ess_body (np, cyl, lambda2, betak2, geo, Dg, emap_g, btog, ess_elem, gf.G[blockIdx.z],
-                   nglobal, gf.F[blockIdx.z], stride, nz, base_mode);
+  const int k = solved_plane (blockIdx.y, nz, base_mode), lane = threadIdx.x;
+  const double* lines = fs.lines[blockIdx.z];
+  if ((int) blockIdx.x >= ness) {
+    double* G = fs.G[blockIdx.z] + (size_t) k * nglobal;
+    const int g0 = ((int) blockIdx.x - ness) * 256;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int g = g0 + 64 * q + lane;
+      if (g < nglobal) {
+        const int sl = ess[g];
+        G[g] = (sl >= 0 && lines) ? lines[(size_t) k * lstride + sl] : 0.0;
+      }
+    }
+    return;
+  }
+  const int npnp = np * np, next = 4 * (np - 1);
+  const int e = ess_elem[blockIdx.x];
+  __shared__ double D[256], g[256], R[256], S[256], Hg[256];
+  __shared__ int emap[256];
+  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; emap[i] = emap_g[i]; g[i] = 0.0; }
+  __syncthreads ();
+  const int* bt = btog + (size_t) e * next;
+  for (int i = lane; i < next; i += 64) {
+    const int sl = ess[bt[i]];
+    g[emap[i]] = (sl >= 0 && lines) ? lines[(size_t) k * lstride + sl] : 0.0;
+  }
+  __syncthreads ();
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  helm_op_wave (np, D, ge, lambda2, betak2[k >> 1], cyl, g, R, S, Hg, lane);
+  double* f = fs.F[blockIdx.z] + (size_t) k * stride + (size_t) e * npnp;
+  for (int i = lane; i < npnp; i += 64) f[i] = -(f[i] * ge[G_Q8 * npnp + i] + Hg[i]);
 }
 
 // f (row-major; already weak on essential elements, else -M f is formed
@@ -1287,11 +1325,6 @@ __global__ void k_assemble_tw (int nglobal, const int* __restrict__ gptr,
 
 // The grouped solve's per-field kernels for up to three fields in one
 // launch (blockIdx.z selects the field; its right-hand sides are 2z, 2z+1)
-struct FieldSet {
-  const double* lines[3];
-  double* G[3];
-  double* F[3];
-};
 __global__ void k_essential3 (int nglobal, const int* __restrict__ ess, FieldSet fs,
                               int lstride, int nz, int base_mode)
 {
@@ -1757,8 +1790,9 @@ int launch_recover_group (const sx_msys* s, int nf, int nz, int stride, const Fi
   return SX_OK;
 }
 
+// ess_done: k_ess_weak has formed the essential elements' weak form
 int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, hipStream_t st,
-              int weak = 0)
+              int weak = 0, bool ess_done = false)
 {
   const sx_mesh* m = s->m;
   const int npl = n_solved_planes (nz, s->base_mode);
@@ -1774,7 +1808,7 @@ int condense (const sx_msys* s, int nz, int stride, double* F, const double* G, 
     SX_HIP (hipGetLastError ());
     return SX_OK;
   }
-  if (m->ness && !weak)
+  if (m->ness && !weak && !ess_done)
     hipLaunchKernelGGL (k_weak_ess, dim3 (m->ness, npl), dim3 (64), 0, st, m->np, m->cyl,
                         s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
                         m->d_ess_elem, G, m->nglobal, F, stride, nz, s->base_mode);
@@ -1847,14 +1881,28 @@ int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, doubl
   const int npl = n_solved_planes (nz, s->base_mode);
   const int lstride = m->nbound * m->np;
   const double* lines = (d_lines && m->nbound) ? d_lines : nullptr;
+  // the essential values and the essential elements' weak form in one
+  // launch where the tiled condensation follows
+  const bool ew = tiled_ok (m) && m->ness;
   {
     PhaseScope ps (0, st, timed);
-    hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
-                        st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+    if (ew) {
+      FieldSet fs {};
+      fs.lines[0] = lines;
+      fs.G[0] = s->d_G;
+      fs.F[0] = d_f;
+      hipLaunchKernelGGL (k_ess_weak, dim3 (m->ness + (m->nglobal + 255) / 256, npl), dim3 (64),
+                          0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
+                          m->d_emap, m->d_btog, m->d_ess_elem, m->ness, m->d_ess, fs, lstride,
+                          m->nglobal, stride, nz, s->base_mode);
+    } else {
+      hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
+                          st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
+    }
   }
   {
     PhaseScope ps (1, st, timed);
-    if ((rc = condense (s, nz, stride, d_f, s->d_G, st))) return rc;
+    if ((rc = condense (s, nz, stride, d_f, s->d_G, st, 0, ew))) return rc;
   }
   // a twisted system's bottom piece addressed by the assembly and the
   // recovery themselves (no gather and scatter launches around the sweeps)
@@ -2150,22 +2198,21 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
   const int npmr = reg_npm (m->np);
   if (grouped) {
     FieldSet fs {};
-    FieldGF gf {};
     FieldPtrs F {};
     for (int f = 0; f < nf; f++) {
       fs.lines[f] = (d_lines && d_lines[f] && m->nbound) ? d_lines[f] : nullptr;
       fs.G[f] = s->d_G + f * gstride;
       fs.F[f] = d_f[f];
-      gf.G[f] = fs.G[f];
-      gf.F[f] = d_f[f];
       F.p[f] = d_f[f];
     }
-    hipLaunchKernelGGL (k_essential3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256), 0,
-                        st, m->nglobal, m->d_ess, fs, lstride, nz, s->base_mode);
     if (m->ness)
-      hipLaunchKernelGGL (k_weak_ess3, dim3 (m->ness, npl, nf), dim3 (64), 0, st, m->np, m->cyl,
-                          s->lambda2, s->d_betak2, m->d_geo, m->d_D, m->d_emap, m->d_btog,
-                          m->d_ess_elem, gf, m->nglobal, stride, nz, s->base_mode);
+      hipLaunchKernelGGL (k_ess_weak, dim3 (m->ness + (m->nglobal + 255) / 256, npl, nf),
+                          dim3 (64), 0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo,
+                          m->d_D, m->d_emap, m->d_btog, m->d_ess_elem, m->ness, m->d_ess, fs,
+                          lstride, m->nglobal, stride, nz, s->base_mode);
+    else
+      hipLaunchKernelGGL (k_essential3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256),
+                          0, st, m->nglobal, m->d_ess, fs, lstride, nz, s->base_mode);
     const int rc = npmr == 8 ? launch_condense_group<8> (s, nf, nz, stride, F, st)
                  : npmr == 10 ? launch_condense_group<10> (s, nf, nz, stride, F, st)
                               : launch_condense_group<12> (s, nf, nz, stride, F, st);
