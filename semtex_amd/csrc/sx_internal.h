@@ -147,6 +147,7 @@ struct sx_mesh {
   int* d_tile = nullptr;       // [ntile][3] class, first index into d_elist, count
   int* d_elist = nullptr;      // [nel] element numbers, class-sorted
   int* d_ess_elem = nullptr;   // [ness] elements with essential nodes (emask)
+  int* d_ess_slot = nullptr;   // [ness][next] their slots' line index (d_ess) / -1
   // per-tile tables of the register-operand tile kernels, read by block
   // index with no prior load: [ntile][tile_ints (next)], per tile the class,
   // the element count, 2 unused, the ETILE element numbers (the last one

@@ -220,7 +220,7 @@ extern "C" int sx_mesh_destroy (sx_mesh_t m)
                   m->d_emask, m->d_emap, m->d_gptr, m->d_gsrc, m->d_ess,
                   m->d_nptr, m->d_nidx, m->d_nw, m->d_ppos, m->d_pmn,
                   m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem,
-                  m->d_ttab};
+                  m->d_ttab, m->d_ess_slot};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete m;
   return SX_OK;
@@ -432,6 +432,11 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   }
   m->ness = (int) ess_elem.size ();
   if (ess_elem.empty ()) ess_elem.push_back (0);
+  // the line slot of every boundary slot of the essential elements (-1: none)
+  std::vector<int> ess_slot ((size_t) ess_elem.size () * m->next, -1);
+  for (int i = 0; i < m->ness; i++)
+    for (int a = 0; a < m->next; a++)
+      ess_slot[(size_t) i * m->next + a] = ess[m->btog[(size_t) ess_elem[i] * m->next + a]];
 
   int rc = SX_OK;
   std::vector<int> cls (m->cls);
@@ -445,7 +450,8 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       (rc = upload (&m->d_ppos, ppos)) || (rc = upload (&m->d_pmn, pmn)) ||
       (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc)) ||
       (rc = upload (&m->d_tile, tile)) || (rc = upload (&m->d_elist, elist)) ||
-      (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_ttab, ttab))) {
+      (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_ttab, ttab)) ||
+      (rc = upload (&m->d_ess_slot, ess_slot))) {
     sx_mesh_destroy (m);
     return rc;
   }

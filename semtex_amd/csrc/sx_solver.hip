@@ -523,14 +523,15 @@ struct FieldSet {
 
 // k_essential and k_weak_ess in one launch, the weak form reading the
 // essential values from the boundary lines itself (the value k_essential
-// writes to G: lines[ess[g]], 0 when the node takes none): blocks
+// writes to G: lines[ess[g]], 0 when the node takes none; ess_slot holds
+// ess[g] for the essential elements' boundary slots): blocks
 // [0, ness) form -(M f + H g) of one essential element, the rest fill G,
 // 256 global nodes each.  blockIdx.y: plane, blockIdx.z: field.
 __global__ __launch_bounds__(64) void k_ess_weak (
     int np, int cyl, double lambda2, const double* __restrict__ betak2,
     const double* __restrict__ geo, const double* __restrict__ Dg,
-    const int* __restrict__ emap_g, const int* __restrict__ btog,
-    const int* __restrict__ ess_elem, int ness, const int* __restrict__ ess, FieldSet fs,
+    const int* __restrict__ emap_g, const int* __restrict__ ess_elem,
+    const int* __restrict__ ess_slot, int ness, const int* __restrict__ ess, FieldSet fs,
     int lstride, int nglobal, int stride, int nz, int base_mode)
 {
   const int k = solved_plane (blockIdx.y, nz, base_mode), lane = threadIdx.x;
@@ -554,9 +555,9 @@ __global__ __launch_bounds__(64) void k_ess_weak (
   __shared__ int emap[256];
   for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; emap[i] = emap_g[i]; g[i] = 0.0; }
   __syncthreads ();
-  const int* bt = btog + (size_t) e * next;
+  const int* es = ess_slot + (size_t) blockIdx.x * next;
   for (int i = lane; i < next; i += 64) {
-    const int sl = ess[bt[i]];
+    const int sl = es[i];
     g[emap[i]] = (sl >= 0 && lines) ? lines[(size_t) k * lstride + sl] : 0.0;
   }
   __syncthreads ();
@@ -1893,8 +1894,8 @@ int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, doubl
       fs.F[0] = d_f;
       hipLaunchKernelGGL (k_ess_weak, dim3 (m->ness + (m->nglobal + 255) / 256, npl), dim3 (64),
                           0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
-                          m->d_emap, m->d_btog, m->d_ess_elem, m->ness, m->d_ess, fs, lstride,
-                          m->nglobal, stride, nz, s->base_mode);
+                          m->d_emap, m->d_ess_elem, m->d_ess_slot, m->ness, m->d_ess, fs,
+                          lstride, m->nglobal, stride, nz, s->base_mode);
     } else {
       hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                           st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
@@ -2208,8 +2209,8 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
     if (m->ness)
       hipLaunchKernelGGL (k_ess_weak, dim3 (m->ness + (m->nglobal + 255) / 256, npl, nf),
                           dim3 (64), 0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo,
-                          m->d_D, m->d_emap, m->d_btog, m->d_ess_elem, m->ness, m->d_ess, fs,
-                          lstride, m->nglobal, stride, nz, s->base_mode);
+                          m->d_D, m->d_emap, m->d_ess_elem, m->d_ess_slot, m->ness, m->d_ess,
+                          fs, lstride, m->nglobal, stride, nz, s->base_mode);
     else
       hipLaunchKernelGGL (k_essential3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256),
                           0, st, m->nglobal, m->d_ess, fs, lstride, nz, s->base_mode);
