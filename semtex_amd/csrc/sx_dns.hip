@@ -12,11 +12,12 @@
 // Kernels:
 //   k_adv_xy     x and y terms of the nonlinear term of one component per
 //                element-plane (products, Element::grad contractions)
-//   k_div_h      setPForce: div H / dt per element-plane
+//   k_wave_div   waveProp and setPForce: H of every component and div H / dt
+//                per element and plane pair
 //   k_project3   project: F_i = H_i alpha + beta d_i p, all components
 //   k_grad       x / y derivative per element (Element::grad)
 //   k_gradz      Fourier z derivative (AuxField::gradient(2))
-//   k_ew_*       element-wise field algebra (waveProp, the unfused z terms)
+//   k_ew_*       element-wise field algebra (the unfused z terms)
 //   k_smooth_*   mass-averaged direct stiffness sum on element boundaries
 //   k_hopbc      -d(u.n)/dt + n.N - nu n.curlcurl u on boundary segments
 //   k_plines     extrapolated dp/dn line data for the pressure solve
@@ -148,54 +149,6 @@ __device__ __forceinline__ double dz_at (const double* f, int k, int stride, siz
                  : -bk * f[(size_t) (k + 1) * stride + node];
 }
 
-// setPForce (dns/integrate.cpp:242-257): out = div H / dt, per
-// element-plane of the Fourier field H (x, y by Element::grad, z as
-// AuxField::gradient(2)); H2 null for two components
-__global__ __launch_bounds__(64) void k_div_h (int np, const double* __restrict__ geo,
-                                               const double* __restrict__ Dg,
-                                               const double* __restrict__ H0,
-                                               const double* __restrict__ H1,
-                                               const double* __restrict__ H2, double* out,
-                                               int stride, double beta, double dt)
-{
-  const int npnp = np * np, e = blockIdx.x, k = blockIdx.y, lane = threadIdx.x;
-  __shared__ double D[256], a[256], b[256];
-  const size_t eo = (size_t) e * npnp, off = (size_t) k * stride + eo;
-  const double* ge = geo + (size_t) e * NGEO * npnp;
-  double rx[4], sx[4], ry[4], sy[4], gz[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++) {       // np <= 16: at most 4 nodes per lane
-    const int i = lane + 64 * t;
-    if (i < npnp) {
-      D[i] = Dg[i];
-      a[i] = H0[off + i];
-      b[i] = H1[off + i];
-      rx[t] = ge[G_RX * npnp + i];
-      sx[t] = ge[G_SX * npnp + i];
-      ry[t] = ge[G_RY * npnp + i];
-      sy[t] = ge[G_SY * npnp + i];
-      gz[t] = H2 ? dz_at (H2, k, stride, eo + i, beta) : 0.0;
-    }
-  }
-  __syncthreads ();
-#pragma unroll
-  for (int t = 0; t < 4; t++) {
-    const int ij = lane + 64 * t;
-    if (ij >= npnp) break;
-    const int i = ij / np, j = ij % np;
-    double ar = 0.0, as = 0.0, br = 0.0, bs = 0.0;
-    for (int q = 0; q < np; q++) {
-      const double dj = D[j * np + q], di = D[i * np + q];
-      ar += a[i * np + q] * dj;
-      as += di * a[q * np + j];
-      br += b[i * np + q] * dj;
-      bs += di * b[q * np + j];
-    }
-    double sum = (ar * rx[t] + as * sx[t]) + (br * ry[t] + bs * sy[t]);
-    if (H2) sum += gz[t];
-    out[off + ij] = sum / dt;
-  }
-}
 
 // project (dns/integrate.cpp:260-289) for every component at once:
 // F_i = H_i alpha + beta d_i p, per element-plane (x, y by Element::grad,
@@ -328,24 +281,89 @@ struct Wave3 {
   Levels us[3], uf[3];
   double* unyq[3];
 };
-__global__ void k_ew_wave (Wave3 wv, int Je, size_t n, size_t S)
+// waveProp and setPForce in one pass (dns/integrate.cpp:204-257): one
+// workgroup per element and plane pair (2y, 2y+1), a wave per plane
+// (threadIdx.y; np <= 16, so a lane holds at most 4 nodes).  Each node's
+// H_c = sum_q us.c[q] Us[q] + uf.c[q] Uf[q] (summed in the reference's
+// order) is formed and written; the divergence out = (d_x H0 + d_y H1 +
+// d_z H2) / dt (x, y by Element::grad, z as AuxField::gradient(2)) is then
+// formed from the values in LDS, the z derivative from the pair's other
+// plane.  The unsolved Nyquist plane 1 of Us[0] is carried into unyq, and
+// the element-plane padding of H (S > nel np^2) is written too.
+__global__ __launch_bounds__(128) void k_wave_div (Wave3 wv, int Je, int nc, int nz, int np,
+                                                   int nel, const double* __restrict__ geo,
+                                                   const double* __restrict__ Dg, double* out,
+                                                   int S, double beta, double dt)
 {
-  const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double* H = wv.H[blockIdx.y];
-  const Levels& us = wv.us[blockIdx.y];
-  const Levels& uf = wv.uf[blockIdx.y];
-  double* unyq = wv.unyq[blockIdx.y];
-  // unyq (when given): the Nyquist plane 1 of us0 carried into u, which the
-  // solves leave alone
-  if (unyq && i >= S && i < 2 * S) unyq[i] = us.f[0][i];
-  double h = 0.0;
-  for (int q = 0; q < Je; q++) {
-    h += us.c[q] * us.f[q][i];
-    h += uf.c[q] * uf.f[q][i];
+  const int npnp = np * np, e = blockIdx.x, pl = threadIdx.y, lane = threadIdx.x;
+  const int k = 2 * blockIdx.y + pl;
+  const bool kin = k < nz;
+  __shared__ double D[256], hs[3][2][256];           // H0, H1, H2 of both planes
+  const size_t eo = (size_t) e * npnp, off = (size_t) k * S + eo;
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  auto wave_at = [&] (int c, size_t i) __attribute__ ((always_inline)) {
+    const Levels& us = wv.us[c];
+    const Levels& uf = wv.uf[c];
+    double h = 0.0;
+    for (int q = 0; q < Je; q++) {
+      h += us.c[q] * us.f[q][i];
+      h += uf.c[q] * uf.f[q][i];
+    }
+    return h;
+  };
+  double rx[4], sx[4], ry[4], sy[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int i = lane + 64 * t;
+    if (i < npnp) {
+      if (pl == 0) D[i] = Dg[i];
+      rx[t] = ge[G_RX * npnp + i];
+      sx[t] = ge[G_SX * npnp + i];
+      ry[t] = ge[G_RY * npnp + i];
+      sy[t] = ge[G_SY * npnp + i];
+      for (int c = 0; c < nc; c++) {
+        double h = 0.0;
+        if (kin) {
+          h = wave_at (c, off + i);
+          wv.H[c][off + i] = h;
+          if (wv.unyq[c] && k == 1) wv.unyq[c][off + i] = wv.us[c].f[0][off + i];
+        }
+        hs[c][pl][i] = h;
+      }
+    }
   }
-  H[i] = h;
+  if (kin && e == 0 && lane == 0)
+    for (size_t i = (size_t) k * S + (size_t) nel * npnp; i < (size_t) (k + 1) * S; i++)
+      for (int c = 0; c < nc; c++) {
+        wv.H[c][i] = wave_at (c, i);
+        if (wv.unyq[c] && k == 1) wv.unyq[c][i] = wv.us[c].f[0][i];
+      }
+  __syncthreads ();
+  if (!kin) return;
+  const double* a = hs[0][pl];
+  const double* b = hs[1][pl];
+  const int m = k >> 1;
+  const double bk = beta * m;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij >= npnp) break;
+    const int i = ij / np, j = ij % np;
+    double ar = 0.0, as = 0.0, br = 0.0, bs = 0.0;
+    for (int q = 0; q < np; q++) {
+      const double dj = D[j * np + q], di = D[i * np + q];
+      ar += a[i * np + q] * dj;
+      as += di * a[q * np + j];
+      br += b[i * np + q] * dj;
+      bs += di * b[q * np + j];
+    }
+    double sum = (ar * rx[t] + as * sx[t]) + (br * ry[t] + bs * sy[t]);
+    if (nc == 3)
+      sum += m < 1 ? 0.0 : ((k & 1) ? bk * hs[2][0][ij] : -bk * hs[2][1][ij]);
+    out[off + ij] = sum / dt;
+  }
 }
+
 // setPForce (dns/integrate.cpp:242-257): f = (g0 + g1 [+ g2]) / dt
 // Field::smooth: the average imass[g] sum Q4 f over the element-boundary
 // slots of global node g, written back to every one of those slots.  The
@@ -863,15 +881,13 @@ int dns_step (sx_dns* d, hipStream_t st)
         wv.uf[i].c[q] = EXc[Je][q] * d->dt;
       }
     }
-    hipLaunchKernelGGL (k_ew_wave, dim3 (nb256 (d->n), nc), dim3 (256), 0, st, wv, Je, d->n,
-                        (size_t) d->S);
+    // -- and setPForce: div u^ / dt, in the same pass
+    hipLaunchKernelGGL (k_wave_div, dim3 (d->mu->nel, std::max (1, d->nz / 2)), dim3 (64, 2), 0,
+                        st, wv, Je, nc, d->nz, d->mu->np, d->mu->nel, d->mu->d_geo, d->mu->d_D,
+                        d->tmp, d->S, d->beta, d->dt);
   }
   roll (d->Uf, nord, nc);
-  // -- setPForce: div u^ / dt
   const sx_mesh* m = d->mu;
-  hipLaunchKernelGGL (k_div_h, dim3 (m->nel, d->nz), dim3 (64), 0, st, m->np, m->d_geo, m->d_D,
-                      d->H[0], d->H[1], nc == 3 ? d->H[2] : nullptr, d->tmp, d->S, d->beta,
-                      d->dt);
   SX_HIP (hipGetLastError ());
   // -- pressure
   if ((rc = sx_field_solve (d->Mp, d->nz, d->S, d->tmp, d->p, plines, st))) return rc;
