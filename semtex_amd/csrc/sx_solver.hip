@@ -1326,12 +1326,6 @@ __global__ void k_assemble_tw (int nglobal, const int* __restrict__ gptr,
 
 // The grouped solve's per-field kernels for up to three fields in one
 // launch (blockIdx.z selects the field; its right-hand sides are 2z, 2z+1)
-__global__ void k_essential3 (int nglobal, const int* __restrict__ ess, FieldSet fs,
-                              int lstride, int nz, int base_mode)
-{
-  k_essential_body (nglobal, ess, fs.lines[blockIdx.z], lstride, fs.G[blockIdx.z], nz,
-                    base_mode);
-}
 __global__ void k_assemble3 (int nglobal, const int* __restrict__ gptr,
                              const int* __restrict__ gsrc, const int* __restrict__ nptr,
                              const int* __restrict__ nidx, const double* __restrict__ nw,
@@ -1896,7 +1890,8 @@ int field_solve_direct (const sx_msys* s, int nz, int stride, double* d_f, doubl
                           0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo, m->d_D,
                           m->d_emap, m->d_ess_elem, m->d_ess_slot, m->ness, m->d_ess, fs,
                           lstride, m->nglobal, stride, nz, s->base_mode);
-    } else {
+    } else if (m->ness) {
+      // (with no element holding an essential node no kernel reads G)
       hipLaunchKernelGGL (k_essential, dim3 ((m->nglobal + 255) / 256, npl), dim3 (256), 0,
                           st, m->nglobal, m->d_ess, lines, lstride, s->d_G, nz, s->base_mode);
     }
@@ -2211,9 +2206,7 @@ int field_solve_group (sx_msys* s, int nf, int nz, int stride, double* const* d_
                           dim3 (64), 0, st, m->np, m->cyl, s->lambda2, s->d_betak2, m->d_geo,
                           m->d_D, m->d_emap, m->d_ess_elem, m->d_ess_slot, m->ness, m->d_ess,
                           fs, lstride, m->nglobal, stride, nz, s->base_mode);
-    else
-      hipLaunchKernelGGL (k_essential3, dim3 ((m->nglobal + 255) / 256, npl, nf), dim3 (256),
-                          0, st, m->nglobal, m->d_ess, fs, lstride, nz, s->base_mode);
+    // (with no element holding an essential node no kernel reads G)
     const int rc = npmr == 8 ? launch_condense_group<8> (s, nf, nz, stride, F, st)
                  : npmr == 10 ? launch_condense_group<10> (s, nf, nz, stride, F, st)
                               : launch_condense_group<12> (s, nf, nz, stride, F, st);
