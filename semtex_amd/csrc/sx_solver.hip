@@ -549,22 +549,73 @@ __global__ __launch_bounds__(64) void k_ess_weak (
     }
     return;
   }
+  // -(M f + H g) as helm_op_wave forms H g (the same expressions), with
+  // every load issued up front: the operator, the boundary slot's line
+  // value (next <= 44 < 64 on this path, np <= 12: one slot a lane), and
+  // the metric factors, mass weights and field of the lane's nodes (at
+  // most 3 a lane)
   const int npnp = np * np, next = 4 * (np - 1);
   const int e = ess_elem[blockIdx.x];
-  __shared__ double D[256], g[256], R[256], S[256], Hg[256];
-  __shared__ int emap[256];
-  for (int i = lane; i < npnp; i += 64) { D[i] = Dg[i]; emap[i] = emap_g[i]; g[i] = 0.0; }
-  __syncthreads ();
-  const int* es = ess_slot + (size_t) blockIdx.x * next;
-  for (int i = lane; i < next; i += 64) {
-    const int sl = es[i];
-    g[emap[i]] = (sl >= 0 && lines) ? lines[(size_t) k * lstride + sl] : 0.0;
+  __shared__ double D[256], g[256], R[256], S[256];
+  const double* ge = geo + (size_t) e * NGEO * npnp;
+  double* f = fs.F[blockIdx.z] + (size_t) k * stride + (size_t) e * npnp;
+  const double bk2 = betak2[k >> 1];
+  int ei = 0;
+  double gv = 0.0;
+  if (lane < next) {
+    const int sl = ess_slot[(size_t) blockIdx.x * next + lane];
+    ei = emap_g[lane];
+    gv = (sl >= 0 && lines) ? lines[(size_t) k * lstride + sl] : 0.0;
+  }
+  double q1[4], q2[4], q3[4], q4[4], q8[4], fv[4], hc[4], o[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij < npnp) {
+      D[ij] = Dg[ij];
+      g[ij] = 0.0;
+      q1[t] = ge[G_Q1 * npnp + ij];
+      q2[t] = ge[G_Q2 * npnp + ij];
+      q3[t] = ge[G_Q3 * npnp + ij];
+      q4[t] = ge[G_Q4 * npnp + ij];
+      q8[t] = ge[G_Q8 * npnp + ij];
+      fv[t] = f[ij];
+      if (cyl) {
+        const double y = ge[G_Y * npnp + ij], r2 = y * y;
+        hc[t] = (r2 > EPSDP) ? bk2 / r2 + lambda2 : 0.0;
+      } else hc[t] = bk2 + lambda2;
+    }
   }
   __syncthreads ();
-  const double* ge = geo + (size_t) e * NGEO * npnp;
-  helm_op_wave (np, D, ge, lambda2, betak2[k >> 1], cyl, g, R, S, Hg, lane);
-  double* f = fs.F[blockIdx.z] + (size_t) k * stride + (size_t) e * npnp;
-  for (int i = lane; i < npnp; i += 64) f[i] = -(f[i] * ge[G_Q8 * npnp + i] + Hg[i]);
+  if (lane < next) g[ei] = gv;
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij < npnp) {
+      const int i = ij / np, j = ij % np;
+      double r = 0.0, s = 0.0;
+      for (int q = 0; q < np; q++) {
+        r += g[i * np + q] * D[j * np + q];
+        s += D[i * np + q] * g[q * np + j];
+      }
+      R[ij] = q1[t] * r + q3[t] * s;
+      S[ij] = q2[t] * s + q3[t] * r;
+      o[t] = q4[t] * g[ij] * hc[t];
+    }
+  }
+  __syncthreads ();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    const int ij = lane + 64 * t;
+    if (ij < npnp) {
+      const int i = ij / np, j = ij % np;
+      double tt = 0.0;
+      for (int q = 0; q < np; q++) tt += D[q * np + i] * S[q * np + j] + R[i * np + q] * D[q * np + j];
+      o[t] += tt;
+      f[ij] = -(fv[t] * q8[t] + o[t]);
+    }
+  }
 }
 
 // f (row-major; already weak on essential elements, else -M f is formed
