@@ -369,26 +369,37 @@ __global__ __launch_bounds__(128) void k_wave_div (Wave3 wv, int Je, int nc, int
 // slots of global node g, written back to every one of those slots.  The
 // slot lists of the global nodes are disjoint, so each thread reads and
 // writes only its own slots (one pass, no intermediate global array).
-__global__ void k_smooth (int npnp, int nglobal, const int* __restrict__ gptr,
-                          const int* __restrict__ gsrc, const int* __restrict__ emap,
-                          const double* __restrict__ geo, const double* __restrict__ imass,
-                          Comp3 cmp, int S)
+__global__ void k_smooth (int nglobal, const int* __restrict__ gptr,
+                          const int* __restrict__ gpos, const double* __restrict__ gq4,
+                          const double* __restrict__ imass, Comp3 cmp, int S)
 {
   double* f = cmp.out[blockIdx.z];
   const int g = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
   if (g >= nglobal) return;
   double* fk = f + (size_t) k * S;
   const int c0 = gptr[g], c1 = gptr[g + 1];
+  // the first four slots' offsets and weights, then their values, each in
+  // one round (a node shared by more elements takes the rest one by one);
+  // summed in slot order
+  int p[4];
+  double w[4], v[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    p[u] = c0 + u < c1 ? gpos[c0 + u] : 0;
+    w[u] = c0 + u < c1 ? gq4[c0 + u] : 0.0;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) v[u] = c0 + u < c1 ? fk[p[u]] : 0.0;
   double s = 0.0;
-  for (int c = c0; c < c1; c++) {
-    const int slot = gsrc[c], e = slot / npnp, ij = emap[slot % npnp];
-    s += geo[((size_t) e * NGEO + G_Q4) * npnp + ij] * fk[(size_t) e * npnp + ij];
-  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (c0 + u < c1) s += w[u] * v[u];
+  for (int c = c0 + 4; c < c1; c++) s += gq4[c] * fk[gpos[c]];
   s *= imass[g];
-  for (int c = c0; c < c1; c++) {
-    const int slot = gsrc[c];
-    fk[(size_t) (slot / npnp) * npnp + emap[slot % npnp]] = s;
-  }
+#pragma unroll
+  for (int u = 0; u < 4; u++)
+    if (c0 + u < c1) fk[p[u]] = s;
+  for (int c = c0 + 4; c < c1; c++) fk[gpos[c]] = s;
 }
 
 // Computed pressure BC terms on boundary segment b, Fourier mode m
@@ -632,8 +643,7 @@ int smooth_n (sx_dns* d, double* const* f, int nf, hipStream_t st)
   Comp3 c {};
   for (int i = 0; i < nf; i++) c.out[i] = f[i];
   hipLaunchKernelGGL (k_smooth, dim3 ((m->nglobal + 255) / 256, d->nz, nf), dim3 (256), 0, st,
-                      m->npnp, m->nglobal, m->d_gptr, m->d_gsrc, m->d_emap, m->d_geo, d->imass,
-                      c, d->S);
+                      m->nglobal, m->d_gptr, m->d_gpos, m->d_gq4, d->imass, c, d->S);
   SX_HIP (hipGetLastError ());
   return SX_OK;
 }

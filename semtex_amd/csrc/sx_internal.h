@@ -133,6 +133,8 @@ struct sx_mesh {
   int* d_emap = nullptr;       // [npnp]
   int* d_gptr = nullptr;       // [nglobal+1] CSR: global node -> plane slots
   int* d_gsrc = nullptr;       //   slot = e*npnp + i (emap-ordered storage)
+  int* d_gpos = nullptr;       //   the slot's row-major plane offset e*npnp + emap[i]
+  double* d_gq4 = nullptr;     //   and its Q4 weight
   int* d_ess = nullptr;        // [nglobal] line index of essential value / -1
   int* d_nptr = nullptr;       // [nglobal+1] CSR natural-BC contributions
   int* d_nidx = nullptr;       //   line index

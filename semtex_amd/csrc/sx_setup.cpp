@@ -220,7 +220,7 @@ extern "C" int sx_mesh_destroy (sx_mesh_t m)
                   m->d_emask, m->d_emap, m->d_gptr, m->d_gsrc, m->d_ess,
                   m->d_nptr, m->d_nidx, m->d_nw, m->d_ppos, m->d_pmn,
                   m->d_pptr, m->d_psrc, m->d_tile, m->d_elist, m->d_ess_elem,
-                  m->d_ttab, m->d_ess_slot};
+                  m->d_ttab, m->d_ess_slot, m->d_gpos, m->d_gq4};
   for (void* p : ptrs) if (p) (void) hipFree (p);
   delete m;
   return SX_OK;
@@ -432,6 +432,15 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
   }
   m->ness = (int) ess_elem.size ();
   if (ess_elem.empty ()) ess_elem.push_back (0);
+  // every slot of the CSR lists as its row-major plane offset and its Q4
+  // weight (the smoothing's operands without the index chain)
+  std::vector<int> gpos (std::max<size_t> (gsrc.size (), 1), 0);
+  std::vector<double> gq4 (gpos.size (), 0.0);
+  for (size_t c = 0; c < gsrc.size (); c++) {
+    const int slot = gsrc[c], e = slot / npnp, ij = m->emap[slot % npnp];
+    gpos[c] = e * npnp + ij;
+    gq4[c] = geo[((size_t) e * NGEO + G_Q4) * npnp + ij];
+  }
   // the line slot of every boundary slot of the essential elements (-1: none)
   std::vector<int> ess_slot ((size_t) ess_elem.size () * m->next, -1);
   for (int i = 0; i < m->ness; i++)
@@ -451,7 +460,8 @@ extern "C" int sx_mesh_create (int np, int nel, int cyl, const double* x,
       (rc = upload (&m->d_pptr, pptr)) || (rc = upload (&m->d_psrc, psrc)) ||
       (rc = upload (&m->d_tile, tile)) || (rc = upload (&m->d_elist, elist)) ||
       (rc = upload (&m->d_ess_elem, ess_elem)) || (rc = upload (&m->d_ttab, ttab)) ||
-      (rc = upload (&m->d_ess_slot, ess_slot))) {
+      (rc = upload (&m->d_ess_slot, ess_slot)) || (rc = upload (&m->d_gpos, gpos)) ||
+      (rc = upload (&m->d_gq4, gq4))) {
     sx_mesh_destroy (m);
     return rc;
   }
